@@ -1,0 +1,110 @@
+/*
+ * odpx.h -- extensions of libodp_hip beyond the reference's API.  None of these
+ * symbols exists in the reference; they never change the behaviour of the
+ * ODP_* entry points declared in odp.h.
+ *
+ *  - seeded synthetic graphs and the 2-opt swap (host code);
+ *  - plan control, autotuning and kernel timing of the ODP_* evaluator;
+ *  - device-resident evaluation (the adjacency already in HBM);
+ *  - batched evaluation of many graphs of uneven size in one persistent launch;
+ *  - a serving engine with continuous batching of evaluation requests.
+ * Every evaluation returns the same integers as ODP_Set_aspl would.
+ */
+#ifndef ODP_HIP_ODPX_H
+#define ODP_HIP_ODPX_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- seeded graphs (odp_gen.c) ---------------------------------------- */
+void     ODPX_Rng_seed(uint64_t seed, uint64_t state[4]);
+uint64_t ODPX_Rng_next(uint64_t state[4]);
+/* random simple connected degree-regular graph -> adjacency int[nodes][degree]; 0 on success */
+int ODPX_Generate_regular(int nodes, int degree, uint64_t seed, int *adjacency);
+/* rotationally symmetric grid graph (s in {1,2,4}) with edge length <= length -> full table int[w*h][degree] */
+int ODPX_Generate_grid(int width, int height, int degree, int length, int symmetries,
+                       uint64_t seed, int iters_per_edge, int *adjacency);
+/* first-quadrant / left-half rows of a symmetric grid table, as ODP_Init_aspl_grid_s takes them */
+int ODPX_Grid_base_table(int width, int height, int degree, int symmetries, const int *full, int *base);
+/* circulant-symmetric general graph -> base table int[nodes/symmetries][degree] */
+int ODPX_Generate_general_s(int nodes, int degree, int symmetries, uint64_t seed, int *base);
+/* replace edges a-b, c-d by a-c, b-d in a full table */
+int ODPX_Swap_edges(int nodes, int degree, int *adjacency, int a, int b, int c, int d);
+/* one random valid 2-opt move; swap[4] receives {a, b, c, d} */
+int ODPX_Two_opt(int nodes, int degree, int *adjacency, uint64_t rng[4], int swap[4]);
+
+/* ---- evaluator plans -------------------------------------------------- */
+#define ODPX_PLAN_AUTO     0
+#define ODPX_PLAN_RESIDENT 1   /* LDS-resident blocks, one persistent launch      */
+#define ODPX_PLAN_LEVEL    2   /* one launch per BFS level over HBM bit buffers   */
+
+typedef struct {
+  int plan;       /* ODPX_PLAN_RESIDENT or ODPX_PLAN_LEVEL                          */
+  int wb;         /* resident: 32-bit words per row of a block (1, 2, 4, 8 or 16)    */
+  int threads;    /* resident: threads per workgroup (256, 512 or 1024)              */
+  int kw;         /* level: 64-bit words per row of a column block; 0 = whole row    */
+  int upw;        /* level: row segments per wave per launch                         */
+  int seg_major;  /* level: 1 = all rows of one column segment before the next        */
+  int xcd;        /* level: 1 = each XCD takes one contiguous range of workgroups     */
+} ODPX_Plan;
+
+/* Plan of the current evaluator (after Init).  Set: 0 on success, -1 if invalid.
+ * Without a call, Init picks one (env ODP_HIP_PLAN=resident|level overrides). */
+int  ODPX_Get_plan(ODPX_Plan *plan);
+int  ODPX_Set_plan(const ODPX_Plan *plan);
+/* Times every candidate plan on `adjacency` (same layout as ODP_Set_aspl),
+ * checks that all of them return identical results (exit(1) otherwise), and
+ * keeps the fastest.  Returns the number of candidates tried. */
+int  ODPX_Autotune(void *adjacency, int repeats);
+
+/* ---- device-resident evaluation ---------------------------------------- */
+/* Uploads adjacency to the evaluator's device copy without evaluating. */
+void ODPX_Upload_adjacency(void *adjacency);
+/* Evaluates the adjacency already on the device (after ODP_Set_aspl or
+ * ODPX_Upload_adjacency, and any ODPX_Swap_device since). */
+void ODPX_Set_aspl_resident(int *diameter, long *sum, double *ASPL);
+/* Applies edge swap a-b, c-d -> a-c, b-d to the device copy (s == 1 only);
+ * 0 on success, -1 if an edge is missing. */
+int  ODPX_Swap_device(int a, int b, int c, int d);
+
+/* ---- kernel timing ------------------------------------------------------ */
+typedef struct {
+  long long evals;            /* evaluations since reset                           */
+  double    eval_ms;          /* wall time of those evaluations (host clock)       */
+  long long level_launches;   /* gather-level launches that did work               */
+  double    level_ms;         /* their summed duration, HIP events on the launch stream */
+  double    level_bytes;      /* their algorithmic bytes: rows*((d+2)*k*8 + 4d)     */
+  long long idle_launches;    /* gather-level launches after every block finished  */
+  double    idle_ms;
+  long long resident_launches;
+  double    resident_ms;
+  double    resident_bytes;   /* same formula over the levels the blocks ran       */
+  long long other_launches;   /* prepare + level-0 kernels                         */
+  double    other_ms;
+} ODPX_Stats;
+void ODPX_Stats_enable(int on);
+void ODPX_Stats_reset(void);
+void ODPX_Stats_get(ODPX_Stats *out);
+
+/* ---- batched evaluation ------------------------------------------------- */
+/* Evaluates ngraphs general (s = 1) graphs.  Graph g has nodes[g] vertices,
+ * table width degrees[g], and its adjacency int[nodes[g]][degrees[g]] starts
+ * at packed_adjacency + offsets[g]; -1 entries are absent edges.  Results as
+ * ODP_Set_aspl.  Graphs that fit the LDS-resident kernel run in one persistent
+ * launch; larger ones run the level kernels.  Returns 0. */
+int ODPX_Batch_eval(int ngraphs, const int *nodes, const int *degrees, const long long *offsets,
+                    const int *packed_adjacency, int *diameter, long *sum, double *ASPL);
+/* Same graphs, evaluated one after another with the single-graph level path
+ * (the reference's pattern of one evaluation per call); the comparison point
+ * for the batched path.  plan: ODPX_PLAN_LEVEL or ODPX_PLAN_AUTO. */
+int ODPX_Loop_eval(int ngraphs, const int *nodes, const int *degrees, const long long *offsets,
+                   const int *packed_adjacency, int plan, int *diameter, long *sum, double *ASPL);
+/* Frees the device buffers the batched and looped paths keep between calls. */
+void ODPX_Batch_release(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1002 @@
+// aspl_hip.hip -- host side of libodp_hip: the ODP_* lifecycle (the drop-in
+// for aspl_cuda.cu), plan selection and autotuning, kernel timing, and the
+// batched / looped evaluation paths.  Kernels live in kernels.hpp.
+//
+// Exact arithmetic (aspl.c:47-74, 81-118, 332-336): every block of l sources
+// contributes sum_L (n*l - count_L) * s over its non-terminal levels, the
+// diameter is max over blocks of (terminal level + 1), the ordered-pair sum
+// starts at n*(n-1), ASPL = sum / ((n-1)*n) in double and the returned sum is
+// (long)(sum / 2.0).  A block whose count stops growing before it reaches n*l
+// marks the graph disconnected: diameter n+1, hence INT_MAX / LONG_MAX /
+// DBL_MAX, exactly what the reference reaches after n levels.
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <limits.h>
+#include <float.h>
+#include <math.h>
+#include <time.h>
+#include <unistd.h>
+#include <sys/time.h>
+#include <vector>
+#include <algorithm>
+#include "odp.h"
+#include "odpx.h"
+#include "kernels.hpp"
+
+#define ODP_ERROR(...) do { fprintf(stderr, __VA_ARGS__); exit(1); } while (0)
+#define HIPC(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) \
+  ODP_ERROR("libodp_hip: %s failed: %s (%s:%d)\n", #x, hipGetErrorString(e_), __FILE__, __LINE__); } while (0)
+
+static const int KIND_MATRIX = 1, KIND_SAVING = 2;
+static const int GPU_CHUNK_WORDS = 64;          // column-block width of MATRIX_SAVING (parameter.h:13)
+static const double MEM_THRESHOLD_MB = 2048.0;  // MATRIX -> MATRIX_SAVING switch (parameter.h:4)
+static const size_t LDS_LIMIT = 163840;         // LDS a gfx950 workgroup may allocate
+static const size_t LDS_STATIC = 512;           // k_resident's own __shared__ data, rounded up
+static const size_t LDS_DYN_MAX = LDS_LIMIT - LDS_STATIC;
+static const int RING = 260;                    // per-level counter slots kept on the device
+static const int MAX_BATCH_LEVELS = 256;        // levels launched between two host reads (< RING - 2)
+
+static double wall_time()
+{
+  struct timeval t;
+  gettimeofday(&t, NULL);
+  return t.tv_sec + 1.0e-6 * t.tv_usec;
+}
+
+static inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
+static inline int ilog2(int x) { int r = 0; while ((1 << (r + 1)) <= x) r++; return r; }
+
+// ---------------------------------------------------------------------------
+// device context: one stream (default priority) for everything
+
+static struct { bool ready; hipStream_t stream; int ncu; } C;
+
+static void ctx_init()
+{
+  if (C.ready) return;
+  int dev;
+  HIPC(hipGetDevice(&dev));
+  hipDeviceProp_t prop;
+  HIPC(hipGetDeviceProperties(&prop, dev));
+  C.ncu = prop.multiProcessorCount;
+  HIPC(hipStreamCreateWithFlags(&C.stream, hipStreamNonBlocking));
+  HIPC(hipFuncSetAttribute((const void*)k_resident, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_DYN_MAX));
+  C.ready = true;
+}
+
+template <typename T> static void dalloc(T** p, size_t count)
+{
+  *p = nullptr;
+  if (count) HIPC(hipMalloc((void**)p, count * sizeof(T)));
+}
+template <typename T> static void dfree(T** p) { if (*p) HIPC(hipFree(*p)); *p = nullptr; }
+template <typename T> static void halloc(T** p, size_t count)
+{
+  *p = nullptr;
+  if (count) HIPC(hipHostMalloc((void**)p, count * sizeof(T), hipHostMallocDefault));
+}
+template <typename T> static void hfree(T** p) { if (*p) HIPC(hipHostFree(*p)); *p = nullptr; }
+
+// ---------------------------------------------------------------------------
+// kernel timing: HIP events recorded on the launch stream around launches
+
+enum { T_OTHER = 0, T_LEVEL = 1, T_RESIDENT = 2 };
+struct TimedLaunch { hipEvent_t a, b; int kind, level; };
+static bool g_stats_on = false;
+static ODPX_Stats g_stats;
+static std::vector<hipEvent_t> g_evpool;
+static std::vector<TimedLaunch> g_pending;
+
+static hipEvent_t ev_get()
+{
+  if (!g_evpool.empty()) { hipEvent_t e = g_evpool.back(); g_evpool.pop_back(); return e; }
+  hipEvent_t e;
+  HIPC(hipEventCreate(&e));
+  return e;
+}
+static void timed_begin(TimedLaunch& t, int kind, int level)
+{
+  t.kind = kind; t.level = level;
+  if (!g_stats_on) return;
+  t.a = ev_get(); t.b = ev_get();
+  HIPC(hipEventRecord(t.a, C.stream));
+}
+static void timed_end(TimedLaunch& t)
+{
+  if (!g_stats_on) return;
+  HIPC(hipEventRecord(t.b, C.stream));
+  g_pending.push_back(t);
+}
+
+// ---------------------------------------------------------------------------
+// one graph on the device, the row-neighbour table and its plan
+
+struct LevelRun {
+  int n = 0, ntasks = 0, kw = 0, kwp = 0, upw = 4, seg_major = 1, xcd = 0, urow = 0, total_wg = 0;
+  size_t buf_words = 0;                 // capacity of bufs (64-bit words)
+  size_t task_cap = 0;
+  u64* bufs = nullptr; u64* cnt = nullptr; u32* done = nullptr;
+  LevelTask* tasks = nullptr; int* wg_start = nullptr;
+  u64* h_cnt = nullptr;
+  std::vector<int> l;                   // sources per task
+};
+
+struct ResRun {
+  int ntasks = 0, wb = 1, threads = 1024, grid = 0;
+  size_t lds = 0, task_cap = 0;
+  ResTask* tasks = nullptr; ResOut* out = nullptr; int* qhead = nullptr;
+  ResOut* h_out = nullptr;
+};
+
+struct Evaluator {
+  // graph
+  int family = 0, n = 0, s = 1, degree = 0, dpad = 0, width = 0, height = 0, grid_s = 0, nsrc = 0;
+  int has_numdeg = 0;
+  double avg_deg = 0;
+  size_t adj_cap = 0, nbr_cap = 0;
+  int *d_adj = nullptr, *d_numdeg = nullptr, *d_itable = nullptr, *d_nbr = nullptr, *d_err = nullptr;
+  int *h_stage = nullptr, *h_err = nullptr;
+  size_t stage_cap = 0;
+  ODPX_Plan plan{};
+  LevelRun lr;
+  ResRun rr;
+  int guess_levels = 0;                 // levels the last level-plan evaluation needed
+
+  size_t adj_elems() const { return (size_t)(n / s) * degree; }
+
+  void release()
+  {
+    dfree(&d_adj); dfree(&d_numdeg); dfree(&d_itable); dfree(&d_nbr); dfree(&d_err);
+    hfree(&h_stage); hfree(&h_err);
+    dfree(&lr.bufs); dfree(&lr.cnt); dfree(&lr.done); dfree(&lr.tasks); dfree(&lr.wg_start); hfree(&lr.h_cnt);
+    dfree(&rr.tasks); dfree(&rr.out); dfree(&rr.qhead); hfree(&rr.h_out);
+    lr = LevelRun(); rr = ResRun();
+    adj_cap = nbr_cap = stage_cap = 0;
+    guess_levels = 0;
+  }
+
+  // shape of the graph; (re)allocates the graph buffers when they grow
+  void shape(int fam, int nodes, int deg, int sym, int w, int h, const int* numdeg_full, int gs)
+  {
+    ctx_init();
+    family = fam; n = nodes; degree = deg; s = sym; width = w; height = h; grid_s = gs;
+    nsrc = n / s;
+    dpad = round_up(deg > 0 ? deg : 1, 4);
+    has_numdeg = numdeg_full != nullptr;
+    if (adj_elems() > adj_cap) { dfree(&d_adj); dalloc(&d_adj, adj_elems()); adj_cap = adj_elems(); }
+    if (adj_elems() > stage_cap) { hfree(&h_stage); halloc(&h_stage, adj_elems()); stage_cap = adj_elems(); }
+    size_t ne = (size_t)n * dpad;
+    if (ne > nbr_cap) { dfree(&d_nbr); dalloc(&d_nbr, ne); nbr_cap = ne; }
+    if (!d_err) { dalloc(&d_err, 1); halloc(&h_err, 1); HIPC(hipMemset(d_err, 0, sizeof(int))); }
+    dfree(&d_numdeg);
+    dfree(&d_itable);
+    avg_deg = deg;
+    if (numdeg_full) {
+      dalloc(&d_numdeg, (size_t)n);
+      HIPC(hipMemcpy(d_numdeg, numdeg_full, sizeof(int) * n, hipMemcpyHostToDevice));
+      double t = 0;
+      for (int i = 0; i < n; i++) t += numdeg_full[i];
+      avg_deg = t / n;
+    }
+  }
+
+  void set_itable(const int* itable)
+  {
+    dalloc(&d_itable, (size_t)n);
+    HIPC(hipMemcpy(d_itable, itable, sizeof(int) * n, hipMemcpyHostToDevice));
+  }
+
+  void upload(const void* adjacency)
+  {
+    memcpy(h_stage, adjacency, sizeof(int) * adj_elems());
+    HIPC(hipMemcpyAsync(d_adj, h_stage, sizeof(int) * adj_elems(), hipMemcpyHostToDevice, C.stream));
+  }
+
+  // ---- plans ----
+  int resident_wb_max() const
+  {
+    for (int wb = 16; wb >= 1; wb >>= 1)
+      if ((size_t)2 * n * wb * 4 <= LDS_DYN_MAX) return wb;
+    return 0;
+  }
+
+  ODPX_Plan default_plan(int kind) const
+  {
+    ODPX_Plan p{};
+    const char* env = getenv("ODP_HIP_PLAN");
+    bool force_level = env && strcmp(env, "level") == 0;
+    bool force_res = env && strcmp(env, "resident") == 0;
+    int wbmax = resident_wb_max();
+    if ((wbmax > 0 && !force_level) || (force_res && wbmax > 0)) {
+      p.plan = ODPX_PLAN_RESIDENT;
+      // widest block that still gives every CU a block, else the narrowest
+      int wb = 1;
+      for (int c = wbmax; c >= 1; c >>= 1)
+        if ((nsrc + 32 * c - 1) / (32 * c) >= C.ncu) { wb = c; break; }
+      p.wb = wb;
+      p.threads = n >= 4096 ? 1024 : 256;
+      return p;
+    }
+    p.plan = ODPX_PLAN_LEVEL;
+    int W = (nsrc + 63) / 64;
+    p.kw = (kind == KIND_MATRIX) ? 0 : std::min(GPU_CHUNK_WORDS, W);
+    p.upw = 4;
+    p.seg_major = grid_s ? 0 : 1;
+    p.xcd = grid_s ? 1 : 0;
+    return p;
+  }
+
+  bool plan_valid(const ODPX_Plan& p) const
+  {
+    if (p.plan == ODPX_PLAN_RESIDENT) {
+      if (p.wb != 1 && p.wb != 2 && p.wb != 4 && p.wb != 8 && p.wb != 16) return false;
+      if (p.threads != 256 && p.threads != 512 && p.threads != 1024) return false;
+      return (size_t)2 * n * p.wb * 4 <= LDS_DYN_MAX;
+    }
+    if (p.plan == ODPX_PLAN_LEVEL) return p.kw >= 0 && p.upw >= 1 && p.upw <= 64;
+    return false;
+  }
+
+  // ---- level plan: buffers and task table ----
+  void setup_level()
+  {
+    LevelRun& r = lr;
+    int W = (nsrc + 63) / 64;
+    int kw = plan.kw > 0 ? std::min(plan.kw, W) : W;
+    int kwp = kw <= 128 ? (1 << (ilog2(std::max(kw, 2) - 1) + 1)) : round_up(kw, 128);
+    if (kw == 2 || kw == 1) kwp = 2;
+    if (kwp < kw) kwp = kw;
+    int ntasks = (W + kw - 1) / kw;
+    int lanes = std::min(64, kwp / 2);
+    int lpr_log2 = ilog2(lanes);
+    int rpw = 64 >> lpr_log2;
+    int segs = kwp / (2 * lanes);
+    int rgroups = (n + rpw - 1) / rpw;
+    int units = rgroups * segs;
+    size_t words = (size_t)ntasks * 2 * n * kwp;
+    if (words > r.buf_words) { dfree(&r.bufs); dalloc(&r.bufs, words); r.buf_words = words; }
+    if ((size_t)ntasks > r.task_cap) {
+      dfree(&r.cnt); dfree(&r.done); dfree(&r.tasks); dfree(&r.wg_start); hfree(&r.h_cnt);
+      dalloc(&r.cnt, (size_t)RING * ntasks); dalloc(&r.done, (size_t)ntasks);
+      dalloc(&r.tasks, (size_t)ntasks); dalloc(&r.wg_start, (size_t)ntasks + 1);
+      halloc(&r.h_cnt, (size_t)RING * ntasks);
+      r.task_cap = ntasks;
+    }
+    // the previous contents are never read: level 0 rewrites every word it uses
+    r.n = n; r.ntasks = ntasks; r.kw = kw; r.kwp = kwp;
+    r.upw = plan.upw; r.seg_major = plan.seg_major; r.xcd = plan.xcd; r.urow = (lpr_log2 == 6);
+    int per_task_wg = (units + 4 * r.upw - 1) / (4 * r.upw);
+    r.total_wg = per_task_wg * ntasks;
+    std::vector<LevelTask> ht(ntasks);
+    std::vector<int> ws(ntasks + 1);
+    r.l.assign(ntasks, 0);
+    for (int t = 0; t < ntasks; t++) {
+      LevelTask& T = ht[t];
+      T.nbr = d_nbr;
+      T.buf0 = r.bufs + (size_t)t * 2 * n * kwp;
+      T.buf1 = T.buf0 + (size_t)n * kwp;
+      T.cnt = r.cnt + t;
+      T.done = r.done + t;
+      T.n = n; T.dpad = dpad; T.kwp = kwp;
+      T.src0 = t * kw * 64;
+      T.l = std::min(kw * 64, nsrc - T.src0);
+      T.lpr_log2 = lpr_log2; T.segs = segs; T.rgroups = rgroups; T.units = units;
+      T.seg_major = r.seg_major; T.cstride = ntasks; T.pad = 0;
+      r.l[t] = T.l;
+      ws[t] = t * per_task_wg;
+    }
+    ws[ntasks] = r.total_wg;
+    HIPC(hipMemcpy(r.tasks, ht.data(), sizeof(LevelTask) * ntasks, hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(r.wg_start, ws.data(), sizeof(int) * (ntasks + 1), hipMemcpyHostToDevice));
+  }
+
+  // ---- resident plan: task table ----
+  void setup_resident()
+  {
+    ResRun& r = rr;
+    r.wb = plan.wb; r.threads = plan.threads;
+    int b = 32 * r.wb;
+    int ntasks = (nsrc + b - 1) / b;
+    if ((size_t)ntasks > r.task_cap) {
+      dfree(&r.tasks);
+      dalloc(&r.tasks, (size_t)ntasks);
+      r.task_cap = ntasks;
+    }
+    if (!r.out) { dalloc(&r.out, 1); dalloc(&r.qhead, 1); halloc(&r.h_out, 1); }
+    std::vector<ResTask> ht(ntasks);
+    for (int t = 0; t < ntasks; t++) {
+      ResTask& T = ht[t];
+      T.nbr = d_nbr; T.n = n; T.dpad = dpad; T.src0 = t * b; T.l = std::min(b, nsrc - t * b);
+      T.s = s; T.graph = 0; T.wb = r.wb; T.pad = 0;
+    }
+    HIPC(hipMemcpy(r.tasks, ht.data(), sizeof(ResTask) * ntasks, hipMemcpyHostToDevice));
+    r.ntasks = ntasks;
+    r.lds = (size_t)2 * n * r.wb * 4;
+    int per_cu = (int)std::max<size_t>(1, LDS_LIMIT / (r.lds + LDS_STATIC));
+    per_cu = std::min(per_cu, 2048 / r.threads);
+    r.grid = std::max(1, std::min(ntasks, per_cu * C.ncu));
+  }
+
+  void apply_plan(const ODPX_Plan& p)
+  {
+    plan = p;
+    if (plan.plan == ODPX_PLAN_LEVEL) setup_level();
+    else setup_resident();
+    guess_levels = 0;
+  }
+
+  // ---- evaluation of the adjacency on the device ----
+  void prepare(u64* z64, int nz64, u32* z32, int nz32)
+  {
+    PrepArgs a;
+    a.adj = d_adj; a.numdeg = d_numdeg; a.itable = d_itable; a.nbr = d_nbr; a.err = d_err;
+    a.n = n; a.s = s; a.degree = degree; a.dpad = dpad; a.width = width; a.height = height; a.grid_s = grid_s;
+    a.zero64 = z64; a.nzero64 = nz64; a.zero32 = z32; a.nzero32 = nz32;
+    int64_t total = (int64_t)n * dpad;
+    int grid = (int)std::min<int64_t>(std::max<int64_t>(1, (total + 255) / 256), 4096);
+    TimedLaunch tl;
+    timed_begin(tl, T_OTHER, -1);
+    hipLaunchKernelGGL(k_prepare, dim3(grid), dim3(256), 0, C.stream, a);
+    HIPC(hipGetLastError());
+    timed_end(tl);
+  }
+
+  void check_err()
+  {
+    if (*h_err) {
+      int e = *h_err;
+      *h_err = 0;
+      HIPC(hipMemset(d_err, 0, sizeof(int)));
+      if (e & 2) ODP_ERROR("num_degrees holds a value outside [0, degree]\n");
+      ODP_ERROR("adjacency holds a vertex id outside [0, nodes) within num_degrees\n");
+    }
+  }
+
+  // returns the ordered-pair excess over n(n-1) and the diameter; disconnected -> *disc
+  void run_level(u64* excess, int* diam, bool* disc)
+  {
+    LevelRun& r = lr;
+    prepare(r.cnt, r.ntasks, r.done, r.ntasks);   // ring slot 0 and the done flags
+    {
+      TimedLaunch tl;
+      timed_begin(tl, T_OTHER, 0);
+      hipLaunchKernelGGL(k_level0, dim3(r.total_wg), dim3(LEVEL_WG), 0, C.stream,
+                         r.tasks, r.wg_start, r.ntasks, RING, r.upw, r.xcd);
+      HIPC(hipGetLastError());
+      timed_end(tl);
+    }
+    struct St { int running; u64 sum, prev; int lwork; };
+    std::vector<St> st(r.ntasks, St{1, 0, 0, 0});
+    int launched = 1, examined = 0, running = r.ntasks;
+    int first = guess_levels > 0 ? guess_levels : 8;
+    int batch_end = std::min(n, std::max(2, first));
+    bool any_disc = false;
+    int max_d = 1;
+    for (;;) {
+      for (int L = launched; L < batch_end; L++) {
+        TimedLaunch tl;
+        timed_begin(tl, T_LEVEL, L);
+        if (r.urow)
+          hipLaunchKernelGGL(k_level<true>, dim3(r.total_wg), dim3(LEVEL_WG), 0, C.stream,
+                             r.tasks, r.wg_start, r.ntasks, L, RING, r.upw, r.xcd);
+        else
+          hipLaunchKernelGGL(k_level<false>, dim3(r.total_wg), dim3(LEVEL_WG), 0, C.stream,
+                             r.tasks, r.wg_start, r.ntasks, L, RING, r.upw, r.xcd);
+        HIPC(hipGetLastError());
+        timed_end(tl);
+      }
+      launched = std::max(launched, batch_end);
+      HIPC(hipMemcpyAsync(r.h_cnt, r.cnt, sizeof(u64) * RING * r.ntasks, hipMemcpyDeviceToHost, C.stream));
+      HIPC(hipMemcpyAsync(h_err, d_err, sizeof(int), hipMemcpyDeviceToHost, C.stream));
+      HIPC(hipStreamSynchronize(C.stream));
+      check_err();
+      for (int t = 0; t < r.ntasks; t++) {
+        St& q = st[t];
+        if (!q.running) continue;
+        const u64 target = (u64)n * (u64)r.l[t];
+        for (int L = examined; L < launched; L++) {
+          const u64 c = r.h_cnt[(size_t)(L % RING) * r.ntasks + t];
+          if (c == target) { q.running = 0; q.lwork = L; max_d = std::max(max_d, L + 1); break; }
+          if ((L >= 1 && c == q.prev) || L + 1 >= n) { q.running = 0; q.lwork = L; any_disc = true; break; }
+          q.sum += (target - c) * (u64)s;
+          q.prev = c;
+        }
+        if (!q.running) running--;
+      }
+      examined = launched;
+      if (running == 0) break;
+      if (launched >= n) { any_disc = true; break; }
+      int step = std::min(MAX_BATCH_LEVELS, std::max(launched, 8));
+      batch_end = std::min(n, launched + step);
+    }
+    int lmax = 0;
+    u64 tot = 0;
+    for (int t = 0; t < r.ntasks; t++) { tot += st[t].sum; lmax = std::max(lmax, st[t].lwork); }
+    guess_levels = lmax + 1;
+    *excess = tot; *diam = max_d; *disc = any_disc;
+    if (g_stats_on) {
+      HIPC(hipStreamSynchronize(C.stream));
+      for (TimedLaunch& tl : g_pending) {
+        float ms = 0;
+        HIPC(hipEventElapsedTime(&ms, tl.a, tl.b));
+        g_evpool.push_back(tl.a); g_evpool.push_back(tl.b);
+        if (tl.kind == T_LEVEL) {
+          double bytes = 0;
+          for (int t = 0; t < r.ntasks; t++)
+            if (tl.level <= st[t].lwork)
+              bytes += (double)n * ((avg_deg + 2.0) * ((r.l[t] + 63) / 64) * 8.0 + 4.0 * avg_deg);
+          if (bytes > 0) { g_stats.level_launches++; g_stats.level_ms += ms; g_stats.level_bytes += bytes; }
+          else { g_stats.idle_launches++; g_stats.idle_ms += ms; }
+        } else {
+          g_stats.other_launches++; g_stats.other_ms += ms;
+        }
+      }
+      g_pending.clear();
+    }
+  }
+
+  void run_resident(u64* excess, int* diam, bool* disc)
+  {
+    ResRun& r = rr;
+    prepare((u64*)r.out, (int)(sizeof(ResOut) / 8), (u32*)r.qhead, 1);
+    TimedLaunch tl;
+    timed_begin(tl, T_RESIDENT, 0);
+    hipLaunchKernelGGL(k_resident, dim3(r.grid), dim3(r.threads), r.lds, C.stream, r.tasks, r.ntasks, r.qhead, r.out);
+    HIPC(hipGetLastError());
+    timed_end(tl);
+    HIPC(hipMemcpyAsync(r.h_out, r.out, sizeof(ResOut), hipMemcpyDeviceToHost, C.stream));
+    HIPC(hipMemcpyAsync(h_err, d_err, sizeof(int), hipMemcpyDeviceToHost, C.stream));
+    HIPC(hipStreamSynchronize(C.stream));
+    check_err();
+    *excess = r.h_out->sum;
+    *diam = std::max(1, r.h_out->diam);
+    *disc = (r.h_out->flags & 1) != 0;
+    if (g_stats_on) {
+      for (TimedLaunch& t : g_pending) {
+        float ms = 0;
+        HIPC(hipEventElapsedTime(&ms, t.a, t.b));
+        g_evpool.push_back(t.a); g_evpool.push_back(t.b);
+        if (t.kind == T_RESIDENT) {
+          int kb = (32 * r.wb + 63) / 64;
+          (void)kb;
+          double row = (avg_deg + 2.0) * (std::min(32 * r.wb, nsrc) / 8.0) + 4.0 * avg_deg;
+          g_stats.resident_launches++; g_stats.resident_ms += ms;
+          g_stats.resident_bytes += (double)r.h_out->levels * n * row;
+        } else {
+          g_stats.other_launches++; g_stats.other_ms += ms;
+        }
+      }
+      g_pending.clear();
+    }
+  }
+
+  // full evaluation of the adjacency on the device, reference arithmetic
+  void evaluate(int* diameter, long* sum, double* ASPL)
+  {
+    double t0 = wall_time();
+    u64 excess = 0;
+    int d = 1;
+    bool disc = false;
+    if (plan.plan == ODPX_PLAN_LEVEL) run_level(&excess, &d, &disc);
+    else run_resident(&excess, &d, &disc);
+    long lsum = (long)n * (n - 1) + (long)excess;
+    if (disc) d = n + 1;
+    *diameter = d;
+    *ASPL = lsum / (((double)n - 1) * n);
+    lsum /= 2.0;
+    *sum = lsum;
+    if (g_stats_on) { g_stats.evals++; g_stats.eval_ms += 1e3 * (wall_time() - t0); }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// reference kind selection and memory estimate (utils.c:1206-1266, GPU branch)
+
+static double ref_mem_mb(int kind, int nodes, int degree, int symmetries, bool numdeg, bool grid_s)
+{
+  double ab;
+  if (kind == KIND_MATRIX) ab = nodes * ((double)nodes / (4.0 * symmetries));
+  else ab = 16.0 * nodes * GPU_CHUNK_WORDS;
+  if (grid_s) ab += 4.0 * nodes;
+  double res = 8.0 * 448, adj = 4.0 * (nodes / symmetries) * degree, deg = numdeg ? 4.0 * nodes : 0;
+  return (ab + res + adj + deg) / (1024.0 * 1024.0);
+}
+
+static int ref_kind(int nodes, int degree, int symmetries, bool numdeg, bool grid_s)
+{
+  const char* val = getenv("ODP_ASPL");
+  if (!val) return ref_mem_mb(KIND_MATRIX, nodes, degree, symmetries, numdeg, grid_s) <= MEM_THRESHOLD_MB
+                   ? KIND_MATRIX : KIND_SAVING;
+  if (strcmp(val, "MATRIX") == 0) return KIND_MATRIX;
+  if (strcmp(val, "MATRIX_SAVING") == 0) return KIND_SAVING;
+  if (strcmp(val, "BFS") == 0) ODP_ERROR("BFS for HIP is not implemented.\n");
+  ODP_ERROR("Unknown ASPL value (%s)\n", val);
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// the ODP_* lifecycle (aspl_cuda.cu:128-263)
+
+static struct {
+  bool active;
+  int kind;
+  Evaluator ev;
+  bool profile;
+  double elapsed;
+  unsigned times;
+} E;
+
+static int rotate_host(int v, int width, int height, int s, int angle)
+{
+  int w = v / height, h = v % height;
+  if (s == 2) return (width - w - 1) * height + (height - h - 1);
+  if (angle == 90) return h * height + (height - w - 1);
+  if (angle == 180) return (height - w - 1) * height + (height - h - 1);
+  return (height - h - 1) * height + w;
+}
+
+// orbit-major row permutation of a 2/4-fold symmetric grid (utils.c:2103-2124)
+static void make_itable(int width, int height, int s, int* itable)
+{
+  int based = width * height / s;
+  if (s == 2) {
+    for (int i = 0; i < based; i++) { itable[i] = i; itable[rotate_host(i, width, height, 2, 180)] = i + based; }
+  } else {
+    int bh = height / 2;
+    for (int i = 0; i < based; i++) {
+      int v = (i / bh) * height + (i % bh);
+      itable[v] = i;
+      itable[rotate_host(v, width, height, 4, 90)] = i + based;
+      itable[rotate_host(v, width, height, 4, 180)] = i + 2 * based;
+      itable[rotate_host(v, width, height, 4, 270)] = i + 3 * based;
+    }
+  }
+}
+
+static bool profile_env()
+{
+  const char* v = getenv("ODP_PROFILE");
+  return v && atoi(v) == 1;
+}
+
+static void lifecycle_init(int family, int nodes, int degree, const int* numdeg_full, int s,
+                           int width, int height, bool grid_s, const int* itable)
+{
+  if (E.active) ODP_ERROR("ODP_Init_aspl_* called twice without ODP_Finalize_aspl\n");
+  if (s < 1 || nodes % s != 0) ODP_ERROR("nodes(%d) must be divisible by symmetries(%d)\n", nodes, s);
+  if (nodes < 1 || degree < 0) ODP_ERROR("Invalid nodes(%d) or degree(%d)\n", nodes, degree);
+  E.kind = ref_kind(nodes, degree, s, numdeg_full != nullptr, grid_s);
+  E.ev.shape(family, nodes, degree, s, width, height, numdeg_full, grid_s ? 1 : 0);
+  if (grid_s) E.ev.set_itable(itable);
+  E.ev.apply_plan(E.ev.default_plan(E.kind));
+  E.profile = profile_env();
+  E.elapsed = 0;
+  E.times = 0;
+  E.active = true;
+}
+
+extern "C" void ODP_Init_aspl_general(int nodes, int degree, int* num_degrees)
+{
+  lifecycle_init(0, nodes, degree, num_degrees, 1, 0, -1, false, nullptr);
+}
+
+extern "C" void ODP_Init_aspl_general_s(int nodes, int degree, int* num_degrees, int symmetries)
+{
+  if (symmetries < 1 || nodes % symmetries != 0)
+    ODP_ERROR("nodes(%d) must be divisible by symmetries(%d)\n", nodes, symmetries);
+  std::vector<int> full;
+  if (num_degrees) {          // aspl_cuda.cu:166-175: the base array repeated per orbit
+    int based = nodes / symmetries;
+    full.resize(nodes);
+    for (int i = 0; i < symmetries; i++)
+      for (int j = 0; j < based; j++) full[(size_t)i * based + j] = num_degrees[j];
+  }
+  lifecycle_init(0, nodes, degree, num_degrees ? full.data() : nullptr, symmetries, 0, -1, false, nullptr);
+}
+
+extern "C" void ODP_Init_aspl_grid(int width, int height, int degree, int* num_degrees)
+{
+  lifecycle_init(1, width * height, degree, num_degrees, 1, width, height, false, nullptr);
+}
+
+extern "C" void ODP_Init_aspl_grid_s(int width, int height, int degree, int* num_degrees, int symmetries)
+{
+  int nodes = width * height;
+  bool gs = (symmetries == 2 || symmetries == 4);
+  if (symmetries < 1 || nodes % symmetries != 0)
+    ODP_ERROR("nodes(%d) must be divisible by symmetries(%d)\n", nodes, symmetries);
+  if (symmetries == 2 && width % 2 != 0) ODP_ERROR("width(%d) must be divisible by 2\n", width);
+  if (symmetries == 4 && (width % 2 != 0 || height % 2 != 0))
+    ODP_ERROR("width(%d) and height(%d) must be divisible by 2\n", width, height);
+  if (symmetries == 4 && width != height) ODP_ERROR("Must be the same as width(%d) and height(%d)\n", width, height);
+  std::vector<int> itable;
+  if (gs) { itable.resize(nodes); make_itable(width, height, symmetries, itable.data()); }
+  std::vector<int> full;
+  if (num_degrees) {          // aspl_cuda.cu:200-219: every orbit member gets its base vertex's degree
+    int based = nodes / symmetries;
+    full.resize(nodes);
+    if (symmetries == 2) {
+      for (int i = 0; i < based; i++) {
+        full[i] = num_degrees[i];
+        full[rotate_host(i, width, height, 2, 180)] = num_degrees[i];
+      }
+    } else if (symmetries == 4) {
+      int bh = height / 2;
+      for (int i = 0; i < based; i++) {
+        int v = (i / bh) * height + (i % bh);
+        full[v] = num_degrees[i];
+        full[rotate_host(v, width, height, 4, 90)] = num_degrees[i];
+        full[rotate_host(v, width, height, 4, 180)] = num_degrees[i];
+        full[rotate_host(v, width, height, 4, 270)] = num_degrees[i];
+      }
+    } else {
+      for (int i = 0; i < symmetries; i++)
+        for (int j = 0; j < based; j++) full[(size_t)i * based + j] = num_degrees[j];
+    }
+  }
+  lifecycle_init(1, nodes, degree, num_degrees ? full.data() : nullptr, symmetries, width, height, gs,
+                 gs ? itable.data() : nullptr);
+}
+
+static void need_active(const char* fn)
+{
+  if (!E.active) ODP_ERROR("%s called before ODP_Init_aspl_*\n", fn);
+}
+
+static void finish_result(int* diameter, long* sum, double* ASPL)
+{
+  if (*diameter > E.ev.n) {   // aspl_cuda.cu:256-260
+    *diameter = INT_MAX;
+    *sum = LONG_MAX;
+    *ASPL = DBL_MAX;
+  }
+}
+
+extern "C" void ODP_Set_aspl(void* adjacency, int* diameter, long* sum, double* ASPL)
+{
+  need_active("ODP_Set_aspl");
+  double t = wall_time();
+  E.ev.upload(adjacency);
+  E.ev.evaluate(diameter, sum, ASPL);
+  E.elapsed += wall_time() - t;
+  finish_result(diameter, sum, ASPL);
+  E.times++;
+}
+
+extern "C" void ODP_Finalize_aspl(void)
+{
+  if (!E.active) return;
+  if (E.profile) {             // utils.c:1268-1294, flavour "HIP"
+    char host[256] = {0};
+    gethostname(host, sizeof(host) - 1);
+    time_t t = time(NULL);
+    double mem = ref_mem_mb(E.kind, E.ev.n, E.ev.degree, E.ev.s, E.ev.has_numdeg != 0, E.ev.grid_s != 0);
+    printf("------ Profile for SET_ASPL ------\n");
+    printf("Date            = %s", ctime(&t));
+    printf("Hostname        = %s\n", host);
+    printf("Number of Times = %u\n", E.times);
+    printf("Total Time      = %f sec.\n", E.elapsed);
+    printf("Average Time    = %f sec.\n", E.times ? E.elapsed / E.times : 0.0);
+    printf("Algorithm       = %s (HIP)\n", E.kind == KIND_MATRIX ? "MATRIX" : "MATRIX_SAVING");
+    printf("Symmetries      = %d\n", E.ev.s);
+    printf("Memory Usage    = %.3f MB\n", mem);
+    printf("Num of Procs    = 1\n");
+    printf("Num of Threads  = 1\n");
+    printf("--------- End of Profile ---------\n");
+  }
+  E.ev.release();
+  E.active = false;
+  E.kind = 0;
+  E.times = 0;
+  E.elapsed = 0;
+}
+
+// ---------------------------------------------------------------------------
+// plans, autotuning, device-resident evaluation, timing
+
+extern "C" int ODPX_Get_plan(ODPX_Plan* p)
+{
+  if (!E.active) return -1;
+  *p = E.ev.plan;
+  return 0;
+}
+
+extern "C" int ODPX_Set_plan(const ODPX_Plan* p)
+{
+  if (!E.active || !E.ev.plan_valid(*p)) return -1;
+  E.ev.apply_plan(*p);
+  return 0;
+}
+
+extern "C" void ODPX_Upload_adjacency(void* adjacency)
+{
+  need_active("ODPX_Upload_adjacency");
+  E.ev.upload(adjacency);
+  HIPC(hipStreamSynchronize(C.stream));
+}
+
+extern "C" void ODPX_Set_aspl_resident(int* diameter, long* sum, double* ASPL)
+{
+  need_active("ODPX_Set_aspl_resident");
+  double t = wall_time();
+  E.ev.evaluate(diameter, sum, ASPL);
+  E.elapsed += wall_time() - t;
+  finish_result(diameter, sum, ASPL);
+  E.times++;
+}
+
+__global__ void k_swap(int* adj, int degree, int a, int b, int c, int d, int* err)
+{
+  const int t = threadIdx.x;
+  if (t >= 4) return;
+  const int row = t == 0 ? a : t == 1 ? b : t == 2 ? c : d;
+  const int from = t == 0 ? b : t == 1 ? a : t == 2 ? d : c;
+  const int to = t == 0 ? c : t == 1 ? d : t == 2 ? a : b;
+  int* r = adj + (int64_t)row * degree;
+  for (int j = 0; j < degree; j++)
+    if (r[j] == from) { r[j] = to; return; }
+  atomicOr(err, 4);
+}
+
+extern "C" int ODPX_Swap_device(int a, int b, int c, int d)
+{
+  need_active("ODPX_Swap_device");
+  Evaluator& v = E.ev;
+  if (v.s != 1) return -1;
+  int ids[4] = {a, b, c, d};
+  for (int x : ids) if (x < 0 || x >= v.n) return -1;
+  hipLaunchKernelGGL(k_swap, dim3(1), dim3(64), 0, C.stream, v.d_adj, v.degree, a, b, c, d, v.d_err);
+  HIPC(hipGetLastError());
+  HIPC(hipMemcpyAsync(v.h_err, v.d_err, sizeof(int), hipMemcpyDeviceToHost, C.stream));
+  HIPC(hipStreamSynchronize(C.stream));
+  if (*v.h_err & 4) {
+    *v.h_err = 0;
+    HIPC(hipMemset(v.d_err, 0, sizeof(int)));
+    return -1;
+  }
+  return 0;
+}
+
+extern "C" void ODPX_Stats_enable(int on) { g_stats_on = on != 0; }
+extern "C" void ODPX_Stats_reset(void) { memset(&g_stats, 0, sizeof g_stats); }
+extern "C" void ODPX_Stats_get(ODPX_Stats* out) { *out = g_stats; }
+
+extern "C" int ODPX_Autotune(void* adjacency, int repeats)
+{
+  need_active("ODPX_Autotune");
+  Evaluator& v = E.ev;
+  std::vector<ODPX_Plan> cand;
+  int wbmax = v.resident_wb_max();
+  for (int wb = 1; wb <= wbmax; wb <<= 1)
+    for (int th : {256, 512, 1024}) {
+      ODPX_Plan p{};
+      p.plan = ODPX_PLAN_RESIDENT; p.wb = wb; p.threads = th;
+      cand.push_back(p);
+    }
+  int W = (v.nsrc + 63) / 64;
+  std::vector<int> kws = {0};
+  for (int k : {512, 256, 128, 64, 32, 16}) if (k < W) kws.push_back(k);
+  for (int kw : kws)
+    for (int sm = 0; sm < 2; sm++)
+      for (int x = 0; x < 2; x++) {
+        ODPX_Plan p{};
+        p.plan = ODPX_PLAN_LEVEL; p.kw = kw; p.upw = 4; p.seg_major = sm; p.xcd = x;
+        cand.push_back(p);
+      }
+  if (repeats < 1) repeats = 1;
+  v.upload(adjacency);
+  double best = 1e30;
+  ODPX_Plan bestp = v.plan;
+  int rd = 0; long rs = 0; double ra = 0;
+  bool have = false;
+  for (const ODPX_Plan& p : cand) {
+    v.apply_plan(p);
+    int d; long sm; double as;
+    v.evaluate(&d, &sm, &as);   // warm-up, also fixes the level guess
+    double t0 = wall_time();
+    for (int i = 0; i < repeats; i++) v.evaluate(&d, &sm, &as);
+    double t = (wall_time() - t0) / repeats;
+    if (!have) { rd = d; rs = sm; ra = as; have = true; }
+    else if (d != rd || sm != rs || memcmp(&as, &ra, sizeof(double)) != 0)
+      ODP_ERROR("libodp_hip: plans disagree (%d %ld vs %d %ld)\n", d, sm, rd, rs);
+    if (t < best) { best = t; bestp = p; }
+  }
+  v.apply_plan(bestp);
+  return (int)cand.size();
+}
+
+// ---------------------------------------------------------------------------
+// batched evaluation: every graph small enough for LDS in one persistent launch
+
+struct BatchGraph { long long adj_off, nbr_off, ent_start; int n, degree, dpad, pad; };
+
+__global__ void k_prepare_batch(const BatchGraph* __restrict__ gs, int ng, long long total,
+                                const int* __restrict__ adj, int* __restrict__ nbr, int* err,
+                                ResOut* out, int* qhead)
+{
+  const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long g = tid; g < ng; g += stride) { out[g].sum = 0; out[g].diam = 0; out[g].flags = 0; out[g].levels = 0; }
+  if (tid == 0) *qhead = 0;
+  for (long long idx = tid; idx < total; idx += stride) {
+    int lo = 0, hi = ng - 1;
+    while (lo < hi) {
+      int mid = (lo + hi + 1) >> 1;
+      if (gs[mid].ent_start <= idx) lo = mid; else hi = mid - 1;
+    }
+    const BatchGraph G = gs[lo];
+    const long long e = idx - G.ent_start;
+    const int i = (int)(e / G.dpad), j = (int)(e - (long long)i * G.dpad);
+    int out_v = i;
+    if (j < G.degree) {
+      const int v = adj[G.adj_off + (long long)i * G.degree + j];
+      if (v >= 0 && v < G.n) out_v = v;
+      else if (v != -1) atomicOr(err, 1);
+    }
+    nbr[G.nbr_off + e] = out_v;
+  }
+}
+
+static struct {
+  size_t adj_cap = 0, nbr_cap = 0, g_cap = 0, t_cap = 0;
+  int *d_adj = nullptr, *d_nbr = nullptr, *d_err = nullptr, *d_qhead = nullptr, *h_err = nullptr;
+  BatchGraph* d_g = nullptr;
+  ResOut *d_out = nullptr, *h_out = nullptr;
+  ResTask* d_t = nullptr;
+  Evaluator single;          // graphs too large for the LDS path, and the looped path
+} B;
+
+static int batch_wb(int n)
+{
+  for (int wb = 16; wb >= 1; wb >>= 1)
+    if ((size_t)2 * n * wb * 4 <= LDS_DYN_MAX) return wb;
+  return 0;
+}
+
+static void finish_one(int n, u64 excess, int d, bool disc, int* diameter, long* sum, double* ASPL)
+{
+  long lsum = (long)n * (n - 1) + (long)excess;
+  if (disc) d = n + 1;
+  double aspl = lsum / (((double)n - 1) * n);
+  lsum /= 2.0;
+  if (d > n) { d = INT_MAX; lsum = LONG_MAX; aspl = DBL_MAX; }
+  *diameter = d; *sum = lsum; *ASPL = aspl;
+}
+
+static void single_eval(Evaluator& v, int n, int degree, const int* adj, int plan_kind,
+                        int* diameter, long* sum, double* ASPL)
+{
+  v.shape(0, n, degree, 1, 0, -1, nullptr, 0);
+  ODPX_Plan p = v.default_plan(KIND_MATRIX);
+  if (plan_kind == ODPX_PLAN_LEVEL && p.plan != ODPX_PLAN_LEVEL) {
+    p = ODPX_Plan{};
+    p.plan = ODPX_PLAN_LEVEL; p.kw = 0; p.upw = 4; p.seg_major = 1; p.xcd = 0;
+  }
+  v.apply_plan(p);
+  v.upload(adj);
+  v.evaluate(diameter, sum, ASPL);
+  if (*diameter > n) { *diameter = INT_MAX; *sum = LONG_MAX; *ASPL = DBL_MAX; }
+}
+
+extern "C" int ODPX_Batch_eval(int ngraphs, const int* nodes, const int* degrees, const long long* offsets,
+                               const int* packed, int* diameter, long* sum, double* ASPL)
+{
+  ctx_init();
+  if (ngraphs <= 0) return 0;
+  std::vector<int> small, large;
+  long long adj_end = 0;
+  for (int g = 0; g < ngraphs; g++) {
+    if (nodes[g] < 1 || degrees[g] < 0) ODP_ERROR("ODPX_Batch_eval: invalid graph %d\n", g);
+    adj_end = std::max(adj_end, offsets[g] + (long long)nodes[g] * degrees[g]);
+    (batch_wb(nodes[g]) > 0 ? small : large).push_back(g);
+  }
+  if (!small.empty()) {
+    std::vector<BatchGraph> hg(small.size());
+    long long nbr_tot = 0;
+    size_t lds = 0;
+    std::vector<ResTask> tasks;
+    std::vector<double> cost;
+    for (size_t k = 0; k < small.size(); k++) {
+      int g = small[k];
+      BatchGraph& G = hg[k];
+      G.adj_off = offsets[g]; G.n = nodes[g]; G.degree = degrees[g]; G.dpad = round_up(std::max(1, degrees[g]), 4);
+      G.nbr_off = nbr_tot; G.ent_start = nbr_tot; G.pad = 0;
+      nbr_tot += (long long)G.n * G.dpad;
+    }
+    if ((size_t)adj_end > B.adj_cap) { dfree(&B.d_adj); dalloc(&B.d_adj, (size_t)adj_end); B.adj_cap = adj_end; }
+    if ((size_t)nbr_tot > B.nbr_cap) { dfree(&B.d_nbr); dalloc(&B.d_nbr, (size_t)nbr_tot); B.nbr_cap = nbr_tot; }
+    if (small.size() > B.g_cap) {
+      dfree(&B.d_g); dfree(&B.d_out); hfree(&B.h_out);
+      dalloc(&B.d_g, small.size()); dalloc(&B.d_out, small.size()); halloc(&B.h_out, small.size());
+      B.g_cap = small.size();
+    }
+    if (!B.d_err) { dalloc(&B.d_err, 1); dalloc(&B.d_qhead, 1); halloc(&B.h_err, 1); HIPC(hipMemset(B.d_err, 0, 4)); }
+    for (size_t k = 0; k < small.size(); k++) {
+      const BatchGraph& G = hg[k];
+      int wb = batch_wb(G.n), b = 32 * wb, nt = (G.n + b - 1) / b;
+      lds = std::max(lds, (size_t)2 * G.n * wb * 4);
+      for (int t = 0; t < nt; t++) {
+        ResTask T;
+        T.nbr = B.d_nbr + G.nbr_off; T.n = G.n; T.dpad = G.dpad; T.src0 = t * b; T.l = std::min(b, G.n - t * b);
+        T.s = 1; T.graph = (int)k; T.wb = wb; T.pad = 0;
+        tasks.push_back(T);
+        cost.push_back((double)G.n * G.dpad * (1.0 + log((double)G.n) / log(std::max(2.0, G.degree - 1.0))));
+      }
+    }
+    // longest blocks first, so the queue's tail is short
+    std::vector<int> ord(tasks.size());
+    for (size_t i = 0; i < ord.size(); i++) ord[i] = (int)i;
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+    std::vector<ResTask> sorted(tasks.size());
+    for (size_t i = 0; i < ord.size(); i++) sorted[i] = tasks[ord[i]];
+    if (sorted.size() > B.t_cap) { dfree(&B.d_t); dalloc(&B.d_t, sorted.size()); B.t_cap = sorted.size(); }
+    HIPC(hipMemcpyAsync(B.d_adj, packed, sizeof(int) * adj_end, hipMemcpyHostToDevice, C.stream));
+    HIPC(hipMemcpyAsync(B.d_g, hg.data(), sizeof(BatchGraph) * hg.size(), hipMemcpyHostToDevice, C.stream));
+    HIPC(hipMemcpyAsync(B.d_t, sorted.data(), sizeof(ResTask) * sorted.size(), hipMemcpyHostToDevice, C.stream));
+    TimedLaunch tp;
+    timed_begin(tp, T_OTHER, -1);
+    int pgrid = (int)std::min<long long>(std::max<long long>(1, (nbr_tot + 255) / 256), 8192);
+    hipLaunchKernelGGL(k_prepare_batch, dim3(pgrid), dim3(256), 0, C.stream, B.d_g, (int)hg.size(), nbr_tot,
+                       B.d_adj, B.d_nbr, B.d_err, B.d_out, B.d_qhead);
+    HIPC(hipGetLastError());
+    timed_end(tp);
+    int per_cu = (int)std::max<size_t>(1, LDS_LIMIT / (lds + LDS_STATIC));
+    per_cu = std::min(per_cu, 2);
+    int grid = std::max(1, std::min((int)sorted.size(), per_cu * C.ncu));
+    TimedLaunch tr;
+    timed_begin(tr, T_RESIDENT, 0);
+    hipLaunchKernelGGL(k_resident, dim3(grid), dim3(1024), lds, C.stream, B.d_t, (int)sorted.size(), B.d_qhead, B.d_out);
+    HIPC(hipGetLastError());
+    timed_end(tr);
+    HIPC(hipMemcpyAsync(B.h_out, B.d_out, sizeof(ResOut) * small.size(), hipMemcpyDeviceToHost, C.stream));
+    HIPC(hipMemcpyAsync(B.h_err, B.d_err, sizeof(int), hipMemcpyDeviceToHost, C.stream));
+    HIPC(hipStreamSynchronize(C.stream));
+    if (*B.h_err) ODP_ERROR("ODPX_Batch_eval: adjacency holds a vertex id outside [0, nodes)\n");
+    for (size_t k = 0; k < small.size(); k++) {
+      int g = small[k];
+      const ResOut& o = B.h_out[k];
+      finish_one(nodes[g], o.sum, std::max(1, o.diam), (o.flags & 1) != 0, &diameter[g], &sum[g], &ASPL[g]);
+    }
+    if (g_stats_on) {
+      for (TimedLaunch& t : g_pending) {
+        float ms = 0;
+        HIPC(hipEventElapsedTime(&ms, t.a, t.b));
+        g_evpool.push_back(t.a); g_evpool.push_back(t.b);
+        if (t.kind == T_RESIDENT) {
+          double bytes = 0;
+          for (size_t k = 0; k < small.size(); k++) {
+            const BatchGraph& G = hg[k];
+            int wb = batch_wb(G.n);
+            double row = (G.degree + 2.0) * (std::min(32 * wb, G.n) / 8.0) + 4.0 * G.degree;
+            bytes += (double)B.h_out[k].levels * G.n * row;
+          }
+          g_stats.resident_launches++; g_stats.resident_ms += ms; g_stats.resident_bytes += bytes;
+        } else { g_stats.other_launches++; g_stats.other_ms += ms; }
+      }
+      g_pending.clear();
+      g_stats.evals += (long long)small.size();
+    }
+  }
+  for (int g : large)
+    single_eval(B.single, nodes[g], degrees[g], packed + offsets[g], ODPX_PLAN_AUTO, &diameter[g], &sum[g], &ASPL[g]);
+  return 0;
+}
+
+extern "C" int ODPX_Loop_eval(int ngraphs, const int* nodes, const int* degrees, const long long* offsets,
+                              const int* packed, int plan, int* diameter, long* sum, double* ASPL)
+{
+  ctx_init();
+  for (int g = 0; g < ngraphs; g++)
+    single_eval(B.single, nodes[g], degrees[g], packed + offsets[g], plan, &diameter[g], &sum[g], &ASPL[g]);
+  return 0;
+}
+
+extern "C" void ODPX_Batch_release(void)
+{
+  dfree(&B.d_adj); dfree(&B.d_nbr); dfree(&B.d_err); dfree(&B.d_qhead); hfree(&B.h_err);
+  dfree(&B.d_g); dfree(&B.d_out); hfree(&B.h_out); dfree(&B.d_t);
+  B.adj_cap = B.nbr_cap = B.g_cap = B.t_cap = 0;
+  B.single.release();
+}

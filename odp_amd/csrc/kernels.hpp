@@ -1,0 +1,484 @@
+// kernels.hpp -- gfx950 (CDNA4, wave64) kernels of the bit-matrix ASPL/diameter
+// method.  Included once, by aspl_hip.hip.
+//
+// The method is the reference's (aspl.c:35-119, aspl_cuda.cu:53-126,
+// utils_cuda.cu:97-211): bit c of row v says "source c reaches v"; each level
+// computes next[v] = cur[v] | OR_{u in N(v)} cur[u] on 64-bit words and
+// popcounts next.  Two kernel families implement it:
+//
+//  * level kernels (k_level0, k_level): one launch per BFS level for every
+//    column block ("task") of one or more graphs.  A wave owns a segment of
+//    64..128 words of a row (16 B per lane, so every neighbour gather is one
+//    coalesced 512 B..1 KiB stream).  The popcount is fused and reduced per
+//    wave into one 64-bit atomic; termination (count == n*l) is detected on
+//    the device by the next level's workgroups, so the host needs no sync per
+//    level (the reference syncs every level, aspl_cuda.cu:67).  Counts live in
+//    a ring of `ring` slots per task; level L zeroes slot L+1 before level L+1
+//    adds into it, and the host reads the ring between batches of levels.
+//  * resident kernel (k_resident): for graphs small enough that a block of
+//    32*wb sources of BOTH bit buffers fits in the 160 KiB LDS of a CU,
+//    one workgroup runs every level of one block in LDS.  Workgroups pull
+//    blocks from a work queue (one atomic counter), so many graphs of uneven
+//    size are evaluated in one persistent launch.
+//
+// Neighbour ids reaching these kernels come from the row-neighbour table
+// built by k_prepare: every entry is a valid row index (short rows are padded
+// with the row's own index, which ORs in nothing new), so no gather can leave
+// the bit buffers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+typedef unsigned long long u64;
+typedef unsigned int u32;
+
+struct __attribute__((aligned(16))) u64x2 { u64 x, y; };
+
+#define ODP_NXCD 8          // XCDs of an MI355X; workgroups are dealt round-robin
+#define LEVEL_WG 256        // threads per level-kernel workgroup (4 waves)
+
+// ---------------------------------------------------------------------------
+// shared helpers
+
+__device__ __forceinline__ u64 wave_sum(u64 v)
+{
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// valid-bit mask of word w of a block holding l sources
+__device__ __forceinline__ u64 full_word(int l, int w)
+{
+  int v = l - 64 * w;
+  return v >= 64 ? ~0ULL : (v <= 0 ? 0ULL : ((1ULL << v) - 1ULL));
+}
+
+// blockIdx -> logical index so that each XCD gets one contiguous range
+__device__ __forceinline__ int remap_xcd(int b, int nb)
+{
+  int q = nb / ODP_NXCD, r = nb % ODP_NXCD, x = b % ODP_NXCD, i = b / ODP_NXCD;
+  return x * q + (x < r ? x : r) + i;
+}
+
+// ---------------------------------------------------------------------------
+// k_prepare: caller adjacency -> row-neighbour table.
+//
+//  s == 1              : row i, neighbour adj[i][j]                      (utils_cuda.cu:101-113)
+//  general, s > 1      : row i = p*b+m, neighbour adj[m][j] + p*b (mod n) (utils_cuda.cu:133-151)
+//  grid, s in {2, 4}   : vertex i lives in row itable[i]; neighbour is the
+//                        rotated first-quadrant entry (utils_cuda.cu:3-59, 117-132)
+// Entries j >= num_degrees[i] become the row itself.  Ids outside [0, n) set
+// *err and are replaced by the row itself (the reference would read out of
+// bounds; the host turns *err into the reference's print-and-exit).
+// The first threads also reset the per-evaluation counters.
+
+struct PrepArgs {
+  const int* adj;        // [(n/s)][degree]
+  const int* numdeg;     // [n] expanded, or null
+  const int* itable;     // [n] or null
+  int* nbr;              // [n][dpad]
+  int* err;
+  int n, s, degree, dpad, width, height, grid_s;
+  u64* zero64; int nzero64;   // counters reset to 0 at the start of an evaluation
+  u32* zero32; int nzero32;
+};
+
+__device__ __forceinline__ int grid_rotate(int v, int width, int height, int s, int angle)
+{
+  int w = v / height, h = v % height;
+  if (s == 2) return (width - w - 1) * height + (height - h - 1);
+  if (angle == 90) return h * height + (height - w - 1);
+  if (angle == 180) return (height - w - 1) * height + (height - h - 1);
+  return (height - h - 1) * height + w;
+}
+
+// neighbour d of vertex v of a 2/4-fold symmetric grid; -1 when the table
+// entry is not a vertex id
+__device__ __forceinline__ int grid_nbr(const int* adj, int width, int height, int degree, int s, int v, int d)
+{
+  int n = width * height, bw = width / 2, bh = height / 2, w = v / height, h = v % height;
+  int base, back = 0;
+  if (s == 2) {
+    if (w < bw) base = v;
+    else { base = grid_rotate(v, width, height, 2, 180); back = 180; }
+  } else {
+    int x = v, lx;
+    if (w < bw && h < bh) back = 0;
+    else if (w < bw) { x = grid_rotate(v, width, height, 4, 270); back = 90; }
+    else if (h >= bh) { x = grid_rotate(v, width, height, 4, 180); back = 180; }
+    else { x = grid_rotate(v, width, height, 4, 90); back = 270; }
+    lx = (x / height) * bh + (x % height);
+    base = lx;
+  }
+  int y = adj[(int64_t)base * degree + d];
+  if (y < 0 || y >= n) return -1;
+  return back ? grid_rotate(y, width, height, s, back) : y;
+}
+
+__global__ void k_prepare(PrepArgs a)
+{
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = tid; i < a.nzero64; i += stride) a.zero64[i] = 0;
+  for (int64_t i = tid; i < a.nzero32; i += stride) a.zero32[i] = 0;
+  const int64_t total = (int64_t)a.n * a.dpad;
+  const int b = a.n / a.s;
+  for (int64_t idx = tid; idx < total; idx += stride) {
+    const int i = (int)(idx / a.dpad), j = (int)(idx - (int64_t)i * a.dpad);
+    const int row = a.grid_s ? a.itable[i] : i;
+    int d = a.numdeg ? a.numdeg[i] : a.degree;
+    if (d < 0 || d > a.degree) { atomicOr(a.err, 2); d = d < 0 ? 0 : a.degree; }
+    int out = row;
+    if (j < d) {
+      int v;
+      if (a.grid_s) {
+        v = grid_nbr(a.adj, a.width, a.height, a.degree, a.s, i, j);
+      } else if (a.s == 1) {
+        v = a.adj[(int64_t)i * a.degree + j];
+        if (v >= a.n) v = -1;
+      } else {
+        const int p = i / b, m = i - p * b;
+        v = a.adj[(int64_t)m * a.degree + j];
+        if (v >= a.n) v = -1;
+        if (v >= 0) { v += p * b; if (v >= a.n) v -= a.n; }
+      }
+      if (v < 0) atomicOr(a.err, 1);
+      else out = a.grid_s ? a.itable[v] : v;
+    }
+    a.nbr[(int64_t)row * a.dpad + j] = out;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// level kernels
+
+struct LevelTask {
+  const int* nbr;      // [n][dpad]
+  u64* buf0;           // [n][kwp]  level L reads buf[L&1], writes buf[(L+1)&1]
+  u64* buf1;
+  u64* cnt;            // per-level popcount of this task: cnt[(L % ring) * cstride]
+  u32* done;           // set once the task has terminated
+  int n, dpad, kwp;    // rows, neighbour-table width, words per row in the buffers
+  int src0, l;         // first source (row index) of the block, sources in it
+  int lpr_log2;        // log2(lanes per row): 6 -> one row per wave
+  int segs;            // 2*64-word segments per row (kwp / (2 * lanes per row))
+  int rgroups;         // ceil(n / rows per wave)
+  int units;           // rgroups * segs wave-units
+  int seg_major;       // unit order: segment-major (1) or row-major (0)
+  int cstride;         // level stride of cnt
+  int pad;
+};
+
+__device__ __forceinline__ int find_task(const int* __restrict__ wg_start, int ntasks, int g)
+{
+  int lo = 0, hi = ntasks - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (wg_start[mid] <= g) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+
+struct UnitGeom { int r, w; };
+
+__device__ __forceinline__ UnitGeom unit_geom(const LevelTask& T, int u, int lane)
+{
+  int seg, rg;
+  if (T.seg_major) { seg = u / T.rgroups; rg = u - seg * T.rgroups; }
+  else { rg = u / T.segs; seg = u - rg * T.segs; }
+  const int lpr = 1 << T.lpr_log2;
+  UnitGeom g;
+  g.r = (rg << (6 - T.lpr_log2)) + (lane >> T.lpr_log2);
+  g.w = seg * (2 * lpr) + 2 * (lane & (lpr - 1));
+  return g;
+}
+
+__device__ __forceinline__ void put_bit(u64& x0, u64& x1, int c, int l, int w)
+{
+  if ((unsigned)c < (unsigned)l) {
+    const int cw = c >> 6;
+    const u64 bit = 1ULL << (c & 63);
+    if (cw == w) x0 |= bit;
+    else if (cw == w + 1) x1 |= bit;
+  }
+}
+
+// Level 0 of every task straight from the identity: next[r] holds the bits of
+// r itself and of its neighbours (cur = identity needs no reads).
+__global__ __launch_bounds__(LEVEL_WG) void k_level0(const LevelTask* __restrict__ tasks,
+                                                     const int* __restrict__ wg_start, int ntasks,
+                                                     int ring, int upw, int xcd)
+{
+  int g = blockIdx.x;
+  if (xcd) g = remap_xcd(g, gridDim.x);
+  const int t = ntasks > 1 ? find_task(wg_start, ntasks, g) : 0;
+  const LevelTask T = tasks[t];
+  const int wg = g - wg_start[t];
+  if (wg == 0 && threadIdx.x == 0) T.cnt[(int64_t)(1 % ring) * T.cstride] = 0;   // level 1's slot
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  u64 pop = 0;
+  const int base = wg * 4 * upw;
+  for (int i = 0; i < upw; i++) {
+    const int u = base + i * 4 + wave;
+    if (u >= T.units) break;
+    const UnitGeom q = unit_geom(T, u, lane);
+    if (q.r < T.n) {
+      u64 x0 = 0, x1 = 0;
+      put_bit(x0, x1, q.r - T.src0, T.l, q.w);
+      const int* nb = T.nbr + (int64_t)q.r * T.dpad;
+      for (int j = 0; j < T.dpad; j += 4) {
+        const int4 v = *(const int4*)(nb + j);
+        put_bit(x0, x1, v.x - T.src0, T.l, q.w);
+        put_bit(x0, x1, v.y - T.src0, T.l, q.w);
+        put_bit(x0, x1, v.z - T.src0, T.l, q.w);
+        put_bit(x0, x1, v.w - T.src0, T.l, q.w);
+      }
+      u64x2 o; o.x = x0; o.y = x1;
+      *(u64x2*)(T.buf1 + (int64_t)q.r * T.kwp + q.w) = o;
+      pop += (u64)(__popcll(x0) + __popcll(x1));
+    }
+  }
+  pop = wave_sum(pop);
+  if (lane == 0 && pop) atomicAdd(T.cnt, pop);
+}
+
+// One gather level L >= 1 of every task.  UROW: one row per wave, so the row's
+// neighbour ids are wave-uniform (scalar loads).
+template <bool UROW>
+__global__ __launch_bounds__(LEVEL_WG) void k_level(const LevelTask* __restrict__ tasks,
+                                                    const int* __restrict__ wg_start, int ntasks,
+                                                    int level, int ring, int upw, int xcd)
+{
+  int g = blockIdx.x;
+  if (xcd) g = remap_xcd(g, gridDim.x);
+  const int t = ntasks > 1 ? find_task(wg_start, ntasks, g) : 0;
+  const LevelTask T = tasks[t];
+  if (*T.done) return;
+  const u64 target = (u64)T.n * (u64)T.l;
+  {
+    const u64 c1 = T.cnt[(int64_t)((level - 1) % ring) * T.cstride];
+    const bool stop = (c1 == target) || (level >= 2 && c1 == T.cnt[(int64_t)((level - 2) % ring) * T.cstride]);
+    if (stop) {
+      if (threadIdx.x == 0) *T.done = 1;
+      return;
+    }
+  }
+  const int wg = g - wg_start[t];
+  if (wg == 0 && threadIdx.x == 0) T.cnt[(int64_t)((level + 1) % ring) * T.cstride] = 0;
+  const u64* __restrict__ cur = (level & 1) ? T.buf1 : T.buf0;
+  u64* __restrict__ nxt = (level & 1) ? T.buf0 : T.buf1;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  const int64_t kwp = T.kwp;
+  u64 pop = 0;
+  const int base = wg * 4 * upw;
+  for (int i = 0; i < upw; i++) {
+    const int u = base + i * 4 + wave;
+    if (u >= T.units) break;
+    UnitGeom q = unit_geom(T, u, lane);
+    if (UROW) q.r = __builtin_amdgcn_readfirstlane(q.r);
+    if (q.r < T.n) {
+      const u64 f0 = full_word(T.l, q.w), f1 = full_word(T.l, q.w + 1);
+      u64x2 x = *(const u64x2*)(cur + q.r * kwp + q.w);
+      if (x.x != f0 || x.y != f1) {
+        const int* nb = T.nbr + (int64_t)q.r * T.dpad;
+#pragma unroll 2
+        for (int j = 0; j < T.dpad; j += 4) {
+          const int4 v = *(const int4*)(nb + j);
+          const u64x2 a = *(const u64x2*)(cur + (int64_t)v.x * kwp + q.w);
+          const u64x2 b = *(const u64x2*)(cur + (int64_t)v.y * kwp + q.w);
+          const u64x2 c = *(const u64x2*)(cur + (int64_t)v.z * kwp + q.w);
+          const u64x2 d = *(const u64x2*)(cur + (int64_t)v.w * kwp + q.w);
+          x.x |= (a.x | b.x) | (c.x | d.x);
+          x.y |= (a.y | b.y) | (c.y | d.y);
+        }
+      }
+      *(u64x2*)(nxt + q.r * kwp + q.w) = x;
+      pop += (u64)(__popcll(x.x) + __popcll(x.y));
+    }
+  }
+  pop = wave_sum(pop);
+  if (lane == 0 && pop) atomicAdd(&T.cnt[(int64_t)(level % ring) * T.cstride], pop);
+}
+
+// ---------------------------------------------------------------------------
+// resident kernel
+
+struct ResTask {
+  const int* nbr;   // [n][dpad]
+  int n, dpad;
+  int src0, l;      // first source row of the block, sources in it (<= 32*wb)
+  int s;            // symmetry factor of the graph
+  int graph;        // index into the per-graph outputs
+  int wb;           // 32-bit words per row of the block: 1, 2, 4, 8 or 16
+  int pad;
+};
+
+struct ResOut {     // per-graph accumulators
+  u64 sum;          // sum over tasks of sum_L (n*l - count_L) * s   (the n(n-1) base excluded)
+  int diam;         // max over tasks of (terminal level + 1)
+  u32 flags;        // bit 0: a task found no new bits before completing (disconnected)
+  u32 levels;       // levels run, summed over the graph's blocks (for byte accounting)
+  u32 pad;
+};
+
+template <int WB> struct RowT { u32 w[WB]; };
+
+template <int WB>
+__device__ __forceinline__ RowT<WB> lds_row(const u32* p)
+{
+  RowT<WB> r;
+  if constexpr (WB == 1) {
+    r.w[0] = p[0];
+  } else if constexpr (WB == 2) {
+    const uint2 v = *(const uint2*)p;
+    r.w[0] = v.x; r.w[1] = v.y;
+  } else {
+#pragma unroll
+    for (int k = 0; k < WB; k += 4) {
+      const uint4 v = *(const uint4*)(p + k);
+      r.w[k] = v.x; r.w[k + 1] = v.y; r.w[k + 2] = v.z; r.w[k + 3] = v.w;
+    }
+  }
+  return r;
+}
+
+template <int WB>
+__device__ __forceinline__ void lds_store(u32* p, const RowT<WB>& r)
+{
+  if constexpr (WB == 1) {
+    p[0] = r.w[0];
+  } else if constexpr (WB == 2) {
+    *(uint2*)p = make_uint2(r.w[0], r.w[1]);
+  } else {
+#pragma unroll
+    for (int k = 0; k < WB; k += 4) *(uint4*)(p + k) = make_uint4(r.w[k], r.w[k + 1], r.w[k + 2], r.w[k + 3]);
+  }
+}
+
+template <int WB>
+__device__ __forceinline__ void row_bit(RowT<WB>& r, int c, int l)
+{
+  if ((unsigned)c < (unsigned)l) {
+#pragma unroll
+    for (int k = 0; k < WB; k++)
+      if ((c >> 5) == k) r.w[k] |= 1u << (c & 31);
+  }
+}
+
+__device__ __forceinline__ u64 block_total(u64 v, u64* part, int lane, int wave, int nw)
+{
+  v = wave_sum(v);
+  if (lane == 0) part[wave] = v;
+  __syncthreads();
+  u64 s = 0;
+  for (int i = 0; i < nw; i++) s += part[i];
+  return s;
+}
+
+// All levels of one block of T.l <= 32*WB sources, both bit buffers in LDS.
+template <int WB>
+__device__ __forceinline__ void res_run(const ResTask& T, u32* lds, u64 (*s_part)[16], ResOut* out,
+                                        int tid, int nth, int lane, int wave, int nw)
+{
+  u32* cur = lds;
+  u32* nxt = lds + (int64_t)T.n * WB;
+  const u64 target = (u64)T.n * (u64)T.l;
+  RowT<WB> full;
+#pragma unroll
+  for (int k = 0; k < WB; k++) {
+    const int v = T.l - 32 * k;
+    full.w[k] = v >= 32 ? ~0u : (v <= 0 ? 0u : ((1u << v) - 1u));
+  }
+  // level 0 from the identity
+  u64 pop = 0;
+  for (int r = tid; r < T.n; r += nth) {
+    RowT<WB> row;
+#pragma unroll
+    for (int k = 0; k < WB; k++) row.w[k] = 0;
+    row_bit<WB>(row, r - T.src0, T.l);
+    const int* nb = T.nbr + (int64_t)r * T.dpad;
+    for (int j = 0; j < T.dpad; j += 4) {
+      const int4 v = *(const int4*)(nb + j);
+      row_bit<WB>(row, v.x - T.src0, T.l);
+      row_bit<WB>(row, v.y - T.src0, T.l);
+      row_bit<WB>(row, v.z - T.src0, T.l);
+      row_bit<WB>(row, v.w - T.src0, T.l);
+    }
+    lds_store<WB>(nxt + (int64_t)r * WB, row);
+#pragma unroll
+    for (int k = 0; k < WB; k++) pop += (u64)__popc(row.w[k]);
+  }
+  int par = 0;
+  u64 c = block_total(pop, s_part[par], lane, wave, nw);
+  u64 prev = 0, sum = 0;
+  int L = 0, D = 0;
+  bool disc = false;
+  for (;;) {
+    if (c == target) { D = L + 1; break; }
+    if ((L >= 1 && c == prev) || L + 1 >= T.n) { disc = true; break; }
+    sum += (target - c) * (u64)T.s;
+    prev = c;
+    L++;
+    u32* tmp = cur; cur = nxt; nxt = tmp;
+    pop = 0;
+    for (int r = tid; r < T.n; r += nth) {
+      RowT<WB> row = lds_row<WB>(cur + (int64_t)r * WB);
+      bool sat = true;
+#pragma unroll
+      for (int k = 0; k < WB; k++) sat = sat && (row.w[k] == full.w[k]);
+      if (!sat) {
+        const int* nb = T.nbr + (int64_t)r * T.dpad;
+        for (int j = 0; j < T.dpad; j += 4) {
+          const int4 v = *(const int4*)(nb + j);
+          const RowT<WB> a = lds_row<WB>(cur + (int64_t)v.x * WB);
+          const RowT<WB> b = lds_row<WB>(cur + (int64_t)v.y * WB);
+          const RowT<WB> cc = lds_row<WB>(cur + (int64_t)v.z * WB);
+          const RowT<WB> d = lds_row<WB>(cur + (int64_t)v.w * WB);
+#pragma unroll
+          for (int k = 0; k < WB; k++) row.w[k] |= (a.w[k] | b.w[k]) | (cc.w[k] | d.w[k]);
+        }
+      }
+      lds_store<WB>(nxt + (int64_t)r * WB, row);
+#pragma unroll
+      for (int k = 0; k < WB; k++) pop += (u64)__popc(row.w[k]);
+    }
+    par ^= 1;
+    c = block_total(pop, s_part[par], lane, wave, nw);
+  }
+  if (tid == 0) {
+    if (sum) atomicAdd(&out[T.graph].sum, sum);
+    atomicMax(&out[T.graph].diam, D);
+    if (disc) atomicOr(&out[T.graph].flags, 1u);
+    atomicAdd(&out[T.graph].levels, (u32)(L + 1));
+  }
+}
+
+// Persistent: every workgroup pulls blocks from *qhead until none is left, so
+// the grid drains once the queue is exhausted.  Blocks of any width run in the
+// same launch (T.wb picks the instantiation); the launch's dynamic LDS must hold
+// 2 * n * wb * 4 bytes for its largest block.
+__global__ __launch_bounds__(1024) void k_resident(const ResTask* __restrict__ tasks, int ntasks,
+                                                   int* __restrict__ qhead, ResOut* __restrict__ out)
+{
+  extern __shared__ u32 lds[];
+  __shared__ int s_task;
+  __shared__ u64 s_part[2][16];
+  const int tid = threadIdx.x, nth = blockDim.x, lane = tid & 63, wave = tid >> 6, nw = nth >> 6;
+  for (;;) {
+    if (tid == 0) s_task = atomicAdd(qhead, 1);
+    __syncthreads();
+    const int t = s_task;
+    if (t >= ntasks) break;
+    const ResTask T = tasks[t];
+    switch (T.wb) {
+      case 1: res_run<1>(T, lds, s_part, out, tid, nth, lane, wave, nw); break;
+      case 2: res_run<2>(T, lds, s_part, out, tid, nth, lane, wave, nw); break;
+      case 4: res_run<4>(T, lds, s_part, out, tid, nth, lane, wave, nw); break;
+      case 8: res_run<8>(T, lds, s_part, out, tid, nth, lane, wave, nw); break;
+      default: res_run<16>(T, lds, s_part, out, tid, nth, lane, wave, nw); break;
+    }
+  }
+}

@@ -1,0 +1,295 @@
+"""HIP path vs the oracle and the reference's golden outputs, through the C ABI.
+
+Every test compares integers exactly and the ASPL double bit-for-bit.
+"""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import odp_amd as odp
+import oracle
+from goldens import (TEST_GENERAL, TEST_GRID, GOLDEN, build_case, complete_graph, cycle_graph, cycle_sum,
+                     general_fixture, grid_fixture, hypercube, parse_aspl_txt, path_graph, petersen,
+                     symmetric_fixtures, two_triangles, vectors)
+
+pytestmark = pytest.mark.gpu
+
+INT_MAX, LONG_MAX, DBL_MAX = 2**31 - 1, 2**63 - 1, sys.float_info.max
+
+RES = odp.PLAN_RESIDENT
+LEV = odp.PLAN_LEVEL
+# a spread of plans; each must return identical results
+PLANS = [
+    None,
+    dict(plan=RES, wb=1, threads=256),
+    dict(plan=RES, wb=2, threads=1024),
+    dict(plan=RES, wb=16, threads=512),
+    dict(plan=LEV, kw=0, upw=4, seg_major=1, xcd=0),
+    dict(plan=LEV, kw=1, upw=1, seg_major=0, xcd=1),
+    dict(plan=LEV, kw=3, upw=2, seg_major=1, xcd=1),
+    dict(plan=LEV, kw=64, upw=8, seg_major=0, xcd=0),
+]
+
+
+def hip_eval(adj, nodes=None, width=None, height=None, symmetries=1, num_degrees=None, plan=None):
+    if width is None:
+        if symmetries == 1:
+            odp.init_aspl_general(nodes, adj.shape[1], num_degrees)
+        else:
+            odp.init_aspl_general_s(nodes, adj.shape[1], num_degrees, symmetries)
+    else:
+        if symmetries == 1:
+            odp.init_aspl_grid(width, height, adj.shape[1], num_degrees)
+        else:
+            odp.init_aspl_grid_s(width, height, adj.shape[1], num_degrees, symmetries)
+    try:
+        if plan is not None:
+            try:
+                odp.set_plan(**plan)
+            except ValueError:
+                return None      # plan does not fit this graph (LDS)
+        return odp.set_aspl(adj)
+    finally:
+        odp.finalize_aspl()
+
+
+def same(a, b):
+    return a[0] == b[0] and a[1] == b[1] and np.float64(a[2]).tobytes() == np.float64(b[2]).tobytes()
+
+
+@pytest.mark.parametrize("idx", range(len(TEST_GENERAL)))
+@pytest.mark.parametrize("plan", [None, PLANS[4], PLANS[1]])
+def test_golden_general(idx, plan):
+    exp = parse_aspl_txt(os.path.join(GOLDEN, "general", "ASPL.txt"))[idx]
+    n, d, adj = general_fixture(TEST_GENERAL[idx])
+    got = hip_eval(adj, nodes=n, plan=plan)
+    assert got[0] == exp["diameter"] and got[1] == exp["sum"] and f"{got[2]:.10f}" == exp["aspl10"]
+    assert same(got, oracle.evaluate(adj, nodes=n))
+
+
+@pytest.mark.parametrize("idx", range(len(TEST_GRID)))
+@pytest.mark.parametrize("plan", [None, PLANS[4]])
+def test_golden_grid(idx, plan):
+    exp = parse_aspl_txt(os.path.join(GOLDEN, "grid", "ASPL.txt"))[idx]
+    w, h, d, _, adj = grid_fixture(TEST_GRID[idx])
+    got = hip_eval(adj, width=w, height=h, plan=plan)
+    assert got[0] == exp["diameter"] and got[1] == exp["sum"] and f"{got[2]:.10f}" == exp["aspl10"]
+
+
+@pytest.mark.parametrize("plan", [None, PLANS[4], PLANS[5]])
+def test_symmetric_fixtures(plan):
+    fx = symmetric_fixtures()
+    for f in fx["general"]:
+        for s in f["symmetries"]:
+            n, d, adj = general_fixture(f["file"], s)
+            got = hip_eval(adj, nodes=n, symmetries=s, plan=plan)
+            assert (got[0], got[1], repr(got[2])) == (f["diameter"], f["sum"], f["aspl"]), (f["file"], s)
+    for f in fx["grid"]:
+        for s in f["symmetries"]:
+            w, h, d, _, adj = grid_fixture(f["file"], s)
+            got = hip_eval(adj, width=w, height=h, symmetries=s, plan=plan)
+            assert (got[0], got[1], repr(got[2])) == (f["diameter"], f["sum"], f["aspl"]), (f["file"], s)
+
+
+@pytest.mark.parametrize("plan", PLANS)
+def test_closed_forms(plan):
+    for n in (2, 5, 33, 64, 65, 130):
+        got = hip_eval(complete_graph(n), nodes=n, plan=plan)
+        if got is not None:
+            assert got[:2] == (1, n * (n - 1) // 2)
+    for n in (3, 10, 64, 129, 300):
+        got = hip_eval(cycle_graph(n), nodes=n, plan=plan)
+        if got is not None:
+            assert got[:2] == (n // 2, cycle_sum(n))
+    got = hip_eval(petersen(), nodes=10, plan=plan)
+    assert got is None or got[:2] == (2, 75)
+    for k in (3, 7, 10):
+        got = hip_eval(hypercube(k), nodes=1 << k, plan=plan)
+        if got is not None:
+            assert got[:2] == (k, k * 4 ** k // 4)
+    for n in (2, 7, 100, 200):
+        adj, nd = path_graph(n)
+        got = hip_eval(adj, nodes=n, num_degrees=nd, plan=plan)
+        if got is not None:
+            assert got[:2] == (n - 1, (n - 1) * n * (n + 1) // 6)
+
+
+@pytest.mark.parametrize("plan", [None, PLANS[4], PLANS[2]])
+def test_disconnected_sentinels(plan):
+    got = hip_eval(two_triangles(), nodes=6, plan=plan)
+    assert got == (INT_MAX, LONG_MAX, DBL_MAX)
+    # isolated vertex via num_degrees
+    adj = cycle_graph(40)
+    nd = np.full(40, 2, np.int32)
+    adj[3] = [3, 3]
+    nd[3] = 0
+    adj[2, 1] = 1
+    adj[4, 0] = 5
+    got = hip_eval(adj, nodes=40, num_degrees=nd, plan=plan)
+    assert got == (INT_MAX, LONG_MAX, DBL_MAX)
+
+
+SMALL_VECTORS = [c for c in vectors() if (c.get("nodes") or c["width"] * c["height"]) <= 30000]
+
+
+@pytest.mark.parametrize("case", SMALL_VECTORS, ids=lambda c: "-".join(str(v) for k, v in c.items()
+                                                                      if k not in ("diameter", "sum", "aspl")))
+def test_reference_vectors(case):
+    kw, adj = build_case(case)
+    want = (case["diameter"], case["sum"], float.fromhex(case["aspl"]))
+    for plan in (None, PLANS[1], PLANS[4], PLANS[6]):
+        got = hip_eval(adj, plan=plan, **kw)
+        if got is not None:
+            assert same(got, want), (plan, got, want)
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_random_vs_oracle(seed):
+    rng = np.random.default_rng(seed)
+    n = int(rng.integers(20, 1500)) * 2
+    d = int(rng.integers(3, 12))
+    adj = odp.generate_regular(n, d, 100 + seed)
+    want = oracle.evaluate(adj, nodes=n)
+    for plan in PLANS:
+        got = hip_eval(adj, nodes=n, plan=plan)
+        if got is not None:
+            assert same(got, want), (plan, got, want)
+
+
+def test_symmetric_generated_vs_expanded():
+    for (n, d, s, seed) in ((600, 4, 3, 1), (1024, 6, 4, 2), (2048, 5, 8, 3)):
+        base = odp.generate_general_s(n, d, s, seed)
+        full = oracle.expand(base, nodes=n, symmetries=s)
+        want = oracle.evaluate(full, nodes=n)
+        for plan in (None, PLANS[4], PLANS[5]):
+            got = hip_eval(base, nodes=n, symmetries=s, plan=plan)
+            assert same(got, want), (n, s, plan)
+    for (w, dg, L, s, seed) in ((20, 4, 3, 4, 1), (24, 6, 4, 2, 2), (40, 8, 4, 4, 3)):
+        full = odp.generate_grid(w, w, dg, L, s, seed)
+        base = odp.grid_base_table(w, w, dg, s, full)
+        want = oracle.evaluate(full, width=w, height=w)
+        for plan in (None, PLANS[4], PLANS[5], PLANS[3]):
+            got = hip_eval(base, width=w, height=w, symmetries=s, plan=plan)
+            if got is not None:
+                assert same(got, want), (w, s, plan)
+
+
+def test_nonregular_symmetric_num_degrees():
+    # sample/graph/grid/w10h10d4r4g4 with a vertex-orbit trimmed through num_degrees
+    w, h, d, _, base = grid_fixture("w10h10d4r4g4", 4)
+    nd = np.full(base.shape[0], d, np.int32)
+    full = oracle.expand(base, width=w, height=h, symmetries=4, num_degrees=nd)
+    want = oracle.evaluate(base, width=w, height=h, symmetries=4, num_degrees=nd)
+    assert same(want, oracle.evaluate(full, width=w, height=h))
+    got = hip_eval(base, width=w, height=h, symmetries=4, num_degrees=nd)
+    assert same(got, want)
+
+
+def test_repeated_lifecycle():
+    adj = odp.generate_regular(500, 5, 3)
+    want = oracle.evaluate(adj, nodes=500)
+    for _ in range(5):
+        assert same(hip_eval(adj, nodes=500), want)
+        w, h, d, _, g = grid_fixture("w6h6d3r2g4", 4)
+        assert hip_eval(g, width=w, height=h, symmetries=4)[:2] == (5, 1978)
+
+
+def test_repeat_calls_same_init():
+    n, d = 3000, 7
+    adj = odp.generate_regular(n, d, 9) if (n * d) % 2 == 0 else None
+    n, d = 3000, 8
+    adj = odp.generate_regular(n, d, 9)
+    rng = odp.Rng(5)
+    for plan in (None, PLANS[4]):
+        odp.init_aspl_general(n, d)
+        if plan:
+            odp.set_plan(**plan)
+        a = adj.copy()
+        for _ in range(4):
+            odp.two_opt(a, rng)
+            assert same(odp.set_aspl(a), oracle.evaluate(a, nodes=n))
+        odp.finalize_aspl()
+
+
+def test_device_swap_matches_host():
+    n, d = 2000, 6
+    adj = odp.generate_regular(n, d, 21)
+    rng = odp.Rng(77)
+    for plan in (None, PLANS[4]):
+        odp.init_aspl_general(n, d)
+        if plan:
+            odp.set_plan(**plan)
+        host = adj.copy()
+        odp.upload_adjacency(host)
+        for _ in range(5):
+            a, b, c, dd = odp.two_opt(host, rng)
+            assert odp.swap_device(a, b, c, dd) == 0
+            assert same(odp.set_aspl_resident(), oracle.evaluate(host, nodes=n))
+        assert odp.swap_device(0, 0, 0, 0) == -1      # missing edge is rejected
+        odp.finalize_aspl()
+
+
+def test_autotune_consistent():
+    n, d = 4096, 8
+    adj = odp.generate_regular(n, d, 5)
+    want = oracle.evaluate(adj, nodes=n)
+    odp.init_aspl_general(n, d)
+    tried = odp.autotune(adj, repeats=1)      # exits the process if two plans disagree
+    assert tried > 4
+    assert same(odp.set_aspl(adj), want)
+    odp.finalize_aspl()
+
+
+def _batch_graphs(seed, count, nmax=3000):
+    rng = np.random.default_rng(seed)
+    graphs = []
+    for g in range(count):
+        n = int(rng.integers(8, nmax // 2)) * 2
+        d = int(rng.choice([3, 4, 8, 12, 16, 24]))
+        d = min(d, n - 1)
+        if (n * d) % 2:
+            d -= 1
+        adj = odp.generate_regular(n, d, 1000 * seed + g)
+        odp.two_opt(adj, odp.Rng(g))
+        graphs.append(adj)
+    return graphs
+
+
+def test_batch_vs_oracle():
+    graphs = _batch_graphs(1, 48)
+    graphs.append(two_triangles())                       # a disconnected one in the mix
+    graphs.append(complete_graph(40))
+    packed = odp.pack(graphs)
+    d, s, a = odp.batch_eval(packed)
+    for g, adj in enumerate(graphs):
+        want = oracle.evaluate(adj, nodes=adj.shape[0])
+        assert same((int(d[g]), int(s[g]), float(a[g])), want), g
+    d2, s2, a2 = odp.loop_eval(packed, odp.PLAN_LEVEL)
+    assert (d2 == d).all() and (s2 == s).all() and (a2.view(np.int64) == a.view(np.int64)).all()
+    d3, s3, a3 = odp.loop_eval(packed, odp.PLAN_AUTO)
+    assert (d3 == d).all() and (s3 == s).all()
+    # alone vs in the mix
+    for g in (0, 17, 48):
+        one = odp.batch_eval(odp.pack([graphs[g]]))
+        assert one[0][0] == d[g] and one[1][0] == s[g] and one[2][0] == a[g]
+
+
+def test_batch_large_graph_falls_back():
+    graphs = [odp.generate_regular(24000, 4, 3), odp.generate_regular(500, 6, 4)]
+    d, s, a = odp.batch_eval(odp.pack(graphs))
+    for g, adj in enumerate(graphs):
+        assert same((int(d[g]), int(s[g]), float(a[g])), oracle.evaluate(adj, nodes=adj.shape[0]))
+
+
+def test_invalid_vertex_id_exits():
+    code = (
+        "import sys; sys.path.insert(0, %r)\n"
+        "import numpy as np, odp_amd as o\n"
+        "a = np.array([[1, 2], [0, 2], [0, 7]], np.int32)\n"
+        "o.init_aspl_general(3, 2)\n"
+        "o.set_aspl(a)\n" % os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 1 and "outside [0, nodes)" in r.stderr
