@@ -169,7 +169,7 @@ struct Evaluator {
     if (adj_elems() > stage_cap) { hfree(&h_stage); halloc(&h_stage, adj_elems()); stage_cap = adj_elems(); }
     size_t ne = (size_t)n * dpad;
     if (ne > nbr_cap) { dfree(&d_nbr); dalloc(&d_nbr, ne); nbr_cap = ne; }
-    if (!d_err) { dalloc(&d_err, 1); halloc(&h_err, 1); HIPC(hipMemset(d_err, 0, sizeof(int))); }
+    if (!d_err) { dalloc(&d_err, 1); halloc(&h_err, 1); HIPC(hipMemsetAsync(d_err, 0, sizeof(int), C.stream)); }
     dfree(&d_numdeg);
     dfree(&d_itable);
     avg_deg = deg;
@@ -348,7 +348,7 @@ struct Evaluator {
     if (*h_err) {
       int e = *h_err;
       *h_err = 0;
-      HIPC(hipMemset(d_err, 0, sizeof(int)));
+      HIPC(hipMemsetAsync(d_err, 0, sizeof(int), C.stream));
       if (e & 2) ODP_ERROR("num_degrees holds a value outside [0, degree]\n");
       ODP_ERROR("adjacency holds a vertex id outside [0, nodes) within num_degrees\n");
     }
@@ -753,7 +753,8 @@ extern "C" int ODPX_Swap_device(int a, int b, int c, int d)
   HIPC(hipStreamSynchronize(C.stream));
   if (*v.h_err & 4) {
     *v.h_err = 0;
-    HIPC(hipMemset(v.d_err, 0, sizeof(int)));
+    HIPC(hipMemsetAsync(v.d_err, 0, sizeof(int), C.stream));
+    HIPC(hipStreamSynchronize(C.stream));
     return -1;
   }
   return 0;
@@ -912,7 +913,7 @@ extern "C" int ODPX_Batch_eval(int ngraphs, const int* nodes, const int* degrees
       dalloc(&B.d_g, small.size()); dalloc(&B.d_out, small.size()); halloc(&B.h_out, small.size());
       B.g_cap = small.size();
     }
-    if (!B.d_err) { dalloc(&B.d_err, 1); dalloc(&B.d_qhead, 1); halloc(&B.h_err, 1); HIPC(hipMemset(B.d_err, 0, 4)); }
+    if (!B.d_err) { dalloc(&B.d_err, 1); dalloc(&B.d_qhead, 1); halloc(&B.h_err, 1); HIPC(hipMemsetAsync(B.d_err, 0, 4, C.stream)); }
     for (size_t k = 0; k < small.size(); k++) {
       const BatchGraph& G = hg[k];
       int wb = batch_wb(G.n), b = 32 * wb, nt = (G.n + b - 1) / b;

@@ -463,7 +463,7 @@ __device__ __forceinline__ void res_run(const ResTask& T, u32* lds, u64 (*s_part
 __global__ __launch_bounds__(1024) void k_resident(const ResTask* __restrict__ tasks, int ntasks,
                                                    int* __restrict__ qhead, ResOut* __restrict__ out)
 {
-  extern __shared__ u32 lds[];
+  extern __shared__ __attribute__((aligned(16))) u32 lds[];
   __shared__ int s_task;
   __shared__ u64 s_part[2][16];
   const int tid = threadIdx.x, nth = blockDim.x, lane = tid & 63, wave = tid >> 6, nw = nth >> 6;
