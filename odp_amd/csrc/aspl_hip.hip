@@ -38,6 +38,14 @@ static const size_t LDS_DYN_MAX = LDS_LIMIT - LDS_STATIC;
 static const int RING = 260;                    // per-level counter slots kept on the device
 static const int MAX_BATCH_LEVELS = 256;        // levels launched between two host reads (< RING - 2)
 
+// ODP_HIP_TRACE=1 prints each host step to stderr (diagnostics)
+static int g_trace = -1;
+static void trace(const char* what)
+{
+  if (g_trace < 0) { const char* v = getenv("ODP_HIP_TRACE"); g_trace = v && atoi(v) == 1; }
+  if (g_trace) { fprintf(stderr, "[libodp_hip] %s\n", what); fflush(stderr); }
+}
+
 static double wall_time()
 {
   struct timeval t;
@@ -56,6 +64,7 @@ static struct { bool ready; hipStream_t stream; int ncu; } C;
 static void ctx_init()
 {
   if (C.ready) return;
+  trace("ctx_init");
   int dev;
   HIPC(hipGetDevice(&dev));
   hipDeviceProp_t prop;
@@ -64,6 +73,7 @@ static void ctx_init()
   HIPC(hipStreamCreateWithFlags(&C.stream, hipStreamNonBlocking));
   HIPC(hipFuncSetAttribute((const void*)k_resident, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_DYN_MAX));
   C.ready = true;
+  trace("ctx ready");
 }
 
 template <typename T> static void dalloc(T** p, size_t count)
@@ -190,6 +200,7 @@ struct Evaluator {
 
   void upload(const void* adjacency)
   {
+    trace("upload");
     memcpy(h_stage, adjacency, sizeof(int) * adj_elems());
     HIPC(hipMemcpyAsync(d_adj, h_stage, sizeof(int) * adj_elems(), hipMemcpyHostToDevice, C.stream));
   }
@@ -358,6 +369,7 @@ struct Evaluator {
   void run_level(u64* excess, int* diam, bool* disc)
   {
     LevelRun& r = lr;
+    trace("run_level");
     prepare(r.cnt, r.ntasks, r.done, r.ntasks);   // ring slot 0 and the done flags
     {
       TimedLaunch tl;
@@ -391,6 +403,7 @@ struct Evaluator {
       HIPC(hipMemcpyAsync(r.h_cnt, r.cnt, sizeof(u64) * RING * r.ntasks, hipMemcpyDeviceToHost, C.stream));
       HIPC(hipMemcpyAsync(h_err, d_err, sizeof(int), hipMemcpyDeviceToHost, C.stream));
       HIPC(hipStreamSynchronize(C.stream));
+      trace("level batch synchronized");
       check_err();
       for (int t = 0; t < r.ntasks; t++) {
         St& q = st[t];
@@ -440,15 +453,18 @@ struct Evaluator {
   void run_resident(u64* excess, int* diam, bool* disc)
   {
     ResRun& r = rr;
+    trace("run_resident");
     prepare((u64*)r.out, (int)(sizeof(ResOut) / 8), (u32*)r.qhead, 1);
     TimedLaunch tl;
     timed_begin(tl, T_RESIDENT, 0);
     hipLaunchKernelGGL(k_resident, dim3(r.grid), dim3(r.threads), r.lds, C.stream, r.tasks, r.ntasks, r.qhead, r.out);
     HIPC(hipGetLastError());
     timed_end(tl);
+    trace("resident launched");
     HIPC(hipMemcpyAsync(r.h_out, r.out, sizeof(ResOut), hipMemcpyDeviceToHost, C.stream));
     HIPC(hipMemcpyAsync(h_err, d_err, sizeof(int), hipMemcpyDeviceToHost, C.stream));
     HIPC(hipStreamSynchronize(C.stream));
+    trace("resident synchronized");
     check_err();
     *excess = r.h_out->sum;
     *diam = std::max(1, r.h_out->diam);

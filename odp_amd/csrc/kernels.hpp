@@ -368,14 +368,28 @@ __device__ __forceinline__ void row_bit(RowT<WB>& r, int c, int l)
   }
 }
 
+// No divergent branch may sit in a loop that contains a barrier.  With an
+// `if (lane == 0)` around the queue pop, the compiler structurized the block
+// loop per lane: lane 0 left the loop to pop the next block while the other
+// lanes re-entered the barrier and re-read the old block id -- a livelock.  So
+// every loop-control value is made wave-uniform with readfirstlane, and
+// single-lane side effects are done by whole waves with zero contributions
+// from the other lanes.
+__device__ __forceinline__ u64 uniform_u64(u64 v)
+{
+  const u32 lo = __builtin_amdgcn_readfirstlane((u32)v);
+  const u32 hi = __builtin_amdgcn_readfirstlane((u32)(v >> 32));
+  return ((u64)hi << 32) | lo;
+}
+
 __device__ __forceinline__ u64 block_total(u64 v, u64* part, int lane, int wave, int nw)
 {
-  v = wave_sum(v);
-  if (lane == 0) part[wave] = v;
+  v = wave_sum(v);      // every lane holds the wave total: all write the same value
+  part[wave] = v;
   __syncthreads();
   u64 s = 0;
   for (int i = 0; i < nw; i++) s += part[i];
-  return s;
+  return uniform_u64(s);
 }
 
 // All levels of one block of T.l <= 32*WB sources, both bit buffers in LDS.
@@ -448,11 +462,12 @@ __device__ __forceinline__ void res_run(const ResTask& T, u32* lds, u64 (*s_part
     par ^= 1;
     c = block_total(pop, s_part[par], lane, wave, nw);
   }
-  if (tid == 0) {
-    if (sum) atomicAdd(&out[T.graph].sum, sum);
+  if (wave == 0) {       // wave-uniform branch; only lane 0 contributes
+    const bool first = lane == 0;
+    atomicAdd(&out[T.graph].sum, first ? sum : 0ULL);
     atomicMax(&out[T.graph].diam, D);
-    if (disc) atomicOr(&out[T.graph].flags, 1u);
-    atomicAdd(&out[T.graph].levels, (u32)(L + 1));
+    atomicOr(&out[T.graph].flags, disc ? 1u : 0u);
+    atomicAdd(&out[T.graph].levels, first ? (u32)(L + 1) : 0u);
   }
 }
 
@@ -466,11 +481,15 @@ __global__ __launch_bounds__(1024) void k_resident(const ResTask* __restrict__ t
   extern __shared__ __attribute__((aligned(16))) u32 lds[];
   __shared__ int s_task;
   __shared__ u64 s_part[2][16];
-  const int tid = threadIdx.x, nth = blockDim.x, lane = tid & 63, wave = tid >> 6, nw = nth >> 6;
+  const int tid = threadIdx.x, nth = blockDim.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), nw = nth >> 6;
   for (;;) {
-    if (tid == 0) s_task = atomicAdd(qhead, 1);
+    if (wave == 0) {       // wave-uniform: lane 0 adds 1, the others add 0
+      const int got = atomicAdd(qhead, lane == 0 ? 1 : 0);
+      s_task = __builtin_amdgcn_readfirstlane(got);
+    }
     __syncthreads();
-    const int t = s_task;
+    const int t = __builtin_amdgcn_readfirstlane(s_task);
     if (t >= ntasks) break;
     const ResTask T = tasks[t];
     switch (T.wb) {
