@@ -1,0 +1,205 @@
+"""Benchmark of the ASPL/diameter evaluator on one MI355X.
+
+Default workload: one seeded random 16-regular simple graph with n=65536,
+evaluated with the MATRIX method (a full 65536 x 65536 bit matrix, 512 MiB per
+buffer).  A step is one full evaluation (every BFS level, exact sum, diameter
+and ASPL) of the adjacency already resident in HBM.  The JSON line's value is
+evaluations per second.
+
+roofline: the dominant kernel (the gather level, k_level) timed with HIP
+events recorded on the library's launch stream around every launch in the
+timed region; achieved = algorithmic bytes of those launches (rows * ((d+2) *
+k * 8 + 4 d) per level, k = 64-bit words per row) / their summed duration.
+traffic: measured HBM bytes per launch, from rocprofv3 counter runs (see
+profiles/), or null.
+
+cpu_baseline: the reference library's OpenMP build (oracle/_ref, compiled from
+the reference sources) timed on this host on the same graph.
+
+Extra objects: "batched" (1024 seeded 2-opt candidates with n in [1024, 16384]
+and d in {4, 8, 12, 16, 24}, one persistent launch, against looping the
+single-graph GPU path) unless --no-extras.
+"""
+import argparse
+import json
+import os
+import statistics
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import numpy as np  # noqa: E402
+
+import odp_amd as odp  # noqa: E402
+
+METRIC = "ASPL evals/s (bit-exact) at n=65536 d=16 and batched 2-opt; HBM GB/s vs peak"
+HBM_PEAK_GBS = 8000.0
+# expected result of the workload graph, from the reference library (tests/golden/vectors.json)
+EXPECT = {(65536, 16, 1): (6, 9346029721, float.fromhex("0x1.1689a2d622d62p+2"))}
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(adj, n, budget_s=20.0):
+    """Reference OpenMP library on a bounded sample of the same workload."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    if not oracle.ref_available(threads=True):
+        return None, None
+    cores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    (d, s, a), t1 = oracle.run_reference(adj, nodes=n, kind=oracle.MATRIX, threads=True, repeat=1,
+                                         omp_threads=cores)
+    per = t1[0] if t1 else 1.0
+    reps = max(2, min(30, int(budget_s / max(per, 1e-3))))
+    (d, s, a), ts = oracle.run_reference(adj, nodes=n, kind=oracle.MATRIX, threads=True, repeat=reps,
+                                         omp_threads=cores)
+    med = statistics.median(ts)
+    return ({"value": 1.0 / med, "unit": "evals/s", "cores": cores, "kind": "reference",
+             "sample": f"{reps} timed ODP_Set_aspl calls (after 1 warm-up) of the reference OpenMP library, "
+                       f"MATRIX, on the same n={n} d={adj.shape[1]} graph; median {med:.3f} s"},
+            (d, s, a))
+
+
+def c4_graphs(count, seed=2024):
+    rng = np.random.default_rng(seed)
+    graphs = []
+    for g in range(count):
+        n = int(rng.integers(512, 8193)) * 2
+        d = int(rng.choice([4, 8, 12, 16, 24]))
+        adj = odp.generate_regular(n, d, seed * 100000 + g)
+        odp.two_opt(adj, odp.Rng(seed * 100000 + g + 7))
+        graphs.append(adj)
+    return graphs
+
+
+def batched_extra(count, loops):
+    t0 = time.perf_counter()
+    graphs = c4_graphs(count)
+    packed = odp.pack(graphs)
+    log(f"batched: generated {count} graphs in {time.perf_counter() - t0:.1f}s")
+    ref = odp.batch_eval(packed)      # warm-up, and the results every other path must match
+    times = []
+    for _ in range(loops):
+        t = time.perf_counter()
+        r = odp.batch_eval(packed)
+        times.append(time.perf_counter() - t)
+        assert all((x == y).all() for x, y in zip(r, ref))
+    tb = min(times)
+    t = time.perf_counter()
+    lp = odp.loop_eval(packed, odp.PLAN_LEVEL)
+    tl = time.perf_counter() - t
+    t = time.perf_counter()
+    la = odp.loop_eval(packed, odp.PLAN_AUTO)
+    ta = time.perf_counter() - t
+    same = all((x == y).all() for x, y in zip(lp, ref)) and all((x == y).all() for x, y in zip(la, ref))
+    return {"graphs": count, "evals_per_s": count / tb, "ms_per_batch": 1e3 * tb,
+            "loop_level_evals_per_s": count / tl, "loop_auto_evals_per_s": count / ta,
+            "speedup_vs_loop_level": tl / tb, "speedup_vs_loop_auto": ta / tb,
+            "identical_results": bool(same),
+            "workload": "1024 seeded random d-regular graphs, n uniform even in [1024,16384], "
+                        "d in {4,8,12,16,24}, one 2-opt swap each"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--nodes", type=int, default=65536)
+    ap.add_argument("--degree", type=int, default=16)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--plan", default="matrix", help="matrix | auto | level:<kw> | resident:<wb>:<threads>")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-extras", action="store_true")
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic.json"))
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the path shards by "
+              f"source-column blocks, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    n, d = args.nodes, args.degree
+    t0 = time.perf_counter()
+    adj = odp.generate_regular(n, d, args.seed)
+    log(f"graph n={n} d={d} seed={args.seed} generated in {time.perf_counter() - t0:.2f}s")
+
+    odp.init_aspl_general(n, d)
+    if args.plan == "matrix":
+        odp.set_plan(plan=odp.PLAN_LEVEL, kw=0, upw=4, seg_major=1, xcd=0)
+    elif args.plan.startswith("level:"):
+        f = args.plan.split(":")
+        odp.set_plan(plan=odp.PLAN_LEVEL, kw=int(f[1]), upw=int(f[2]) if len(f) > 2 else 4,
+                     seg_major=int(f[3]) if len(f) > 3 else 1, xcd=int(f[4]) if len(f) > 4 else 0)
+    elif args.plan.startswith("resident:"):
+        f = args.plan.split(":")
+        odp.set_plan(plan=odp.PLAN_RESIDENT, wb=int(f[1]), threads=int(f[2]))
+    plan = odp.get_plan()
+    first = odp.set_aspl(adj)                      # uploads and evaluates (PCIe-inclusive)
+    want = EXPECT.get((n, d, args.seed))
+    if want is not None and (first[0], first[1]) != want[:2]:
+        raise SystemExit(f"result {first} differs from the reference's {want}")
+    log(f"first evaluation: diameter={first[0]} sum={first[1]} ASPL={first[2]!r}; plan {plan}")
+
+    for _ in range(args.warmup):
+        odp.set_aspl_resident()
+    odp.stats_enable(True)
+    odp.stats_reset()
+    t = time.perf_counter()
+    for i in range(args.steps):
+        r = odp.set_aspl_resident()
+        assert r == first
+    el = time.perf_counter() - t
+    st = odp.stats()
+    odp.stats_enable(False)
+    log(f"timed {args.steps} evaluations in {el:.3f}s; kernel stats {st}")
+
+    # PCIe-inclusive rate of the drop-in call (host adjacency every call), for DESIGN.md only
+    t = time.perf_counter()
+    for _ in range(3):
+        odp.set_aspl(adj)
+    pcie_rate = 3 / (time.perf_counter() - t)
+    odp.finalize_aspl()
+
+    level_ms = st["level_ms"] / max(1, st["level_launches"])
+    achieved = (st["level_bytes"] / max(1, st["level_launches"])) / (level_ms * 1e-3) / 1e9 if level_ms > 0 else 0.0
+    traffic = None
+    if os.path.exists(args.traffic):
+        tr = json.load(open(args.traffic)).get(f"{n}x{d}:{args.plan}")
+        if tr:
+            traffic = tr["bytes_per_launch"]
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                "kernel": "k_level", "launch_ms": round(level_ms, 4),
+                "alg_bytes_per_launch": st["level_bytes"] / max(1, st["level_launches"]),
+                "launches": st["level_launches"]}
+
+    out = {"metric": METRIC, "value": round(args.steps / el, 3), "unit": "evals/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * el / args.steps, 4),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
+           "data": "synthetic",
+           "config": {"workload": f"single seeded random {d}-regular simple graph, n={n}, MATRIX "
+                                  f"(full bit matrix), adjacency resident in HBM",
+                      "nodes": n, "degree": d, "seed": args.seed, "plan": plan,
+                      "diameter": first[0], "sum": first[1], "aspl": first[2],
+                      "pcie_inclusive_evals_per_s": round(pcie_rate, 3)},
+           "roofline": roofline}
+    if not args.no_cpu:
+        cb, cres = cpu_baseline(adj, n)
+        if cb is not None:
+            if cres[:2] != first[:2] or cres[2] != first[2]:
+                raise SystemExit(f"reference {cres} differs from HIP {first}")
+            cb["matches_gpu_result"] = True
+        out["cpu_baseline"] = cb
+    else:
+        out["cpu_baseline"] = None
+    if not args.no_extras:
+        out["batched"] = batched_extra(1024, 3)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()
