@@ -112,7 +112,8 @@ def main():
     ap.add_argument("--nodes", type=int, default=65536)
     ap.add_argument("--degree", type=int, default=16)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--plan", default="matrix", help="matrix | auto | level:<kw> | resident:<wb>:<threads>")
+    ap.add_argument("--plan", default="matrix",
+                    help="matrix | matrix-rows | auto | level:<kw>[:upw:seg_major:xcd:group] | resident:<wb>:<threads>")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic.json"))
@@ -129,11 +130,16 @@ def main():
 
     odp.init_aspl_general(n, d)
     if args.plan == "matrix":
-        odp.set_plan(plan=odp.PLAN_LEVEL, kw=0, upw=4, seg_major=1, xcd=0)
+        # every column block resident (the full bit matrix), 64-word slabs, XCD-owned
+        odp.set_plan(plan=odp.PLAN_LEVEL, kw=64, upw=8, seg_major=0, xcd=1, group=0)
+    elif args.plan == "matrix-rows":
+        # the full bit matrix with whole 8 KiB rows (the reference's layout)
+        odp.set_plan(plan=odp.PLAN_LEVEL, kw=0, upw=4, seg_major=1, xcd=0, group=0)
     elif args.plan.startswith("level:"):
         f = args.plan.split(":")
-        odp.set_plan(plan=odp.PLAN_LEVEL, kw=int(f[1]), upw=int(f[2]) if len(f) > 2 else 4,
-                     seg_major=int(f[3]) if len(f) > 3 else 1, xcd=int(f[4]) if len(f) > 4 else 0)
+        odp.set_plan(plan=odp.PLAN_LEVEL, kw=int(f[1]), upw=int(f[2]) if len(f) > 2 else 8,
+                     seg_major=int(f[3]) if len(f) > 3 else 0, xcd=int(f[4]) if len(f) > 4 else 1,
+                     group=int(f[5]) if len(f) > 5 else 0)
     elif args.plan.startswith("resident:"):
         f = args.plan.split(":")
         odp.set_plan(plan=odp.PLAN_RESIDENT, wb=int(f[1]), threads=int(f[2]))
@@ -182,7 +188,7 @@ def main():
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
            "data": "synthetic",
            "config": {"workload": f"single seeded random {d}-regular simple graph, n={n}, MATRIX "
-                                  f"(full bit matrix), adjacency resident in HBM",
+                                  f"(full bit matrix, every column block resident), adjacency resident in HBM",
                       "nodes": n, "degree": d, "seed": args.seed, "plan": plan,
                       "diameter": first[0], "sum": first[1], "aspl": first[2],
                       "pcie_inclusive_evals_per_s": round(pcie_rate, 3)},

@@ -48,6 +48,7 @@ typedef struct {
   int upw;        /* level: row segments per wave per launch                         */
   int seg_major;  /* level: 1 = all rows of one column segment before the next        */
   int xcd;        /* level: 1 = each XCD takes one contiguous range of workgroups     */
+  int group;      /* level: column blocks resident at once; 0 = all (MATRIX)          */
 } ODPX_Plan;
 
 /* Plan of the current evaluator (after Init).  Set: 0 on success, -1 if invalid.

@@ -31,6 +31,7 @@
 
 static const int KIND_MATRIX = 1, KIND_SAVING = 2;
 static const int GPU_CHUNK_WORDS = 64;          // column-block width of MATRIX_SAVING (parameter.h:13)
+static const int SAVING_GROUP = 8;              // MATRIX_SAVING blocks resident at once (one per XCD)
 static const double MEM_THRESHOLD_MB = 2048.0;  // MATRIX -> MATRIX_SAVING switch (parameter.h:4)
 static const size_t LDS_LIMIT = 163840;         // LDS a gfx950 workgroup may allocate
 static const size_t LDS_STATIC = 512;           // k_resident's own __shared__ data, rounded up
@@ -124,7 +125,7 @@ static void timed_end(TimedLaunch& t)
 // one graph on the device, the row-neighbour table and its plan
 
 struct LevelRun {
-  int n = 0, ntasks = 0, kw = 0, kwp = 0, upw = 4, seg_major = 1, xcd = 0, urow = 0, total_wg = 0;
+  int n = 0, ntasks = 0, group = 0, kw = 0, kwp = 0, upw = 4, seg_major = 1, xcd = 0, urow = 0, per_task_wg = 0;
   size_t buf_words = 0;                 // capacity of bufs (64-bit words)
   size_t task_cap = 0;
   u64* bufs = nullptr; u64* cnt = nullptr; u32* done = nullptr;
@@ -230,12 +231,18 @@ struct Evaluator {
       p.threads = n >= 4096 ? 1024 : 256;
       return p;
     }
+    // Column blocks of 64 words (4096 sources), each stored as its own
+    // contiguous n x 512 B slab, with each XCD owning whole slabs.  MATRIX
+    // keeps every block resident and advances all of them per launch (the full
+    // bit matrix, blocked); MATRIX_SAVING keeps SAVING_GROUP blocks resident
+    // at a time, bounding memory as the reference does (aspl.c:77-119).
     p.plan = ODPX_PLAN_LEVEL;
     int W = (nsrc + 63) / 64;
-    p.kw = (kind == KIND_MATRIX) ? 0 : std::min(GPU_CHUNK_WORDS, W);
-    p.upw = 4;
-    p.seg_major = grid_s ? 0 : 1;
-    p.xcd = grid_s ? 1 : 0;
+    p.kw = std::min(GPU_CHUNK_WORDS, W);
+    p.upw = 8;
+    p.seg_major = 0;
+    p.xcd = 1;
+    p.group = (kind == KIND_MATRIX) ? 0 : SAVING_GROUP;
     return p;
   }
 
@@ -246,7 +253,7 @@ struct Evaluator {
       if (p.threads != 256 && p.threads != 512 && p.threads != 1024) return false;
       return (size_t)2 * n * p.wb * 4 <= LDS_DYN_MAX;
     }
-    if (p.plan == ODPX_PLAN_LEVEL) return p.kw >= 0 && p.upw >= 1 && p.upw <= 64;
+    if (p.plan == ODPX_PLAN_LEVEL) return p.kw >= 0 && p.upw >= 1 && p.upw <= 64 && p.group >= 0;
     return false;
   }
 
@@ -257,16 +264,17 @@ struct Evaluator {
     int W = (nsrc + 63) / 64;
     int kw = plan.kw > 0 ? std::min(plan.kw, W) : W;
     int kwp = kw <= 128 ? (1 << (ilog2(std::max(kw, 2) - 1) + 1)) : round_up(kw, 128);
-    if (kw == 2 || kw == 1) kwp = 2;
     if (kwp < kw) kwp = kw;
     int ntasks = (W + kw - 1) / kw;
+    int group = plan.group > 0 ? std::min(plan.group, ntasks) : ntasks;
     int lanes = std::min(64, kwp / 2);
     int lpr_log2 = ilog2(lanes);
     int rpw = 64 >> lpr_log2;
     int segs = kwp / (2 * lanes);
     int rgroups = (n + rpw - 1) / rpw;
     int units = rgroups * segs;
-    size_t words = (size_t)ntasks * 2 * n * kwp;
+    // bit buffers for `group` column blocks; group g of blocks reuses them
+    size_t words = (size_t)group * 2 * n * kwp;
     if (words > r.buf_words) { dfree(&r.bufs); dalloc(&r.bufs, words); r.buf_words = words; }
     if ((size_t)ntasks > r.task_cap) {
       dfree(&r.cnt); dfree(&r.done); dfree(&r.tasks); dfree(&r.wg_start); hfree(&r.h_cnt);
@@ -276,17 +284,16 @@ struct Evaluator {
       r.task_cap = ntasks;
     }
     // the previous contents are never read: level 0 rewrites every word it uses
-    r.n = n; r.ntasks = ntasks; r.kw = kw; r.kwp = kwp;
+    r.n = n; r.ntasks = ntasks; r.group = group; r.kw = kw; r.kwp = kwp;
     r.upw = plan.upw; r.seg_major = plan.seg_major; r.xcd = plan.xcd; r.urow = (lpr_log2 == 6);
-    int per_task_wg = (units + 4 * r.upw - 1) / (4 * r.upw);
-    r.total_wg = per_task_wg * ntasks;
+    r.per_task_wg = (units + 4 * r.upw - 1) / (4 * r.upw);
     std::vector<LevelTask> ht(ntasks);
     std::vector<int> ws(ntasks + 1);
     r.l.assign(ntasks, 0);
     for (int t = 0; t < ntasks; t++) {
       LevelTask& T = ht[t];
       T.nbr = d_nbr;
-      T.buf0 = r.bufs + (size_t)t * 2 * n * kwp;
+      T.buf0 = r.bufs + (size_t)(t % group) * 2 * n * kwp;
       T.buf1 = T.buf0 + (size_t)n * kwp;
       T.cnt = r.cnt + t;
       T.done = r.done + t;
@@ -296,9 +303,9 @@ struct Evaluator {
       T.lpr_log2 = lpr_log2; T.segs = segs; T.rgroups = rgroups; T.units = units;
       T.seg_major = r.seg_major; T.cstride = ntasks; T.pad = 0;
       r.l[t] = T.l;
-      ws[t] = t * per_task_wg;
+      ws[t] = t * r.per_task_wg;     // blocks are equal: the starts of a group rebase to 0
     }
-    ws[ntasks] = r.total_wg;
+    ws[ntasks] = ntasks * r.per_task_wg;
     HIPC(hipMemcpy(r.tasks, ht.data(), sizeof(LevelTask) * ntasks, hipMemcpyHostToDevice));
     HIPC(hipMemcpy(r.wg_start, ws.data(), sizeof(int) * (ntasks + 1), hipMemcpyHostToDevice));
   }
@@ -365,89 +372,103 @@ struct Evaluator {
     }
   }
 
-  // returns the ordered-pair excess over n(n-1) and the diameter; disconnected -> *disc
+  // returns the ordered-pair excess over n(n-1) and the diameter; disconnected -> *disc.
+  // Column blocks run `group` at a time (all at once for MATRIX); every launch
+  // advances each block of the group by one level.
   void run_level(u64* excess, int* diam, bool* disc)
   {
     LevelRun& r = lr;
     trace("run_level");
     prepare(r.cnt, r.ntasks, r.done, r.ntasks);   // ring slot 0 and the done flags
-    {
-      TimedLaunch tl;
-      timed_begin(tl, T_OTHER, 0);
-      hipLaunchKernelGGL(k_level0, dim3(r.total_wg), dim3(LEVEL_WG), 0, C.stream,
-                         r.tasks, r.wg_start, r.ntasks, RING, r.upw, r.xcd);
-      HIPC(hipGetLastError());
-      timed_end(tl);
-    }
     struct St { int running; u64 sum, prev; int lwork; };
     std::vector<St> st(r.ntasks, St{1, 0, 0, 0});
-    int launched = 1, examined = 0, running = r.ntasks;
-    int first = guess_levels > 0 ? guess_levels : 8;
-    int batch_end = std::min(n, std::max(2, first));
     bool any_disc = false;
-    int max_d = 1;
-    for (;;) {
-      for (int L = launched; L < batch_end; L++) {
+    int max_d = 1, lmax = 0;
+    for (int first = 0; first < r.ntasks; first += r.group) {
+      const int cnt_t = std::min(r.group, r.ntasks - first);
+      const LevelTask* tasks = r.tasks + first;
+      const int grid = cnt_t * r.per_task_wg;
+      {
         TimedLaunch tl;
-        timed_begin(tl, T_LEVEL, L);
-        if (r.urow)
-          hipLaunchKernelGGL(k_level<true>, dim3(r.total_wg), dim3(LEVEL_WG), 0, C.stream,
-                             r.tasks, r.wg_start, r.ntasks, L, RING, r.upw, r.xcd);
-        else
-          hipLaunchKernelGGL(k_level<false>, dim3(r.total_wg), dim3(LEVEL_WG), 0, C.stream,
-                             r.tasks, r.wg_start, r.ntasks, L, RING, r.upw, r.xcd);
+        timed_begin(tl, T_OTHER, 0);
+        hipLaunchKernelGGL(k_level0, dim3(grid), dim3(LEVEL_WG), 0, C.stream,
+                           tasks, r.wg_start, cnt_t, RING, r.upw, r.xcd);
         HIPC(hipGetLastError());
         timed_end(tl);
       }
-      launched = std::max(launched, batch_end);
-      HIPC(hipMemcpyAsync(r.h_cnt, r.cnt, sizeof(u64) * RING * r.ntasks, hipMemcpyDeviceToHost, C.stream));
-      HIPC(hipMemcpyAsync(h_err, d_err, sizeof(int), hipMemcpyDeviceToHost, C.stream));
-      HIPC(hipStreamSynchronize(C.stream));
-      trace("level batch synchronized");
-      check_err();
-      for (int t = 0; t < r.ntasks; t++) {
-        St& q = st[t];
-        if (!q.running) continue;
-        const u64 target = (u64)n * (u64)r.l[t];
-        for (int L = examined; L < launched; L++) {
-          const u64 c = r.h_cnt[(size_t)(L % RING) * r.ntasks + t];
-          if (c == target) { q.running = 0; q.lwork = L; max_d = std::max(max_d, L + 1); break; }
-          if ((L >= 1 && c == q.prev) || L + 1 >= n) { q.running = 0; q.lwork = L; any_disc = true; break; }
-          q.sum += (target - c) * (u64)s;
-          q.prev = c;
+      int launched = 1, examined = 0, running = cnt_t;
+      int guess = guess_levels > 0 ? guess_levels : 8;
+      int batch_end = std::min(n, std::max(2, guess));
+      for (;;) {
+        for (int L = launched; L < batch_end; L++) {
+          TimedLaunch tl;
+          timed_begin(tl, T_LEVEL, L);
+          if (r.urow)
+            hipLaunchKernelGGL(k_level<true>, dim3(grid), dim3(LEVEL_WG), 0, C.stream,
+                               tasks, r.wg_start, cnt_t, L, RING, r.upw, r.xcd);
+          else
+            hipLaunchKernelGGL(k_level<false>, dim3(grid), dim3(LEVEL_WG), 0, C.stream,
+                               tasks, r.wg_start, cnt_t, L, RING, r.upw, r.xcd);
+          HIPC(hipGetLastError());
+          timed_end(tl);
         }
-        if (!q.running) running--;
+        launched = std::max(launched, batch_end);
+        HIPC(hipMemcpyAsync(r.h_cnt, r.cnt, sizeof(u64) * RING * r.ntasks, hipMemcpyDeviceToHost, C.stream));
+        HIPC(hipMemcpyAsync(h_err, d_err, sizeof(int), hipMemcpyDeviceToHost, C.stream));
+        HIPC(hipStreamSynchronize(C.stream));
+        trace("level batch synchronized");
+        check_err();
+        for (int t = first; t < first + cnt_t; t++) {
+          St& q = st[t];
+          if (!q.running) continue;
+          const u64 target = (u64)n * (u64)r.l[t];
+          for (int L = examined; L < launched; L++) {
+            const u64 c = r.h_cnt[(size_t)(L % RING) * r.ntasks + t];
+            if (c == target) { q.running = 0; q.lwork = L; max_d = std::max(max_d, L + 1); break; }
+            if ((L >= 1 && c == q.prev) || L + 1 >= n) { q.running = 0; q.lwork = L; any_disc = true; break; }
+            q.sum += (target - c) * (u64)s;
+            q.prev = c;
+          }
+          if (!q.running) running--;
+        }
+        examined = launched;
+        if (running == 0) break;
+        if (launched >= n) { any_disc = true; break; }
+        int step = std::min(MAX_BATCH_LEVELS, std::max(launched, 8));
+        batch_end = std::min(n, launched + step);
       }
-      examined = launched;
-      if (running == 0) break;
-      if (launched >= n) { any_disc = true; break; }
-      int step = std::min(MAX_BATCH_LEVELS, std::max(launched, 8));
-      batch_end = std::min(n, launched + step);
+      if (any_disc) break;          // the result is the sentinel whatever the other blocks give
+      for (int t = first; t < first + cnt_t; t++) lmax = std::max(lmax, st[t].lwork);
+      guess_levels = lmax + 1;
+      if (g_stats_on) flush_level_stats(st.data(), first, cnt_t);
     }
-    int lmax = 0;
     u64 tot = 0;
-    for (int t = 0; t < r.ntasks; t++) { tot += st[t].sum; lmax = std::max(lmax, st[t].lwork); }
-    guess_levels = lmax + 1;
+    for (int t = 0; t < r.ntasks; t++) tot += st[t].sum;
     *excess = tot; *diam = max_d; *disc = any_disc;
-    if (g_stats_on) {
-      HIPC(hipStreamSynchronize(C.stream));
-      for (TimedLaunch& tl : g_pending) {
-        float ms = 0;
-        HIPC(hipEventElapsedTime(&ms, tl.a, tl.b));
-        g_evpool.push_back(tl.a); g_evpool.push_back(tl.b);
-        if (tl.kind == T_LEVEL) {
-          double bytes = 0;
-          for (int t = 0; t < r.ntasks; t++)
-            if (tl.level <= st[t].lwork)
-              bytes += (double)n * ((avg_deg + 2.0) * ((r.l[t] + 63) / 64) * 8.0 + 4.0 * avg_deg);
-          if (bytes > 0) { g_stats.level_launches++; g_stats.level_ms += ms; g_stats.level_bytes += bytes; }
-          else { g_stats.idle_launches++; g_stats.idle_ms += ms; }
-        } else {
-          g_stats.other_launches++; g_stats.other_ms += ms;
-        }
+    if (g_stats_on) flush_level_stats(st.data(), 0, 0);
+  }
+
+  template <typename St>
+  void flush_level_stats(const St* st, int first, int cnt_t)
+  {
+    LevelRun& r = lr;
+    HIPC(hipStreamSynchronize(C.stream));
+    for (TimedLaunch& tl : g_pending) {
+      float ms = 0;
+      HIPC(hipEventElapsedTime(&ms, tl.a, tl.b));
+      g_evpool.push_back(tl.a); g_evpool.push_back(tl.b);
+      if (tl.kind == T_LEVEL) {
+        double bytes = 0;
+        for (int t = first; t < first + cnt_t; t++)
+          if (tl.level <= st[t].lwork)
+            bytes += (double)n * ((avg_deg + 2.0) * ((r.l[t] + 63) / 64) * 8.0 + 4.0 * avg_deg);
+        if (bytes > 0) { g_stats.level_launches++; g_stats.level_ms += ms; g_stats.level_bytes += bytes; }
+        else { g_stats.idle_launches++; g_stats.idle_ms += ms; }
+      } else {
+        g_stats.other_launches++; g_stats.other_ms += ms;
       }
-      g_pending.clear();
     }
+    g_pending.clear();
   }
 
   void run_resident(u64* excess, int* diam, bool* disc)
@@ -799,7 +820,8 @@ extern "C" int ODPX_Autotune(void* adjacency, int repeats)
     for (int sm = 0; sm < 2; sm++)
       for (int x = 0; x < 2; x++) {
         ODPX_Plan p{};
-        p.plan = ODPX_PLAN_LEVEL; p.kw = kw; p.upw = 4; p.seg_major = sm; p.xcd = x;
+        p.plan = ODPX_PLAN_LEVEL; p.kw = kw; p.upw = kw == 0 ? 4 : 8; p.seg_major = sm; p.xcd = x;
+        p.group = v.plan.plan == ODPX_PLAN_LEVEL ? v.plan.group : 0;
         cand.push_back(p);
       }
   if (repeats < 1) repeats = 1;
@@ -889,7 +911,7 @@ static void single_eval(Evaluator& v, int n, int degree, const int* adj, int pla
   ODPX_Plan p = v.default_plan(KIND_MATRIX);
   if (plan_kind == ODPX_PLAN_LEVEL && p.plan != ODPX_PLAN_LEVEL) {
     p = ODPX_Plan{};
-    p.plan = ODPX_PLAN_LEVEL; p.kw = 0; p.upw = 4; p.seg_major = 1; p.xcd = 0;
+    p.plan = ODPX_PLAN_LEVEL; p.kw = 64; p.upw = 8; p.seg_major = 0; p.xcd = 1; p.group = 0;
   }
   v.apply_plan(p);
   v.upload(adj);

@@ -31,6 +31,8 @@ PLANS = [
     dict(plan=LEV, kw=1, upw=1, seg_major=0, xcd=1),
     dict(plan=LEV, kw=3, upw=2, seg_major=1, xcd=1),
     dict(plan=LEV, kw=64, upw=8, seg_major=0, xcd=0),
+    dict(plan=LEV, kw=2, upw=2, seg_major=0, xcd=1, group=2),
+    dict(plan=LEV, kw=1, upw=16, seg_major=1, xcd=0, group=1),
 ]
 
 
@@ -140,7 +142,7 @@ SMALL_VECTORS = [c for c in vectors() if (c.get("nodes") or c["width"] * c["heig
 def test_reference_vectors(case):
     kw, adj = build_case(case)
     want = (case["diameter"], case["sum"], float.fromhex(case["aspl"]))
-    for plan in (None, PLANS[1], PLANS[4], PLANS[6]):
+    for plan in (None, PLANS[1], PLANS[4], PLANS[6], PLANS[8]):
         got = hip_eval(adj, plan=plan, **kw)
         if got is not None:
             assert same(got, want), (plan, got, want)
