@@ -1,8 +1,9 @@
 """Benchmark of the ASPL/diameter evaluator on one MI355X.
 
-Default workload: one seeded random 16-regular simple graph with n=65536,
-evaluated with the MATRIX method (a full 65536 x 65536 bit matrix, 512 MiB per
-buffer).  A step is one full evaluation (every BFS level, exact sum, diameter
+Default workload (--workload random65536): one seeded random 16-regular simple
+graph with n=65536, evaluated with the MATRIX method (the full 65536 x 65536 bit
+matrix, 512 MiB per buffer, every column block resident).  Also: grid256 (the
+256x256 4-fold symmetric grid, MATRIX_SAVING) and random1024 (n=1024, d=6).  A step is one full evaluation (every BFS level, exact sum, diameter
 and ASPL) of the adjacency already resident in HBM.  The JSON line's value is
 evaluations per second.
 
@@ -36,31 +37,42 @@ import odp_amd as odp  # noqa: E402
 
 METRIC = "ASPL evals/s (bit-exact) at n=65536 d=16 and batched 2-opt; HBM GB/s vs peak"
 HBM_PEAK_GBS = 8000.0
-# expected result of the workload graph, from the reference library (tests/golden/vectors.json)
-EXPECT = {(65536, 16, 1): (6, 9346029721, float.fromhex("0x1.1689a2d622d62p+2"))}
+# expected results, from the reference library (tests/golden/vectors.json)
+EXPECT = {("random", 65536, 16, 1): (6, 9346029721, float.fromhex("0x1.1689a2d622d62p+2")),
+          ("random", 1024, 6, 1): (6, 2198785, float.fromhex("0x1.0cab32ccb32cdp+2")),
+          ("grid", 65536, 8, 1): (128, 97430397492, float.fromhex("0x1.6af6491989199p+5"))}
+
+WORKLOADS = {
+    # name: (kind, nodes, degree, description)
+    "random65536": ("random", 65536, 16, "single seeded random 16-regular simple graph, n=65536, MATRIX "
+                                         "(full bit matrix, every column block resident), adjacency resident in HBM"),
+    "grid256": ("grid", 65536, 8, "256x256 grid, degree 8, edge length <= 4, 4-fold rotational symmetry "
+                                  "(16384 sources expanded), MATRIX_SAVING, adjacency resident in HBM"),
+    "random1024": ("random", 1024, 6, "single seeded random 6-regular simple graph, n=1024 (CPU-runnable "
+                                      "case), adjacency resident in HBM"),
+}
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(adj, n, budget_s=20.0):
+def cpu_baseline(adj, gkw, kind_name, budget_s=20.0):
     """Reference OpenMP library on a bounded sample of the same workload."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     if not oracle.ref_available(threads=True):
         return None, None
+    kind = {"MATRIX": oracle.MATRIX, "MATRIX_SAVING": oracle.MATRIX_SAVING}[kind_name]
     cores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
-    (d, s, a), t1 = oracle.run_reference(adj, nodes=n, kind=oracle.MATRIX, threads=True, repeat=1,
-                                         omp_threads=cores)
+    (d, s, a), t1 = oracle.run_reference(adj, kind=kind, threads=True, repeat=1, omp_threads=cores, **gkw)
     per = t1[0] if t1 else 1.0
-    reps = max(2, min(30, int(budget_s / max(per, 1e-3))))
-    (d, s, a), ts = oracle.run_reference(adj, nodes=n, kind=oracle.MATRIX, threads=True, repeat=reps,
-                                         omp_threads=cores)
+    reps = max(2, min(30, int(budget_s / max(per, 1e-4))))
+    (d, s, a), ts = oracle.run_reference(adj, kind=kind, threads=True, repeat=reps, omp_threads=cores, **gkw)
     med = statistics.median(ts)
     return ({"value": 1.0 / med, "unit": "evals/s", "cores": cores, "kind": "reference",
              "sample": f"{reps} timed ODP_Set_aspl calls (after 1 warm-up) of the reference OpenMP library, "
-                       f"MATRIX, on the same n={n} d={adj.shape[1]} graph; median {med:.3f} s"},
+                       f"{kind_name}, on the same graph; median {med:.6f} s"},
             (d, s, a))
 
 
@@ -109,11 +121,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--nodes", type=int, default=65536)
-    ap.add_argument("--degree", type=int, default=16)
+    ap.add_argument("--workload", default="random65536", choices=sorted(WORKLOADS))
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--plan", default="matrix",
-                    help="matrix | matrix-rows | auto | level:<kw>[:upw:seg_major:xcd:group] | resident:<wb>:<threads>")
+    ap.add_argument("--plan", default="auto",
+                    help="auto (the library's choice) | matrix | matrix-rows | level:<kw>[:upw:seg_major:xcd:group] "
+                         "| resident:<wb>:<threads>")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic.json"))
@@ -123,15 +135,25 @@ def main():
               f"source-column blocks, see DESIGN.md)", file=sys.stderr)
         sys.exit(2)
 
-    n, d = args.nodes, args.degree
+    gkind, n, d, desc = WORKLOADS[args.workload]
     t0 = time.perf_counter()
-    adj = odp.generate_regular(n, d, args.seed)
-    log(f"graph n={n} d={d} seed={args.seed} generated in {time.perf_counter() - t0:.2f}s")
-
-    odp.init_aspl_general(n, d)
+    if gkind == "random":
+        adj = odp.generate_regular(n, d, args.seed)
+        gkw = dict(nodes=n)
+        kind_name = "MATRIX"
+        os.environ["ODP_ASPL"] = kind_name
+        odp.init_aspl_general(n, d)
+    else:
+        full = odp.generate_grid(256, 256, d, 4, 4, args.seed)
+        adj = odp.grid_base_table(256, 256, d, 4, full)
+        gkw = dict(width=256, height=256, symmetries=4)
+        kind_name = "MATRIX_SAVING"
+        os.environ["ODP_ASPL"] = kind_name
+        odp.init_aspl_grid_s(256, 256, d, None, 4)
+    log(f"{args.workload}: graph generated in {time.perf_counter() - t0:.2f}s")
     if args.plan == "matrix":
-        # every column block resident (the full bit matrix), 64-word slabs, XCD-owned
-        odp.set_plan(plan=odp.PLAN_LEVEL, kw=64, upw=8, seg_major=0, xcd=1, group=0)
+        # every column block resident (the full bit matrix), 16-word slabs, XCD-owned
+        odp.set_plan(plan=odp.PLAN_LEVEL, kw=16, upw=8, seg_major=0, xcd=1, group=0)
     elif args.plan == "matrix-rows":
         # the full bit matrix with whole 8 KiB rows (the reference's layout)
         odp.set_plan(plan=odp.PLAN_LEVEL, kw=0, upw=4, seg_major=1, xcd=0, group=0)
@@ -145,7 +167,7 @@ def main():
         odp.set_plan(plan=odp.PLAN_RESIDENT, wb=int(f[1]), threads=int(f[2]))
     plan = odp.get_plan()
     first = odp.set_aspl(adj)                      # uploads and evaluates (PCIe-inclusive)
-    want = EXPECT.get((n, d, args.seed))
+    want = EXPECT.get((gkind, n, d, args.seed))
     if want is not None and (first[0], first[1]) != want[:2]:
         raise SystemExit(f"result {first} differs from the reference's {want}")
     log(f"first evaluation: diameter={first[0]} sum={first[1]} ASPL={first[2]!r}; plan {plan}")
@@ -174,7 +196,7 @@ def main():
     achieved = (st["level_bytes"] / max(1, st["level_launches"])) / (level_ms * 1e-3) / 1e9 if level_ms > 0 else 0.0
     traffic = None
     if os.path.exists(args.traffic):
-        tr = json.load(open(args.traffic)).get(f"{n}x{d}:{args.plan}")
+        tr = json.load(open(args.traffic)).get(args.workload)
         if tr:
             traffic = tr["bytes_per_launch"]
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
@@ -187,14 +209,13 @@ def main():
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * el / args.steps, 4),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
            "data": "synthetic",
-           "config": {"workload": f"single seeded random {d}-regular simple graph, n={n}, MATRIX "
-                                  f"(full bit matrix, every column block resident), adjacency resident in HBM",
-                      "nodes": n, "degree": d, "seed": args.seed, "plan": plan,
+           "config": {"workload": desc, "name": args.workload,
+                      "nodes": n, "degree": d, "seed": args.seed, "kind": kind_name, "plan": plan,
                       "diameter": first[0], "sum": first[1], "aspl": first[2],
                       "pcie_inclusive_evals_per_s": round(pcie_rate, 3)},
            "roofline": roofline}
     if not args.no_cpu:
-        cb, cres = cpu_baseline(adj, n)
+        cb, cres = cpu_baseline(adj, gkw, kind_name)
         if cb is not None:
             if cres[:2] != first[:2] or cres[2] != first[2]:
                 raise SystemExit(f"reference {cres} differs from HIP {first}")
@@ -202,7 +223,7 @@ def main():
         out["cpu_baseline"] = cb
     else:
         out["cpu_baseline"] = None
-    if not args.no_extras:
+    if not args.no_extras and args.workload == "random65536":
         out["batched"] = batched_extra(1024, 3)
     print(json.dumps(out), flush=True)
 

@@ -31,7 +31,8 @@
 
 static const int KIND_MATRIX = 1, KIND_SAVING = 2;
 static const int GPU_CHUNK_WORDS = 64;          // column-block width of MATRIX_SAVING (parameter.h:13)
-static const int SAVING_GROUP = 8;              // MATRIX_SAVING blocks resident at once (one per XCD)
+static const int SAVING_GROUP = 32;             // MATRIX_SAVING blocks resident at once (4 per XCD)
+static const int LEVEL_BLOCK_WORDS = 16;        // default column-block width of the level plan
 static const double MEM_THRESHOLD_MB = 2048.0;  // MATRIX -> MATRIX_SAVING switch (parameter.h:4)
 static const size_t LDS_LIMIT = 163840;         // LDS a gfx950 workgroup may allocate
 static const size_t LDS_STATIC = 512;           // k_resident's own __shared__ data, rounded up
@@ -231,14 +232,15 @@ struct Evaluator {
       p.threads = n >= 4096 ? 1024 : 256;
       return p;
     }
-    // Column blocks of 64 words (4096 sources), each stored as its own
-    // contiguous n x 512 B slab, with each XCD owning whole slabs.  MATRIX
+    // Column blocks of 16 words (1024 sources), each stored as its own
+    // contiguous n x 128 B slab, with each XCD owning whole slabs, so the rows
+    // one XCD gathers from stay within a few MiB (its L2).  MATRIX
     // keeps every block resident and advances all of them per launch (the full
     // bit matrix, blocked); MATRIX_SAVING keeps SAVING_GROUP blocks resident
     // at a time, bounding memory as the reference does (aspl.c:77-119).
     p.plan = ODPX_PLAN_LEVEL;
     int W = (nsrc + 63) / 64;
-    p.kw = std::min(GPU_CHUNK_WORDS, W);
+    p.kw = std::min(LEVEL_BLOCK_WORDS, W);
     p.upw = 8;
     p.seg_major = 0;
     p.xcd = 1;

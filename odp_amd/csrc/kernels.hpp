@@ -282,9 +282,10 @@ __global__ __launch_bounds__(LEVEL_WG) void k_level(const LevelTask* __restrict_
     if (q.r < T.n) {
       const u64 f0 = full_word(T.l, q.w), f1 = full_word(T.l, q.w + 1);
       u64x2 x = *(const u64x2*)(cur + q.r * kwp + q.w);
-      if (x.x != f0 || x.y != f1) {
+      // Once every lane's words are saturated the remaining neighbours cannot
+      // add a bit, so the wave stops gathering (exact: OR is monotone).
+      if (!__all(x.x == f0 && x.y == f1)) {
         const int* nb = T.nbr + (int64_t)q.r * T.dpad;
-#pragma unroll 2
         for (int j = 0; j < T.dpad; j += 4) {
           const int4 v = *(const int4*)(nb + j);
           const u64x2 a = *(const u64x2*)(cur + (int64_t)v.x * kwp + q.w);
@@ -293,6 +294,7 @@ __global__ __launch_bounds__(LEVEL_WG) void k_level(const LevelTask* __restrict_
           const u64x2 d = *(const u64x2*)(cur + (int64_t)v.w * kwp + q.w);
           x.x |= (a.x | b.x) | (c.x | d.x);
           x.y |= (a.y | b.y) | (c.y | d.y);
+          if (__all(x.x == f0 && x.y == f1)) break;
         }
       }
       *(u64x2*)(nxt + q.r * kwp + q.w) = x;
