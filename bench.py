@@ -76,6 +76,22 @@ def cpu_baseline(adj, gkw, kind_name, budget_s=20.0):
             (d, s, a))
 
 
+def calibrate_copy(mib):
+    """One hipMemcpy device-to-device of `mib` MiB (runs the runtime's copy kernel)."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    a, b = ctypes.c_void_p(), ctypes.c_void_p()
+    size = mib << 20
+    assert hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(size)) == 0
+    assert hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(size)) == 0
+    assert hip.hipMemset(a, 1, ctypes.c_size_t(size)) == 0
+    assert hip.hipMemcpy(b, a, ctypes.c_size_t(size), 3) == 0      # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    log(f"calibration copy: {mib} MiB device-to-device")
+
+
 def c4_graphs(count, seed=2024):
     rng = np.random.default_rng(seed)
     graphs = []
@@ -129,7 +145,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic.json"))
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="first run one device-to-device copy of MIB MiB (a plain copy kernel of known "
+                         "size, to scale rocprofv3 FETCH_SIZE / WRITE_SIZE)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the path shards by "
               f"source-column blocks, see DESIGN.md)", file=sys.stderr)
