@@ -42,7 +42,7 @@ int ODPX_Two_opt(int nodes, int degree, int *adjacency, uint64_t rng[4], int swa
 
 typedef struct {
   int plan;       /* ODPX_PLAN_RESIDENT or ODPX_PLAN_LEVEL                          */
-  int wb;         /* resident: 32-bit words per row of a block (1, 2, 4, 8 or 16)    */
+  int wb;         /* resident: 32-bit words per row of a block (1, 2, 4 or 8)        */
   int threads;    /* resident: threads per workgroup (256, 512 or 1024)              */
   int kw;         /* level: 64-bit words per row of a column block; 0 = whole row    */
   int upw;        /* level: row segments per wave per launch                         */
@@ -90,16 +90,25 @@ void ODPX_Stats_reset(void);
 void ODPX_Stats_get(ODPX_Stats *out);
 
 /* ---- batched evaluation ------------------------------------------------- */
-/* Evaluates ngraphs general (s = 1) graphs.  Graph g has nodes[g] vertices,
- * table width degrees[g], and its adjacency int[nodes[g]][degrees[g]] starts
- * at packed_adjacency + offsets[g]; -1 entries are absent edges.  Results as
- * ODP_Set_aspl.  Graphs that fit the LDS-resident kernel run in one persistent
- * launch; larger ones run the level kernels.  Returns 0. */
+/* Uploads ngraphs general (s = 1) graphs, packed as for ODPX_Batch_eval, and
+ * builds their neighbour tables; they stay on the device until the next load
+ * or ODPX_Batch_release.  Returns 0. */
+int ODPX_Batch_load(int ngraphs, const int *nodes, const int *degrees, const long long *offsets,
+                    const int *packed_adjacency);
+/* Evaluates the loaded graphs: every graph that fits the LDS-resident kernel in
+ * one persistent launch, the others with the single-graph path.  Results
+ * indexed like the loaded graphs.  Returns 0. */
+int ODPX_Batch_run(int *diameter, long *sum, double *ASPL);
+/* Evaluates the loaded graphs one at a time with the single-graph path
+ * (plan ODPX_PLAN_LEVEL or ODPX_PLAN_AUTO): the comparison point. */
+int ODPX_Loop_run(int plan, int *diameter, long *sum, double *ASPL);
+/* ODPX_Batch_load + ODPX_Batch_run.  Graph g has nodes[g] vertices, table
+ * width degrees[g], and its adjacency int[nodes[g]][degrees[g]] starts at
+ * packed_adjacency + offsets[g]; -1 entries are absent edges.  Results as
+ * ODP_Set_aspl.  Returns 0. */
 int ODPX_Batch_eval(int ngraphs, const int *nodes, const int *degrees, const long long *offsets,
                     const int *packed_adjacency, int *diameter, long *sum, double *ASPL);
-/* Same graphs, evaluated one after another with the single-graph level path
- * (the reference's pattern of one evaluation per call); the comparison point
- * for the batched path.  plan: ODPX_PLAN_LEVEL or ODPX_PLAN_AUTO. */
+/* ODPX_Batch_load + ODPX_Loop_run. */
 int ODPX_Loop_eval(int ngraphs, const int *nodes, const int *degrees, const long long *offsets,
                    const int *packed_adjacency, int plan, int *diameter, long *sum, double *ASPL);
 /* Frees the device buffers the batched and looped paths keep between calls. */

@@ -109,6 +109,9 @@ def load():
     _sig(lib, "ODPX_Stats_get", None, [ctypes.POINTER(Stats)])
     batch_args = [_I, _P, _P, _P, _P, _P, _P, _P]
     _sig(lib, "ODPX_Batch_eval", _I, batch_args)
+    _sig(lib, "ODPX_Batch_load", _I, [_I, _P, _P, _P, _P])
+    _sig(lib, "ODPX_Batch_run", _I, [_P, _P, _P])
+    _sig(lib, "ODPX_Loop_run", _I, [_I, _P, _P, _P])
     _sig(lib, "ODPX_Loop_eval", _I, [_I, _P, _P, _P, _P, _I, _P, _P, _P])
     _sig(lib, "ODPX_Batch_release", None, [])
     _lib = lib
@@ -246,6 +249,31 @@ def batch_eval(packed_graphs):
 def loop_eval(packed_graphs, plan=PLAN_LEVEL):
     """ODPX_Loop_eval: one single-graph evaluation per graph."""
     return _batch(load().ODPX_Loop_eval, packed_graphs, plan)
+
+
+def batch_load(packed_graphs):
+    """ODPX_Batch_load: uploads the graphs and builds their tables (kept on the device)."""
+    nodes, degrees, offsets, packed = packed_graphs
+    load().ODPX_Batch_load(len(nodes), _ptr(nodes), _ptr(degrees), _ptr(offsets), _ptr(packed))
+    return len(nodes)
+
+
+def batch_run(ngraphs):
+    """ODPX_Batch_run on the loaded graphs -> (diameter[], sum[], ASPL[])."""
+    d = np.zeros(ngraphs, np.int32)
+    s = np.zeros(ngraphs, np.int64)
+    x = np.zeros(ngraphs, np.float64)
+    load().ODPX_Batch_run(_ptr(d), _ptr(s), _ptr(x))
+    return d, s, x
+
+
+def loop_run(ngraphs, plan=PLAN_LEVEL):
+    """ODPX_Loop_run on the loaded graphs (one single-graph evaluation each)."""
+    d = np.zeros(ngraphs, np.int32)
+    s = np.zeros(ngraphs, np.int64)
+    x = np.zeros(ngraphs, np.float64)
+    load().ODPX_Loop_run(plan, _ptr(d), _ptr(s), _ptr(x))
+    return d, s, x
 
 
 def batch_release():

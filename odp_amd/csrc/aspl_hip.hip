@@ -33,6 +33,7 @@ static const int KIND_MATRIX = 1, KIND_SAVING = 2;
 static const int GPU_CHUNK_WORDS = 64;          // column-block width of MATRIX_SAVING (parameter.h:13)
 static const int SAVING_GROUP = 32;             // MATRIX_SAVING blocks resident at once (4 per XCD)
 static const int LEVEL_BLOCK_WORDS = 16;        // default column-block width of the level plan
+static const int RES_WB_MAX = 8;                // widest resident block: 256 sources
 static const double MEM_THRESHOLD_MB = 2048.0;  // MATRIX -> MATRIX_SAVING switch (parameter.h:4)
 static const size_t LDS_LIMIT = 163840;         // LDS a gfx950 workgroup may allocate
 static const size_t LDS_STATIC = 512;           // k_resident's own __shared__ data, rounded up
@@ -149,6 +150,10 @@ struct Evaluator {
   double avg_deg = 0;
   size_t adj_cap = 0, nbr_cap = 0;
   int *d_adj = nullptr, *d_numdeg = nullptr, *d_itable = nullptr, *d_nbr = nullptr, *d_err = nullptr;
+  unsigned short* d_nbr16 = nullptr;    // 16-bit row-neighbour table of the resident plan
+  const int* ext_adj = nullptr;         // when set, the adjacency is read from this device copy
+  int dp16 = 8;
+  size_t nbr16_cap = 0;
   int *h_stage = nullptr, *h_err = nullptr;
   size_t stage_cap = 0;
   ODPX_Plan plan{};
@@ -160,7 +165,8 @@ struct Evaluator {
 
   void release()
   {
-    dfree(&d_adj); dfree(&d_numdeg); dfree(&d_itable); dfree(&d_nbr); dfree(&d_err);
+    dfree(&d_adj); dfree(&d_numdeg); dfree(&d_itable); dfree(&d_nbr); dfree(&d_err); dfree(&d_nbr16);
+    nbr16_cap = 0;
     hfree(&h_stage); hfree(&h_err);
     dfree(&lr.bufs); dfree(&lr.cnt); dfree(&lr.done); dfree(&lr.tasks); dfree(&lr.wg_start); hfree(&lr.h_cnt);
     dfree(&rr.tasks); dfree(&rr.out); dfree(&rr.qhead); hfree(&rr.h_out);
@@ -181,6 +187,11 @@ struct Evaluator {
     if (adj_elems() > stage_cap) { hfree(&h_stage); halloc(&h_stage, adj_elems()); stage_cap = adj_elems(); }
     size_t ne = (size_t)n * dpad;
     if (ne > nbr_cap) { dfree(&d_nbr); dalloc(&d_nbr, ne); nbr_cap = ne; }
+    dp16 = round_up(deg > 0 ? deg : 1, 8);
+    if (n <= 65535) {
+      size_t ne16 = (size_t)n * dp16;
+      if (ne16 > nbr16_cap) { dfree(&d_nbr16); dalloc(&d_nbr16, ne16); nbr16_cap = ne16; }
+    }
     if (!d_err) { dalloc(&d_err, 1); halloc(&h_err, 1); HIPC(hipMemsetAsync(d_err, 0, sizeof(int), C.stream)); }
     dfree(&d_numdeg);
     dfree(&d_itable);
@@ -208,10 +219,13 @@ struct Evaluator {
   }
 
   // ---- plans ----
-  int resident_wb_max() const
+  // widest block the resident kernel can run for this graph with `threads`
+  // threads: one LDS buffer of n*wb*4 bytes and n <= res_rpt(wb)*threads rows
+  int resident_wb_max(int threads = 1024) const
   {
-    for (int wb = 16; wb >= 1; wb >>= 1)
-      if ((size_t)2 * n * wb * 4 <= LDS_DYN_MAX) return wb;
+    if (n > 65535 || !d_nbr16) return 0;
+    for (int wb = RES_WB_MAX; wb >= 1; wb >>= 1)
+      if ((size_t)n * wb * 4 <= LDS_DYN_MAX && n <= res_rpt(wb) * threads) return wb;
     return 0;
   }
 
@@ -229,7 +243,9 @@ struct Evaluator {
       for (int c = wbmax; c >= 1; c >>= 1)
         if ((nsrc + 32 * c - 1) / (32 * c) >= C.ncu) { wb = c; break; }
       p.wb = wb;
-      p.threads = n >= 4096 ? 1024 : 256;
+      p.threads = 1024;
+      for (int th : {256, 512, 1024})
+        if (n <= res_rpt(wb) * th && (n >= 4096 ? th == 1024 : true)) { p.threads = th; break; }
       return p;
     }
     // Column blocks of 16 words (1024 sources), each stored as its own
@@ -251,9 +267,9 @@ struct Evaluator {
   bool plan_valid(const ODPX_Plan& p) const
   {
     if (p.plan == ODPX_PLAN_RESIDENT) {
-      if (p.wb != 1 && p.wb != 2 && p.wb != 4 && p.wb != 8 && p.wb != 16) return false;
+      if (p.wb != 1 && p.wb != 2 && p.wb != 4 && p.wb != 8) return false;
       if (p.threads != 256 && p.threads != 512 && p.threads != 1024) return false;
-      return (size_t)2 * n * p.wb * 4 <= LDS_DYN_MAX;
+      return n <= 65535 && d_nbr16 && (size_t)n * p.wb * 4 <= LDS_DYN_MAX && n <= res_rpt(p.wb) * p.threads;
     }
     if (p.plan == ODPX_PLAN_LEVEL) return p.kw >= 0 && p.upw >= 1 && p.upw <= 64 && p.group >= 0;
     return false;
@@ -328,12 +344,12 @@ struct Evaluator {
     std::vector<ResTask> ht(ntasks);
     for (int t = 0; t < ntasks; t++) {
       ResTask& T = ht[t];
-      T.nbr = d_nbr; T.n = n; T.dpad = dpad; T.src0 = t * b; T.l = std::min(b, nsrc - t * b);
+      T.nbr = d_nbr16; T.n = n; T.dp = dp16; T.src0 = t * b; T.l = std::min(b, nsrc - t * b);
       T.s = s; T.graph = 0; T.wb = r.wb; T.pad = 0;
     }
     HIPC(hipMemcpy(r.tasks, ht.data(), sizeof(ResTask) * ntasks, hipMemcpyHostToDevice));
     r.ntasks = ntasks;
-    r.lds = (size_t)2 * n * r.wb * 4;
+    r.lds = (size_t)n * r.wb * 4;
     int per_cu = (int)std::max<size_t>(1, LDS_LIMIT / (r.lds + LDS_STATIC));
     per_cu = std::min(per_cu, 2048 / r.threads);
     r.grid = std::max(1, std::min(ntasks, per_cu * C.ncu));
@@ -351,10 +367,14 @@ struct Evaluator {
   void prepare(u64* z64, int nz64, u32* z32, int nz32)
   {
     PrepArgs a;
-    a.adj = d_adj; a.numdeg = d_numdeg; a.itable = d_itable; a.nbr = d_nbr; a.err = d_err;
-    a.n = n; a.s = s; a.degree = degree; a.dpad = dpad; a.width = width; a.height = height; a.grid_s = grid_s;
+    const bool res = plan.plan == ODPX_PLAN_RESIDENT;
+    a.adj = ext_adj ? ext_adj : d_adj; a.numdeg = d_numdeg; a.itable = d_itable; a.err = d_err;
+    a.nbr = res ? nullptr : d_nbr;
+    a.nbr16 = res ? d_nbr16 : nullptr;
+    a.n = n; a.s = s; a.degree = degree; a.dpad = res ? dp16 : dpad;
+    a.width = width; a.height = height; a.grid_s = grid_s;
     a.zero64 = z64; a.nzero64 = nz64; a.zero32 = z32; a.nzero32 = nz32;
-    int64_t total = (int64_t)n * dpad;
+    int64_t total = (int64_t)n * a.dpad;
     int grid = (int)std::min<int64_t>(std::max<int64_t>(1, (total + 255) / 256), 4096);
     TimedLaunch tl;
     timed_begin(tl, T_OTHER, -1);
@@ -498,8 +518,6 @@ struct Evaluator {
         HIPC(hipEventElapsedTime(&ms, t.a, t.b));
         g_evpool.push_back(t.a); g_evpool.push_back(t.b);
         if (t.kind == T_RESIDENT) {
-          int kb = (32 * r.wb + 63) / 64;
-          (void)kb;
           double row = (avg_deg + 2.0) * (std::min(32 * r.wb, nsrc) / 8.0) + 4.0 * avg_deg;
           g_stats.resident_launches++; g_stats.resident_ms += ms;
           g_stats.resident_bytes += (double)r.h_out->levels * n * row;
@@ -808,12 +826,11 @@ extern "C" int ODPX_Autotune(void* adjacency, int repeats)
   need_active("ODPX_Autotune");
   Evaluator& v = E.ev;
   std::vector<ODPX_Plan> cand;
-  int wbmax = v.resident_wb_max();
-  for (int wb = 1; wb <= wbmax; wb <<= 1)
+  for (int wb = 1; wb <= RES_WB_MAX; wb <<= 1)
     for (int th : {256, 512, 1024}) {
       ODPX_Plan p{};
       p.plan = ODPX_PLAN_RESIDENT; p.wb = wb; p.threads = th;
-      cand.push_back(p);
+      if (v.plan_valid(p)) cand.push_back(p);
     }
   int W = (v.nsrc + 63) / 64;
   std::vector<int> kws = {0};
@@ -851,16 +868,13 @@ extern "C" int ODPX_Autotune(void* adjacency, int repeats)
 // ---------------------------------------------------------------------------
 // batched evaluation: every graph small enough for LDS in one persistent launch
 
-struct BatchGraph { long long adj_off, nbr_off, ent_start; int n, degree, dpad, pad; };
+struct BatchGraph { long long adj_off, nbr_off, ent_start; int n, degree, dp, pad; };
 
 __global__ void k_prepare_batch(const BatchGraph* __restrict__ gs, int ng, long long total,
-                                const int* __restrict__ adj, int* __restrict__ nbr, int* err,
-                                ResOut* out, int* qhead)
+                                const int* __restrict__ adj, unsigned short* __restrict__ nbr, int* err)
 {
   const long long tid = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   const long long stride = (long long)gridDim.x * blockDim.x;
-  for (long long g = tid; g < ng; g += stride) { out[g].sum = 0; out[g].diam = 0; out[g].flags = 0; out[g].levels = 0; }
-  if (tid == 0) *qhead = 0;
   for (long long idx = tid; idx < total; idx += stride) {
     int lo = 0, hi = ng - 1;
     while (lo < hi) {
@@ -869,30 +883,43 @@ __global__ void k_prepare_batch(const BatchGraph* __restrict__ gs, int ng, long 
     }
     const BatchGraph G = gs[lo];
     const long long e = idx - G.ent_start;
-    const int i = (int)(e / G.dpad), j = (int)(e - (long long)i * G.dpad);
+    const int i = (int)(e / G.dp), j = (int)(e - (long long)i * G.dp);
     int out_v = i;
     if (j < G.degree) {
       const int v = adj[G.adj_off + (long long)i * G.degree + j];
       if (v >= 0 && v < G.n) out_v = v;
       else if (v != -1) atomicOr(err, 1);
     }
-    nbr[G.nbr_off + e] = out_v;
+    nbr[G.nbr_off + e] = (unsigned short)out_v;
   }
 }
 
 static struct {
+  // the loaded batch
+  int ng = 0;
+  std::vector<int> nodes, degrees;
+  std::vector<long long> offsets;
+  std::vector<int> small, large;        // graphs on the resident path / on the single-graph path
+  std::vector<BatchGraph> hg;           // per small graph
+  std::vector<int> wb;                  // per small graph
+  int ntasks = 0;
+  size_t lds = 0;
+  // device buffers (grow-only)
   size_t adj_cap = 0, nbr_cap = 0, g_cap = 0, t_cap = 0;
-  int *d_adj = nullptr, *d_nbr = nullptr, *d_err = nullptr, *d_qhead = nullptr, *h_err = nullptr;
+  int *d_adj = nullptr, *d_err = nullptr, *d_qhead = nullptr, *h_err = nullptr;
+  unsigned short* d_nbr = nullptr;
   BatchGraph* d_g = nullptr;
   ResOut *d_out = nullptr, *h_out = nullptr;
   ResTask* d_t = nullptr;
   Evaluator single;          // graphs too large for the LDS path, and the looped path
 } B;
 
+// widest block for a graph of n rows in a 1024-thread resident workgroup; 0 if none
 static int batch_wb(int n)
 {
-  for (int wb = 16; wb >= 1; wb >>= 1)
-    if ((size_t)2 * n * wb * 4 <= LDS_DYN_MAX) return wb;
+  if (n > 65535) return 0;
+  for (int wb = RES_WB_MAX; wb >= 1; wb >>= 1)
+    if ((size_t)n * wb * 4 <= LDS_DYN_MAX && n <= res_rpt(wb) * 1024) return wb;
   return 0;
 }
 
@@ -906,64 +933,73 @@ static void finish_one(int n, u64 excess, int d, bool disc, int* diameter, long*
   *diameter = d; *sum = lsum; *ASPL = aspl;
 }
 
-static void single_eval(Evaluator& v, int n, int degree, const int* adj, int plan_kind,
-                        int* diameter, long* sum, double* ASPL)
+// one graph through the single-graph path, its adjacency already on the device
+static void single_eval_dev(Evaluator& v, int n, int degree, const int* dev_adj, int plan_kind,
+                            int* diameter, long* sum, double* ASPL)
 {
   v.shape(0, n, degree, 1, 0, -1, nullptr, 0);
+  v.ext_adj = dev_adj;
   ODPX_Plan p = v.default_plan(KIND_MATRIX);
   if (plan_kind == ODPX_PLAN_LEVEL && p.plan != ODPX_PLAN_LEVEL) {
     p = ODPX_Plan{};
-    p.plan = ODPX_PLAN_LEVEL; p.kw = 64; p.upw = 8; p.seg_major = 0; p.xcd = 1; p.group = 0;
+    p.plan = ODPX_PLAN_LEVEL; p.kw = LEVEL_BLOCK_WORDS; p.upw = 8; p.seg_major = 0; p.xcd = 1; p.group = 0;
   }
   v.apply_plan(p);
-  v.upload(adj);
   v.evaluate(diameter, sum, ASPL);
+  v.ext_adj = nullptr;
   if (*diameter > n) { *diameter = INT_MAX; *sum = LONG_MAX; *ASPL = DBL_MAX; }
 }
 
-extern "C" int ODPX_Batch_eval(int ngraphs, const int* nodes, const int* degrees, const long long* offsets,
-                               const int* packed, int* diameter, long* sum, double* ASPL)
+extern "C" int ODPX_Batch_load(int ngraphs, const int* nodes, const int* degrees, const long long* offsets,
+                               const int* packed)
 {
   ctx_init();
+  B.ng = ngraphs;
+  B.nodes.assign(nodes, nodes + ngraphs);
+  B.degrees.assign(degrees, degrees + ngraphs);
+  B.offsets.assign(offsets, offsets + ngraphs);
+  B.small.clear(); B.large.clear(); B.hg.clear(); B.wb.clear();
+  B.ntasks = 0; B.lds = 0;
   if (ngraphs <= 0) return 0;
-  std::vector<int> small, large;
   long long adj_end = 0;
   for (int g = 0; g < ngraphs; g++) {
-    if (nodes[g] < 1 || degrees[g] < 0) ODP_ERROR("ODPX_Batch_eval: invalid graph %d\n", g);
+    if (nodes[g] < 1 || degrees[g] < 0 || offsets[g] < 0) ODP_ERROR("ODPX_Batch_load: invalid graph %d\n", g);
     adj_end = std::max(adj_end, offsets[g] + (long long)nodes[g] * degrees[g]);
-    (batch_wb(nodes[g]) > 0 ? small : large).push_back(g);
+    (batch_wb(nodes[g]) > 0 ? B.small : B.large).push_back(g);
   }
-  if (!small.empty()) {
-    std::vector<BatchGraph> hg(small.size());
+  if ((size_t)adj_end > B.adj_cap) { dfree(&B.d_adj); dalloc(&B.d_adj, (size_t)adj_end); B.adj_cap = adj_end; }
+  if (!B.d_err) { dalloc(&B.d_err, 1); dalloc(&B.d_qhead, 1); halloc(&B.h_err, 1); HIPC(hipMemsetAsync(B.d_err, 0, 4, C.stream)); }
+  HIPC(hipMemcpyAsync(B.d_adj, packed, sizeof(int) * adj_end, hipMemcpyHostToDevice, C.stream));
+  if (!B.small.empty()) {
     long long nbr_tot = 0;
-    size_t lds = 0;
+    B.hg.resize(B.small.size());
+    B.wb.resize(B.small.size());
     std::vector<ResTask> tasks;
     std::vector<double> cost;
-    for (size_t k = 0; k < small.size(); k++) {
-      int g = small[k];
-      BatchGraph& G = hg[k];
-      G.adj_off = offsets[g]; G.n = nodes[g]; G.degree = degrees[g]; G.dpad = round_up(std::max(1, degrees[g]), 4);
+    for (size_t k = 0; k < B.small.size(); k++) {
+      int g = B.small[k];
+      BatchGraph& G = B.hg[k];
+      G.adj_off = offsets[g]; G.n = nodes[g]; G.degree = degrees[g]; G.dp = round_up(std::max(1, degrees[g]), 8);
       G.nbr_off = nbr_tot; G.ent_start = nbr_tot; G.pad = 0;
-      nbr_tot += (long long)G.n * G.dpad;
+      nbr_tot += (long long)G.n * G.dp;
     }
-    if ((size_t)adj_end > B.adj_cap) { dfree(&B.d_adj); dalloc(&B.d_adj, (size_t)adj_end); B.adj_cap = adj_end; }
     if ((size_t)nbr_tot > B.nbr_cap) { dfree(&B.d_nbr); dalloc(&B.d_nbr, (size_t)nbr_tot); B.nbr_cap = nbr_tot; }
-    if (small.size() > B.g_cap) {
+    if (B.small.size() > B.g_cap) {
       dfree(&B.d_g); dfree(&B.d_out); hfree(&B.h_out);
-      dalloc(&B.d_g, small.size()); dalloc(&B.d_out, small.size()); halloc(&B.h_out, small.size());
-      B.g_cap = small.size();
+      dalloc(&B.d_g, B.small.size()); dalloc(&B.d_out, B.small.size()); halloc(&B.h_out, B.small.size());
+      B.g_cap = B.small.size();
     }
-    if (!B.d_err) { dalloc(&B.d_err, 1); dalloc(&B.d_qhead, 1); halloc(&B.h_err, 1); HIPC(hipMemsetAsync(B.d_err, 0, 4, C.stream)); }
-    for (size_t k = 0; k < small.size(); k++) {
-      const BatchGraph& G = hg[k];
+    for (size_t k = 0; k < B.small.size(); k++) {
+      const BatchGraph& G = B.hg[k];
       int wb = batch_wb(G.n), b = 32 * wb, nt = (G.n + b - 1) / b;
-      lds = std::max(lds, (size_t)2 * G.n * wb * 4);
+      B.wb[k] = wb;
+      B.lds = std::max(B.lds, (size_t)G.n * wb * 4);
       for (int t = 0; t < nt; t++) {
         ResTask T;
-        T.nbr = B.d_nbr + G.nbr_off; T.n = G.n; T.dpad = G.dpad; T.src0 = t * b; T.l = std::min(b, G.n - t * b);
+        T.nbr = B.d_nbr + G.nbr_off; T.n = G.n; T.dp = G.dp; T.src0 = t * b; T.l = std::min(b, G.n - t * b);
         T.s = 1; T.graph = (int)k; T.wb = wb; T.pad = 0;
         tasks.push_back(T);
-        cost.push_back((double)G.n * G.dpad * (1.0 + log((double)G.n) / log(std::max(2.0, G.degree - 1.0))));
+        cost.push_back((double)G.n * G.dp * (1.0 + log((double)G.n) / log(std::max(2.0, G.degree - 1.0))));
       }
     }
     // longest blocks first, so the queue's tail is short
@@ -973,65 +1009,85 @@ extern "C" int ODPX_Batch_eval(int ngraphs, const int* nodes, const int* degrees
     std::vector<ResTask> sorted(tasks.size());
     for (size_t i = 0; i < ord.size(); i++) sorted[i] = tasks[ord[i]];
     if (sorted.size() > B.t_cap) { dfree(&B.d_t); dalloc(&B.d_t, sorted.size()); B.t_cap = sorted.size(); }
-    HIPC(hipMemcpyAsync(B.d_adj, packed, sizeof(int) * adj_end, hipMemcpyHostToDevice, C.stream));
-    HIPC(hipMemcpyAsync(B.d_g, hg.data(), sizeof(BatchGraph) * hg.size(), hipMemcpyHostToDevice, C.stream));
+    B.ntasks = (int)sorted.size();
+    HIPC(hipMemcpyAsync(B.d_g, B.hg.data(), sizeof(BatchGraph) * B.hg.size(), hipMemcpyHostToDevice, C.stream));
     HIPC(hipMemcpyAsync(B.d_t, sorted.data(), sizeof(ResTask) * sorted.size(), hipMemcpyHostToDevice, C.stream));
-    TimedLaunch tp;
-    timed_begin(tp, T_OTHER, -1);
     int pgrid = (int)std::min<long long>(std::max<long long>(1, (nbr_tot + 255) / 256), 8192);
-    hipLaunchKernelGGL(k_prepare_batch, dim3(pgrid), dim3(256), 0, C.stream, B.d_g, (int)hg.size(), nbr_tot,
-                       B.d_adj, B.d_nbr, B.d_err, B.d_out, B.d_qhead);
+    hipLaunchKernelGGL(k_prepare_batch, dim3(pgrid), dim3(256), 0, C.stream, B.d_g, (int)B.hg.size(), nbr_tot,
+                       B.d_adj, B.d_nbr, B.d_err);
     HIPC(hipGetLastError());
-    timed_end(tp);
-    int per_cu = (int)std::max<size_t>(1, LDS_LIMIT / (lds + LDS_STATIC));
+  }
+  HIPC(hipMemcpyAsync(B.h_err, B.d_err, sizeof(int), hipMemcpyDeviceToHost, C.stream));
+  HIPC(hipStreamSynchronize(C.stream));
+  if (*B.h_err) ODP_ERROR("ODPX_Batch_load: adjacency holds a vertex id outside [0, nodes)\n");
+  return 0;
+}
+
+extern "C" int ODPX_Batch_run(int* diameter, long* sum, double* ASPL)
+{
+  ctx_init();
+  double t0 = wall_time();
+  if (!B.small.empty()) {
+    HIPC(hipMemsetAsync(B.d_out, 0, sizeof(ResOut) * B.small.size(), C.stream));
+    HIPC(hipMemsetAsync(B.d_qhead, 0, sizeof(int), C.stream));
+    int per_cu = (int)std::max<size_t>(1, LDS_LIMIT / (B.lds + LDS_STATIC));
     per_cu = std::min(per_cu, 2);
-    int grid = std::max(1, std::min((int)sorted.size(), per_cu * C.ncu));
+    int grid = std::max(1, std::min(B.ntasks, per_cu * C.ncu));
     TimedLaunch tr;
     timed_begin(tr, T_RESIDENT, 0);
-    hipLaunchKernelGGL(k_resident, dim3(grid), dim3(1024), lds, C.stream, B.d_t, (int)sorted.size(), B.d_qhead, B.d_out);
+    hipLaunchKernelGGL(k_resident, dim3(grid), dim3(1024), B.lds, C.stream, B.d_t, B.ntasks, B.d_qhead, B.d_out);
     HIPC(hipGetLastError());
     timed_end(tr);
-    HIPC(hipMemcpyAsync(B.h_out, B.d_out, sizeof(ResOut) * small.size(), hipMemcpyDeviceToHost, C.stream));
-    HIPC(hipMemcpyAsync(B.h_err, B.d_err, sizeof(int), hipMemcpyDeviceToHost, C.stream));
+    HIPC(hipMemcpyAsync(B.h_out, B.d_out, sizeof(ResOut) * B.small.size(), hipMemcpyDeviceToHost, C.stream));
     HIPC(hipStreamSynchronize(C.stream));
-    if (*B.h_err) ODP_ERROR("ODPX_Batch_eval: adjacency holds a vertex id outside [0, nodes)\n");
-    for (size_t k = 0; k < small.size(); k++) {
-      int g = small[k];
+    for (size_t k = 0; k < B.small.size(); k++) {
+      int g = B.small[k];
       const ResOut& o = B.h_out[k];
-      finish_one(nodes[g], o.sum, std::max(1, o.diam), (o.flags & 1) != 0, &diameter[g], &sum[g], &ASPL[g]);
+      finish_one(B.nodes[g], o.sum, std::max(1, o.diam), (o.flags & 1) != 0, &diameter[g], &sum[g], &ASPL[g]);
     }
     if (g_stats_on) {
       for (TimedLaunch& t : g_pending) {
         float ms = 0;
         HIPC(hipEventElapsedTime(&ms, t.a, t.b));
         g_evpool.push_back(t.a); g_evpool.push_back(t.b);
-        if (t.kind == T_RESIDENT) {
-          double bytes = 0;
-          for (size_t k = 0; k < small.size(); k++) {
-            const BatchGraph& G = hg[k];
-            int wb = batch_wb(G.n);
-            double row = (G.degree + 2.0) * (std::min(32 * wb, G.n) / 8.0) + 4.0 * G.degree;
-            bytes += (double)B.h_out[k].levels * G.n * row;
-          }
-          g_stats.resident_launches++; g_stats.resident_ms += ms; g_stats.resident_bytes += bytes;
-        } else { g_stats.other_launches++; g_stats.other_ms += ms; }
+        double bytes = 0;
+        for (size_t k = 0; k < B.small.size(); k++) {
+          const BatchGraph& G = B.hg[k];
+          double row = (G.degree + 2.0) * (std::min(32 * B.wb[k], G.n) / 8.0) + 4.0 * G.degree;
+          bytes += (double)B.h_out[k].levels * G.n * row;
+        }
+        g_stats.resident_launches++; g_stats.resident_ms += ms; g_stats.resident_bytes += bytes;
       }
       g_pending.clear();
-      g_stats.evals += (long long)small.size();
     }
   }
-  for (int g : large)
-    single_eval(B.single, nodes[g], degrees[g], packed + offsets[g], ODPX_PLAN_AUTO, &diameter[g], &sum[g], &ASPL[g]);
+  for (int g : B.large)
+    single_eval_dev(B.single, B.nodes[g], B.degrees[g], B.d_adj + B.offsets[g], ODPX_PLAN_AUTO,
+                    &diameter[g], &sum[g], &ASPL[g]);
+  if (g_stats_on) { g_stats.evals += B.ng; g_stats.eval_ms += 1e3 * (wall_time() - t0); }
   return 0;
+}
+
+extern "C" int ODPX_Loop_run(int plan, int* diameter, long* sum, double* ASPL)
+{
+  ctx_init();
+  for (int g = 0; g < B.ng; g++)
+    single_eval_dev(B.single, B.nodes[g], B.degrees[g], B.d_adj + B.offsets[g], plan, &diameter[g], &sum[g], &ASPL[g]);
+  return 0;
+}
+
+extern "C" int ODPX_Batch_eval(int ngraphs, const int* nodes, const int* degrees, const long long* offsets,
+                               const int* packed, int* diameter, long* sum, double* ASPL)
+{
+  ODPX_Batch_load(ngraphs, nodes, degrees, offsets, packed);
+  return ODPX_Batch_run(diameter, sum, ASPL);
 }
 
 extern "C" int ODPX_Loop_eval(int ngraphs, const int* nodes, const int* degrees, const long long* offsets,
                               const int* packed, int plan, int* diameter, long* sum, double* ASPL)
 {
-  ctx_init();
-  for (int g = 0; g < ngraphs; g++)
-    single_eval(B.single, nodes[g], degrees[g], packed + offsets[g], plan, &diameter[g], &sum[g], &ASPL[g]);
-  return 0;
+  ODPX_Batch_load(ngraphs, nodes, degrees, offsets, packed);
+  return ODPX_Loop_run(plan, diameter, sum, ASPL);
 }
 
 extern "C" void ODPX_Batch_release(void)
@@ -1039,5 +1095,6 @@ extern "C" void ODPX_Batch_release(void)
   dfree(&B.d_adj); dfree(&B.d_nbr); dfree(&B.d_err); dfree(&B.d_qhead); hfree(&B.h_err);
   dfree(&B.d_g); dfree(&B.d_out); hfree(&B.h_out); dfree(&B.d_t);
   B.adj_cap = B.nbr_cap = B.g_cap = B.t_cap = 0;
+  B.ng = 0; B.small.clear(); B.large.clear();
   B.single.release();
 }

@@ -77,7 +77,8 @@ struct PrepArgs {
   const int* adj;        // [(n/s)][degree]
   const int* numdeg;     // [n] expanded, or null
   const int* itable;     // [n] or null
-  int* nbr;              // [n][dpad]
+  int* nbr;              // [n][dpad] 32-bit ids, or null
+  unsigned short* nbr16; // [n][dpad] 16-bit ids (n <= 65536), or null
   int* err;
   int n, s, degree, dpad, width, height, grid_s;
   u64* zero64; int nzero64;   // counters reset to 0 at the start of an evaluation
@@ -146,7 +147,8 @@ __global__ void k_prepare(PrepArgs a)
       if (v < 0) atomicOr(a.err, 1);
       else out = a.grid_s ? a.itable[v] : v;
     }
-    a.nbr[(int64_t)row * a.dpad + j] = out;
+    if (a.nbr16) a.nbr16[(int64_t)row * a.dpad + j] = (unsigned short)out;
+    else a.nbr[(int64_t)row * a.dpad + j] = out;
   }
 }
 
@@ -307,24 +309,35 @@ __global__ __launch_bounds__(LEVEL_WG) void k_level(const LevelTask* __restrict_
 
 // ---------------------------------------------------------------------------
 // resident kernel
+//
+// One workgroup runs every level of one block of sources with the block's
+// bit matrix in LDS: u32[n][WB], a single buffer.  Each thread owns rows
+// tid, tid+nth, ... (at most RPT of them) and keeps their next values in
+// registers until the whole workgroup has read the level, then writes them
+// back, so one buffer of n*WB*4 bytes suffices (two barriers per level).
+// Neighbour ids are read as 16-bit row indices (graphs here have n <= 32768),
+// eight per 16-byte load.
 
 struct ResTask {
-  const int* nbr;   // [n][dpad]
-  int n, dpad;
-  int src0, l;      // first source row of the block, sources in it (<= 32*wb)
-  int s;            // symmetry factor of the graph
-  int graph;        // index into the per-graph outputs
-  int wb;           // 32-bit words per row of the block: 1, 2, 4, 8 or 16
+  const unsigned short* nbr;   // [n][dp] 16-bit row indices, padded with the row itself
+  int n, dp;                   // rows; ids per row (multiple of 8)
+  int src0, l;                 // first source row of the block, sources in it (<= 32*wb)
+  int s;                       // symmetry factor of the graph
+  int graph;                   // index into the per-graph outputs
+  int wb;                      // 32-bit words per row of the block: 1, 2, 4 or 8
   int pad;
 };
 
 struct ResOut {     // per-graph accumulators
-  u64 sum;          // sum over tasks of sum_L (n*l - count_L) * s   (the n(n-1) base excluded)
-  int diam;         // max over tasks of (terminal level + 1)
-  u32 flags;        // bit 0: a task found no new bits before completing (disconnected)
+  u64 sum;          // sum over blocks of sum_L (n*l - count_L) * s   (the n(n-1) base excluded)
+  int diam;         // max over blocks of (terminal level + 1)
+  u32 flags;        // bit 0: a block found no new bits before completing (disconnected)
   u32 levels;       // levels run, summed over the graph's blocks (for byte accounting)
   u32 pad;
 };
+
+// rows per thread held in registers for each width (at most 32 u32 of next rows)
+__host__ __device__ constexpr int res_rpt(int wb) { return wb == 1 ? 16 : (wb == 8 ? 2 : 32 / wb); }
 
 template <int WB> struct RowT { u32 w[WB]; };
 
@@ -364,10 +377,19 @@ template <int WB>
 __device__ __forceinline__ void row_bit(RowT<WB>& r, int c, int l)
 {
   if ((unsigned)c < (unsigned)l) {
+    const int k = c >> 5;
+    const u32 bit = 1u << (c & 31);
 #pragma unroll
-    for (int k = 0; k < WB; k++)
-      if ((c >> 5) == k) r.w[k] |= 1u << (c & 31);
+    for (int i = 0; i < WB; i++) r.w[i] |= (i == k) ? bit : 0u;
   }
+}
+
+template <int WB>
+__device__ __forceinline__ void row_or(RowT<WB>& r, const u32* cur, int v)
+{
+  const RowT<WB> a = lds_row<WB>(cur + v * WB);
+#pragma unroll
+  for (int k = 0; k < WB; k++) r.w[k] |= a.w[k];
 }
 
 // No divergent branch may sit in a loop that contains a barrier.  With an
@@ -384,7 +406,7 @@ __device__ __forceinline__ u64 uniform_u64(u64 v)
   return ((u64)hi << 32) | lo;
 }
 
-__device__ __forceinline__ u64 block_total(u64 v, u64* part, int lane, int wave, int nw)
+__device__ __forceinline__ u64 block_total(u64 v, u64* part, int wave, int nw)
 {
   v = wave_sum(v);      // every lane holds the wave total: all write the same value
   part[wave] = v;
@@ -394,13 +416,12 @@ __device__ __forceinline__ u64 block_total(u64 v, u64* part, int lane, int wave,
   return uniform_u64(s);
 }
 
-// All levels of one block of T.l <= 32*WB sources, both bit buffers in LDS.
+// All levels of one block of T.l <= 32*WB sources.
 template <int WB>
-__device__ __forceinline__ void res_run(const ResTask& T, u32* lds, u64 (*s_part)[16], ResOut* out,
+__device__ __forceinline__ void res_run(const ResTask& T, u32* cur, u64 (*s_part)[16], ResOut* out,
                                         int tid, int nth, int lane, int wave, int nw)
 {
-  u32* cur = lds;
-  u32* nxt = lds + (int64_t)T.n * WB;
+  constexpr int RPT = res_rpt(WB);
   const u64 target = (u64)T.n * (u64)T.l;
   RowT<WB> full;
 #pragma unroll
@@ -408,27 +429,28 @@ __device__ __forceinline__ void res_run(const ResTask& T, u32* lds, u64 (*s_part
     const int v = T.l - 32 * k;
     full.w[k] = v >= 32 ? ~0u : (v <= 0 ? 0u : ((1u << v) - 1u));
   }
-  // level 0 from the identity
+  // level 0 straight from the identity: row r holds r and its neighbours
   u64 pop = 0;
   for (int r = tid; r < T.n; r += nth) {
     RowT<WB> row;
 #pragma unroll
     for (int k = 0; k < WB; k++) row.w[k] = 0;
     row_bit<WB>(row, r - T.src0, T.l);
-    const int* nb = T.nbr + (int64_t)r * T.dpad;
-    for (int j = 0; j < T.dpad; j += 4) {
-      const int4 v = *(const int4*)(nb + j);
-      row_bit<WB>(row, v.x - T.src0, T.l);
-      row_bit<WB>(row, v.y - T.src0, T.l);
-      row_bit<WB>(row, v.z - T.src0, T.l);
-      row_bit<WB>(row, v.w - T.src0, T.l);
+    const uint4* ids = (const uint4*)T.nbr;     // uniform base, 32-bit offsets
+    const u32 off = (u32)r * (u32)(T.dp >> 3);
+    for (int j = 0; j < (T.dp >> 3); j++) {
+      const uint4 v = ids[off + j];
+      row_bit<WB>(row, (int)(v.x & 0xffff) - T.src0, T.l); row_bit<WB>(row, (int)(v.x >> 16) - T.src0, T.l);
+      row_bit<WB>(row, (int)(v.y & 0xffff) - T.src0, T.l); row_bit<WB>(row, (int)(v.y >> 16) - T.src0, T.l);
+      row_bit<WB>(row, (int)(v.z & 0xffff) - T.src0, T.l); row_bit<WB>(row, (int)(v.z >> 16) - T.src0, T.l);
+      row_bit<WB>(row, (int)(v.w & 0xffff) - T.src0, T.l); row_bit<WB>(row, (int)(v.w >> 16) - T.src0, T.l);
     }
-    lds_store<WB>(nxt + (int64_t)r * WB, row);
+    lds_store<WB>(cur + r * WB, row);
 #pragma unroll
     for (int k = 0; k < WB; k++) pop += (u64)__popc(row.w[k]);
   }
   int par = 0;
-  u64 c = block_total(pop, s_part[par], lane, wave, nw);
+  u64 c = block_total(pop, s_part[par], wave, nw);
   u64 prev = 0, sum = 0;
   int L = 0, D = 0;
   bool disc = false;
@@ -438,31 +460,40 @@ __device__ __forceinline__ void res_run(const ResTask& T, u32* lds, u64 (*s_part
     sum += (target - c) * (u64)T.s;
     prev = c;
     L++;
-    u32* tmp = cur; cur = nxt; nxt = tmp;
+    RowT<WB> nx[RPT];
     pop = 0;
-    for (int r = tid; r < T.n; r += nth) {
-      RowT<WB> row = lds_row<WB>(cur + (int64_t)r * WB);
-      bool sat = true;
 #pragma unroll
-      for (int k = 0; k < WB; k++) sat = sat && (row.w[k] == full.w[k]);
-      if (!sat) {
-        const int* nb = T.nbr + (int64_t)r * T.dpad;
-        for (int j = 0; j < T.dpad; j += 4) {
-          const int4 v = *(const int4*)(nb + j);
-          const RowT<WB> a = lds_row<WB>(cur + (int64_t)v.x * WB);
-          const RowT<WB> b = lds_row<WB>(cur + (int64_t)v.y * WB);
-          const RowT<WB> cc = lds_row<WB>(cur + (int64_t)v.z * WB);
-          const RowT<WB> d = lds_row<WB>(cur + (int64_t)v.w * WB);
+    for (int i = 0; i < RPT; i++) {
+      const int r = tid + i * nth;
+      if (r < T.n) {
+        RowT<WB> row = lds_row<WB>(cur + r * WB);
+        bool sat = true;
 #pragma unroll
-          for (int k = 0; k < WB; k++) row.w[k] |= (a.w[k] | b.w[k]) | (cc.w[k] | d.w[k]);
+        for (int k = 0; k < WB; k++) sat = sat && (row.w[k] == full.w[k]);
+        if (!sat) {
+          const uint4* ids = (const uint4*)T.nbr;
+          const u32 off = (u32)r * (u32)(T.dp >> 3);
+          for (int j = 0; j < (T.dp >> 3); j++) {
+            const uint4 v = ids[off + j];
+            row_or<WB>(row, cur, v.x & 0xffff); row_or<WB>(row, cur, v.x >> 16);
+            row_or<WB>(row, cur, v.y & 0xffff); row_or<WB>(row, cur, v.y >> 16);
+            row_or<WB>(row, cur, v.z & 0xffff); row_or<WB>(row, cur, v.z >> 16);
+            row_or<WB>(row, cur, v.w & 0xffff); row_or<WB>(row, cur, v.w >> 16);
+          }
         }
-      }
-      lds_store<WB>(nxt + (int64_t)r * WB, row);
+        nx[i] = row;
 #pragma unroll
-      for (int k = 0; k < WB; k++) pop += (u64)__popc(row.w[k]);
+        for (int k = 0; k < WB; k++) pop += (u64)__popc(row.w[k]);
+      }
     }
     par ^= 1;
-    c = block_total(pop, s_part[par], lane, wave, nw);
+    c = block_total(pop, s_part[par], wave, nw);   // every read of this level is done
+#pragma unroll
+    for (int i = 0; i < RPT; i++) {
+      const int r = tid + i * nth;
+      if (r < T.n) lds_store<WB>(cur + r * WB, nx[i]);
+    }
+    __syncthreads();                               // the new level is visible
   }
   if (wave == 0) {       // wave-uniform branch; only lane 0 contributes
     const bool first = lane == 0;
@@ -475,8 +506,9 @@ __device__ __forceinline__ void res_run(const ResTask& T, u32* lds, u64 (*s_part
 
 // Persistent: every workgroup pulls blocks from *qhead until none is left, so
 // the grid drains once the queue is exhausted.  Blocks of any width run in the
-// same launch (T.wb picks the instantiation); the launch's dynamic LDS must hold
-// 2 * n * wb * 4 bytes for its largest block.
+// same launch (T.wb picks the instantiation).  The launch's dynamic LDS must
+// hold n * wb * 4 bytes for its largest block, and blockDim.x * res_rpt(wb)
+// must cover n for every block.
 __global__ __launch_bounds__(1024) void k_resident(const ResTask* __restrict__ tasks, int ntasks,
                                                    int* __restrict__ qhead, ResOut* __restrict__ out)
 {
@@ -498,8 +530,7 @@ __global__ __launch_bounds__(1024) void k_resident(const ResTask* __restrict__ t
       case 1: res_run<1>(T, lds, s_part, out, tid, nth, lane, wave, nw); break;
       case 2: res_run<2>(T, lds, s_part, out, tid, nth, lane, wave, nw); break;
       case 4: res_run<4>(T, lds, s_part, out, tid, nth, lane, wave, nw); break;
-      case 8: res_run<8>(T, lds, s_part, out, tid, nth, lane, wave, nw); break;
-      default: res_run<16>(T, lds, s_part, out, tid, nth, lane, wave, nw); break;
+      default: res_run<8>(T, lds, s_part, out, tid, nth, lane, wave, nw); break;
     }
   }
 }

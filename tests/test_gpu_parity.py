@@ -26,13 +26,14 @@ PLANS = [
     None,
     dict(plan=RES, wb=1, threads=256),
     dict(plan=RES, wb=2, threads=1024),
-    dict(plan=RES, wb=16, threads=512),
+    dict(plan=RES, wb=8, threads=512),
     dict(plan=LEV, kw=0, upw=4, seg_major=1, xcd=0),
     dict(plan=LEV, kw=1, upw=1, seg_major=0, xcd=1),
     dict(plan=LEV, kw=3, upw=2, seg_major=1, xcd=1),
     dict(plan=LEV, kw=64, upw=8, seg_major=0, xcd=0),
     dict(plan=LEV, kw=2, upw=2, seg_major=0, xcd=1, group=2),
     dict(plan=LEV, kw=1, upw=16, seg_major=1, xcd=0, group=1),
+    dict(plan=RES, wb=4, threads=1024),
 ]
 
 
@@ -273,6 +274,9 @@ def test_batch_vs_oracle():
     assert (d2 == d).all() and (s2 == s).all() and (a2.view(np.int64) == a.view(np.int64)).all()
     d3, s3, a3 = odp.loop_eval(packed, odp.PLAN_AUTO)
     assert (d3 == d).all() and (s3 == s).all()
+    ng = odp.batch_load(packed)
+    d4, s4, a4 = odp.batch_run(ng)
+    assert (d4 == d).all() and (s4 == s).all() and (a4.view(np.int64) == a.view(np.int64)).all()
     # alone vs in the mix
     for g in (0, 17, 48):
         one = odp.batch_eval(odp.pack([graphs[g]]))
