@@ -17,9 +17,12 @@ profiles/), or null.
 cpu_baseline: the reference library's OpenMP build (oracle/_ref, compiled from
 the reference sources) timed on this host on the same graph.
 
-Extra objects: "batched" (1024 seeded 2-opt candidates with n in [1024, 16384]
-and d in {4, 8, 12, 16, 24}, one persistent launch, against looping the
-single-graph GPU path) unless --no-extras.
+Extra objects (random65536, unless --no-extras): "batched" (1024 seeded 2-opt
+candidates with n in [1024, 16384] and d in {4, 8, 12, 16, 24}, resident in HBM,
+one persistent launch, against looping the single-graph GPU path) and "served"
+(64 annealing chains through the serving engine: saturated rate, then Poisson
+arrivals with p50/p99 latency).  Sampled batched and served results are checked
+bit for bit against the reference in the cpu_baseline leg.
 """
 import argparse
 import json
@@ -57,12 +60,21 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(adj, gkw, kind_name, budget_s=20.0):
-    """Reference OpenMP library on a bounded sample of the same workload."""
+def cpu_baseline(adj, gkw, kind_name, budget_s=20.0, checks=()):
+    """Reference OpenMP library on a bounded sample of the same workload; also the
+    reference's results for sampled batched / served graphs (`checks`: (graph,
+    HIP result) pairs), compared bit for bit."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
     if not oracle.ref_available(threads=True):
         return None, None
+    cores0 = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
+    matched = 0
+    for g, res in checks:
+        (d, sm, a), _ = oracle.run_reference(g, nodes=g.shape[0], threads=True, omp_threads=cores0)
+        if (d, sm) != tuple(res[:2]) or a != res[2]:
+            raise SystemExit(f"sampled result {res} differs from the reference's {(d, sm, a)}")
+        matched += 1
     kind = {"MATRIX": oracle.MATRIX, "MATRIX_SAVING": oracle.MATRIX_SAVING}[kind_name]
     cores = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
     (d, s, a), t1 = oracle.run_reference(adj, kind=kind, threads=True, repeat=1, omp_threads=cores, **gkw)
@@ -72,7 +84,8 @@ def cpu_baseline(adj, gkw, kind_name, budget_s=20.0):
     med = statistics.median(ts)
     return ({"value": 1.0 / med, "unit": "evals/s", "cores": cores, "kind": "reference",
              "sample": f"{reps} timed ODP_Set_aspl calls (after 1 warm-up) of the reference OpenMP library, "
-                       f"{kind_name}, on the same graph; median {med:.6f} s"},
+                       f"{kind_name}, on the same graph; median {med:.6f} s",
+             "sampled_batched_served_results_matching_reference": matched},
             (d, s, a))
 
 
@@ -104,6 +117,88 @@ def c4_graphs(count, seed=2024):
     return graphs
 
 
+def served_extra(nchains=64, saturate_s=4.0, poisson_s=6.0, load=0.6, seed=77):
+    """64 annealing chains, n in {4096, 16384, 65536} x d in {8, 16, 32}, each request one
+    random 2-opt swap (accepted: the next request applies to the swapped graph)."""
+    ns, ds = (4096, 16384, 65536), (8, 16, 32)
+    t0 = time.perf_counter()
+    mirrors = [odp.generate_regular(ns[i % 3], ds[(i // 3) % 3], 5000 + i) for i in range(nchains)]
+    log(f"served: generated {nchains} chains in {time.perf_counter() - t0:.1f}s")
+    rng = odp.Rng(seed)
+    srv = odp.Server(small_workers=2, large_workers=4, hbm_budget_gb=64)
+    samples = []            # (graph copy, request id) checked against the reference later
+    want_sample = {2: None, 3: None, 5: None}    # chains 2 (n=65536), 3 (n=4096), 5 (n=65536)
+    try:
+        cids = [srv.add_chain(m) for m in mirrors]
+        sent = [0] * nchains
+
+        def submit(i):
+            sw = odp.two_opt(mirrors[i], rng)
+            rid = srv.submit(cids[i], [sw])
+            sent[i] += 1
+            if i in want_sample and want_sample[i] is None and sent[i] == 3:
+                want_sample[i] = rid
+                samples.append((mirrors[i].copy(), rid))
+            return rid
+
+        # closed loop: every chain keeps one request in flight
+        results = {}
+        for i in range(nchains):
+            submit(i)
+        t = time.perf_counter()
+        done = 0
+        while time.perf_counter() - t < saturate_s:
+            for r in srv.poll():
+                results[r["id"]] = r
+                done += 1
+                submit(r["chain"])
+        srv.drain()
+        for r in srv.poll():
+            results[r["id"]] = r
+        sat_rate = done / saturate_s
+        # open loop: Poisson arrivals at `load` of the saturated rate, chains uniform
+        rate = max(1.0, load * sat_rate)
+        arr = np.random.default_rng(seed)
+        lat, starts = [], {}
+        t = time.perf_counter()
+        next_t = arr.exponential(1.0 / rate)
+        submitted = 0
+        while True:
+            now = time.perf_counter() - t
+            if now >= poisson_s:
+                break
+            if now >= next_t:
+                rid = submit(int(arr.integers(nchains)))
+                starts[rid] = True
+                submitted += 1
+                next_t += arr.exponential(1.0 / rate)
+            for r in srv.poll():
+                results[r["id"]] = r
+                if r["id"] in starts:
+                    lat.append(r["t_done"] - r["t_submit"])
+            if next_t - (time.perf_counter() - t) > 2e-4:
+                time.sleep(1e-4)
+        srv.drain()
+        elapsed = time.perf_counter() - t
+        for r in srv.poll():
+            results[r["id"]] = r
+            if r["id"] in starts:
+                lat.append(r["t_done"] - r["t_submit"])
+        bad = sum(1 for r in results.values() if r["status"] != 0)
+        lat_ms = np.array(lat) * 1e3
+        sample_checks = [(g, (results[rid]["diameter"], results[rid]["sum"], results[rid]["ASPL"]))
+                         for g, rid in samples]
+        return ({"chains": nchains, "saturated_evals_per_s": sat_rate,
+                 "poisson_rate_per_s": rate, "poisson_evals_per_s": len(lat) / elapsed,
+                 "p50_ms": float(np.percentile(lat_ms, 50)), "p99_ms": float(np.percentile(lat_ms, 99)),
+                 "requests": len(results), "failed": bad,
+                 "workload": "64 chains, n in {4096,16384,65536} x d in {8,16,32}, one accepted 2-opt swap per "
+                             "request; closed loop for the saturated rate, then Poisson arrivals at 60% of it"},
+                sample_checks)
+    finally:
+        srv.stop()
+
+
 def batched_extra(count, loops):
     t0 = time.perf_counter()
     graphs = c4_graphs(count)
@@ -129,7 +224,8 @@ def batched_extra(count, loops):
     la = odp.loop_run(ng, odp.PLAN_AUTO)
     ta = time.perf_counter() - t
     same = all((x == y).all() for x, y in zip(lp, ref)) and all((x == y).all() for x, y in zip(la, ref))
-    return {"graphs": count, "evals_per_s": count / tb, "ms_per_batch": 1e3 * tb,
+    checks = [(graphs[g], (int(ref[0][g]), int(ref[1][g]), float(ref[2][g]))) for g in (0, count // 2)]
+    return checks, {"graphs": count, "evals_per_s": count / tb, "ms_per_batch": 1e3 * tb,
             "loop_level_evals_per_s": count / tl, "loop_auto_evals_per_s": count / ta,
             "speedup_vs_loop_level": tl / tb, "speedup_vs_loop_auto": ta / tb,
             "identical_results": bool(same), "load_s": t_load,
@@ -240,8 +336,13 @@ def main():
                       "diameter": first[0], "sum": first[1], "aspl": first[2],
                       "pcie_inclusive_evals_per_s": round(pcie_rate, 3)},
            "roofline": roofline}
+    checks = []
+    if not args.no_extras and args.workload == "random65536":
+        bchecks, out["batched"] = batched_extra(1024, 3)
+        out["served"], schecks = served_extra()
+        checks = bchecks + schecks
     if not args.no_cpu:
-        cb, cres = cpu_baseline(adj, gkw, kind_name)
+        cb, cres = cpu_baseline(adj, gkw, kind_name, checks=checks)
         if cb is not None:
             if cres[:2] != first[:2] or cres[2] != first[2]:
                 raise SystemExit(f"reference {cres} differs from HIP {first}")
@@ -249,8 +350,6 @@ def main():
         out["cpu_baseline"] = cb
     else:
         out["cpu_baseline"] = None
-    if not args.no_extras and args.workload == "random65536":
-        out["batched"] = batched_extra(1024, 3)
     print(json.dumps(out), flush=True)
 
 

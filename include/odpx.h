@@ -7,7 +7,7 @@
  *  - plan control, autotuning and kernel timing of the ODP_* evaluator;
  *  - device-resident evaluation (the adjacency already in HBM);
  *  - batched evaluation of many graphs of uneven size in one persistent launch;
- *  - a serving engine with continuous batching of evaluation requests.
+ *  - a serving engine with continuous batching of evaluation requests (serve.hpp).
  * Every evaluation returns the same integers as ODP_Set_aspl would.
  */
 #ifndef ODP_HIP_ODPX_H
@@ -113,6 +113,39 @@ int ODPX_Loop_eval(int ngraphs, const int *nodes, const int *degrees, const long
                    const int *packed_adjacency, int plan, int *diameter, long *sum, double *ASPL);
 /* Frees the device buffers the batched and looped paths keep between calls. */
 void ODPX_Batch_release(void);
+
+/* ---- serving: continuous batching of 2-opt evaluation requests --------- */
+/* A chain is one graph (general, s = 1) kept in HBM; a request applies a list
+ * of 2-opt swaps (a-b, c-d -> a-c, b-d, each as {a, b, c, d}) to its chain and
+ * evaluates the result exactly.  Requests of one chain run in submission order;
+ * requests of different chains are batched and overlapped on the GPU as
+ * workers free up.  Results equal ODP_Set_aspl on the same graph. */
+typedef struct {
+  long long id;               /* request id returned by ODPX_Serve_submit            */
+  int    chain;
+  int    status;              /* 0, or -1 if a swap named an edge the graph lacks    */
+  int    diameter;
+  long   sum;
+  double ASPL;
+  double t_submit, t_start, t_done;   /* seconds on the host's monotonic clock     */
+} ODPX_Result;
+/* small_workers: launchers of LDS-resident batches; large_workers: evaluators of
+ * graphs too large for LDS, each holding its own bit buffers; hbm_budget_gb:
+ * bound on chain tables plus those buffers.  0 on success. */
+int  ODPX_Serve_start(int small_workers, int large_workers, double hbm_budget_gb);
+/* adjacency int[nodes][degree]; returns the chain id, or -1 (over the HBM budget,
+ * invalid graph, or no server) */
+int  ODPX_Serve_add_chain(int nodes, int degree, const int *adjacency);
+/* returns the request id (>= 1) or -1 */
+long long ODPX_Serve_submit(int chain, int nswaps, const int *swaps);
+/* up to max completed results, in completion order; returns how many */
+int  ODPX_Serve_poll(ODPX_Result *out, int max);
+/* blocks until request id completes; 0, or -1 if the id is unknown */
+int  ODPX_Serve_wait(long long id, ODPX_Result *out);
+/* blocks until every submitted request has completed */
+void ODPX_Serve_drain(void);
+/* drains, stops the workers and frees the chains */
+void ODPX_Serve_stop(void);
 
 #ifdef __cplusplus
 }

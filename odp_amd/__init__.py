@@ -44,6 +44,16 @@ class Stats(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class Result(ctypes.Structure):
+    """ODPX_Result (include/odpx.h)."""
+    _fields_ = [("id", ctypes.c_longlong), ("chain", ctypes.c_int), ("status", ctypes.c_int),
+                ("diameter", ctypes.c_int), ("sum", ctypes.c_long), ("ASPL", ctypes.c_double),
+                ("t_submit", ctypes.c_double), ("t_start", ctypes.c_double), ("t_done", ctypes.c_double)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 _lib = None
 
 _I = ctypes.c_int
@@ -114,6 +124,13 @@ def load():
     _sig(lib, "ODPX_Loop_run", _I, [_I, _P, _P, _P])
     _sig(lib, "ODPX_Loop_eval", _I, [_I, _P, _P, _P, _P, _I, _P, _P, _P])
     _sig(lib, "ODPX_Batch_release", None, [])
+    _sig(lib, "ODPX_Serve_start", _I, [_I, _I, _D])
+    _sig(lib, "ODPX_Serve_add_chain", _I, [_I, _I, _P])
+    _sig(lib, "ODPX_Serve_submit", ctypes.c_longlong, [_I, _I, _P])
+    _sig(lib, "ODPX_Serve_poll", _I, [ctypes.POINTER(Result), _I])
+    _sig(lib, "ODPX_Serve_wait", _I, [ctypes.c_longlong, ctypes.POINTER(Result)])
+    _sig(lib, "ODPX_Serve_drain", None, [])
+    _sig(lib, "ODPX_Serve_stop", None, [])
     _lib = lib
     return lib
 
@@ -381,3 +398,45 @@ def read_edges_grid(path):
     degree = lib.ODP_Get_degree(nodes, lines, _ptr(edge))
     length = lib.ODP_Get_length(lines, h.value, _ptr(edge))
     return edge, w.value, h.value, degree, length
+
+
+# ---- serving ------------------------------------------------------------------
+
+class Server:
+    """ODPX_Serve_*: continuous batching of 2-opt evaluation requests over
+    chains whose graphs stay resident in HBM."""
+
+    def __init__(self, small_workers=2, large_workers=4, hbm_budget_gb=64.0):
+        if load().ODPX_Serve_start(small_workers, large_workers, hbm_budget_gb) != 0:
+            raise RuntimeError("a server is already running")
+
+    def add_chain(self, adjacency):
+        a = _i32(adjacency)
+        cid = load().ODPX_Serve_add_chain(a.shape[0], a.shape[1], _ptr(a))
+        if cid < 0:
+            raise MemoryError("chain refused (HBM budget or invalid graph)")
+        return cid
+
+    def submit(self, chain, swaps=()):
+        sw = _i32(np.array(swaps, dtype=np.int32).reshape(-1, 4)) if len(swaps) else np.zeros((0, 4), np.int32)
+        rid = load().ODPX_Serve_submit(chain, len(sw), _ptr(sw))
+        if rid < 0:
+            raise ValueError("request refused")
+        return rid
+
+    def poll(self, max_results=4096):
+        buf = (Result * max_results)()
+        k = load().ODPX_Serve_poll(buf, max_results)
+        return [buf[i].as_dict() for i in range(k)]
+
+    def wait(self, rid):
+        r = Result()
+        if load().ODPX_Serve_wait(rid, ctypes.byref(r)) != 0:
+            raise KeyError(rid)
+        return r.as_dict()
+
+    def drain(self):
+        load().ODPX_Serve_drain()
+
+    def stop(self):
+        load().ODPX_Serve_stop()

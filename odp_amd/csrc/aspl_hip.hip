@@ -109,16 +109,18 @@ static hipEvent_t ev_get()
   HIPC(hipEventCreate(&e));
   return e;
 }
-static void timed_begin(TimedLaunch& t, int kind, int level)
+// Only evaluations on the library's main stream (the ODP_* lifecycle, the
+// batched and looped paths) are timed; serving workers run untimed.
+static void timed_begin(TimedLaunch& t, int kind, int level, hipStream_t st = nullptr)
 {
   t.kind = kind; t.level = level;
-  if (!g_stats_on) return;
+  if (!g_stats_on || (st && st != C.stream)) { t.a = nullptr; return; }
   t.a = ev_get(); t.b = ev_get();
   HIPC(hipEventRecord(t.a, C.stream));
 }
 static void timed_end(TimedLaunch& t)
 {
-  if (!g_stats_on) return;
+  if (!t.a) return;
   HIPC(hipEventRecord(t.b, C.stream));
   g_pending.push_back(t);
 }
@@ -160,6 +162,8 @@ struct Evaluator {
   LevelRun lr;
   ResRun rr;
   int guess_levels = 0;                 // levels the last level-plan evaluation needed
+  hipStream_t st = nullptr;             // launch stream (the library's main stream unless set)
+  bool timed() const { return g_stats_on && st == C.stream; }
 
   size_t adj_elems() const { return (size_t)(n / s) * degree; }
 
@@ -179,6 +183,7 @@ struct Evaluator {
   void shape(int fam, int nodes, int deg, int sym, int w, int h, const int* numdeg_full, int gs)
   {
     ctx_init();
+    if (!st) st = C.stream;
     family = fam; n = nodes; degree = deg; s = sym; width = w; height = h; grid_s = gs;
     nsrc = n / s;
     dpad = round_up(deg > 0 ? deg : 1, 4);
@@ -192,7 +197,7 @@ struct Evaluator {
       size_t ne16 = (size_t)n * dp16;
       if (ne16 > nbr16_cap) { dfree(&d_nbr16); dalloc(&d_nbr16, ne16); nbr16_cap = ne16; }
     }
-    if (!d_err) { dalloc(&d_err, 1); halloc(&h_err, 1); HIPC(hipMemsetAsync(d_err, 0, sizeof(int), C.stream)); }
+    if (!d_err) { dalloc(&d_err, 1); halloc(&h_err, 1); HIPC(hipMemsetAsync(d_err, 0, sizeof(int), st)); }
     dfree(&d_numdeg);
     dfree(&d_itable);
     avg_deg = deg;
@@ -215,7 +220,7 @@ struct Evaluator {
   {
     trace("upload");
     memcpy(h_stage, adjacency, sizeof(int) * adj_elems());
-    HIPC(hipMemcpyAsync(d_adj, h_stage, sizeof(int) * adj_elems(), hipMemcpyHostToDevice, C.stream));
+    HIPC(hipMemcpyAsync(d_adj, h_stage, sizeof(int) * adj_elems(), hipMemcpyHostToDevice, st));
   }
 
   // ---- plans ----
@@ -377,8 +382,8 @@ struct Evaluator {
     int64_t total = (int64_t)n * a.dpad;
     int grid = (int)std::min<int64_t>(std::max<int64_t>(1, (total + 255) / 256), 4096);
     TimedLaunch tl;
-    timed_begin(tl, T_OTHER, -1);
-    hipLaunchKernelGGL(k_prepare, dim3(grid), dim3(256), 0, C.stream, a);
+    timed_begin(tl, T_OTHER, -1, st);
+    hipLaunchKernelGGL(k_prepare, dim3(grid), dim3(256), 0, st, a);
     HIPC(hipGetLastError());
     timed_end(tl);
   }
@@ -388,7 +393,7 @@ struct Evaluator {
     if (*h_err) {
       int e = *h_err;
       *h_err = 0;
-      HIPC(hipMemsetAsync(d_err, 0, sizeof(int), C.stream));
+      HIPC(hipMemsetAsync(d_err, 0, sizeof(int), st));
       if (e & 2) ODP_ERROR("num_degrees holds a value outside [0, degree]\n");
       ODP_ERROR("adjacency holds a vertex id outside [0, nodes) within num_degrees\n");
     }
@@ -403,7 +408,7 @@ struct Evaluator {
     trace("run_level");
     prepare(r.cnt, r.ntasks, r.done, r.ntasks);   // ring slot 0 and the done flags
     struct St { int running; u64 sum, prev; int lwork; };
-    std::vector<St> st(r.ntasks, St{1, 0, 0, 0});
+    std::vector<St> sts(r.ntasks, St{1, 0, 0, 0});
     bool any_disc = false;
     int max_d = 1, lmax = 0;
     for (int first = 0; first < r.ntasks; first += r.group) {
@@ -412,8 +417,8 @@ struct Evaluator {
       const int grid = cnt_t * r.per_task_wg;
       {
         TimedLaunch tl;
-        timed_begin(tl, T_OTHER, 0);
-        hipLaunchKernelGGL(k_level0, dim3(grid), dim3(LEVEL_WG), 0, C.stream,
+        timed_begin(tl, T_OTHER, 0, st);
+        hipLaunchKernelGGL(k_level0, dim3(grid), dim3(LEVEL_WG), 0, st,
                            tasks, r.wg_start, cnt_t, RING, r.upw, r.xcd);
         HIPC(hipGetLastError());
         timed_end(tl);
@@ -424,24 +429,24 @@ struct Evaluator {
       for (;;) {
         for (int L = launched; L < batch_end; L++) {
           TimedLaunch tl;
-          timed_begin(tl, T_LEVEL, L);
+          timed_begin(tl, T_LEVEL, L, st);
           if (r.urow)
-            hipLaunchKernelGGL(k_level<true>, dim3(grid), dim3(LEVEL_WG), 0, C.stream,
+            hipLaunchKernelGGL(k_level<true>, dim3(grid), dim3(LEVEL_WG), 0, st,
                                tasks, r.wg_start, cnt_t, L, RING, r.upw, r.xcd);
           else
-            hipLaunchKernelGGL(k_level<false>, dim3(grid), dim3(LEVEL_WG), 0, C.stream,
+            hipLaunchKernelGGL(k_level<false>, dim3(grid), dim3(LEVEL_WG), 0, st,
                                tasks, r.wg_start, cnt_t, L, RING, r.upw, r.xcd);
           HIPC(hipGetLastError());
           timed_end(tl);
         }
         launched = std::max(launched, batch_end);
-        HIPC(hipMemcpyAsync(r.h_cnt, r.cnt, sizeof(u64) * RING * r.ntasks, hipMemcpyDeviceToHost, C.stream));
-        HIPC(hipMemcpyAsync(h_err, d_err, sizeof(int), hipMemcpyDeviceToHost, C.stream));
-        HIPC(hipStreamSynchronize(C.stream));
+        HIPC(hipMemcpyAsync(r.h_cnt, r.cnt, sizeof(u64) * RING * r.ntasks, hipMemcpyDeviceToHost, st));
+        HIPC(hipMemcpyAsync(h_err, d_err, sizeof(int), hipMemcpyDeviceToHost, st));
+        HIPC(hipStreamSynchronize(st));
         trace("level batch synchronized");
         check_err();
         for (int t = first; t < first + cnt_t; t++) {
-          St& q = st[t];
+          St& q = sts[t];
           if (!q.running) continue;
           const u64 target = (u64)n * (u64)r.l[t];
           for (int L = examined; L < launched; L++) {
@@ -460,21 +465,21 @@ struct Evaluator {
         batch_end = std::min(n, launched + step);
       }
       if (any_disc) break;          // the result is the sentinel whatever the other blocks give
-      for (int t = first; t < first + cnt_t; t++) lmax = std::max(lmax, st[t].lwork);
+      for (int t = first; t < first + cnt_t; t++) lmax = std::max(lmax, sts[t].lwork);
       guess_levels = lmax + 1;
-      if (g_stats_on) flush_level_stats(st.data(), first, cnt_t);
+      if (timed()) flush_level_stats(sts.data(), first, cnt_t);
     }
     u64 tot = 0;
-    for (int t = 0; t < r.ntasks; t++) tot += st[t].sum;
+    for (int t = 0; t < r.ntasks; t++) tot += sts[t].sum;
     *excess = tot; *diam = max_d; *disc = any_disc;
-    if (g_stats_on) flush_level_stats(st.data(), 0, 0);
+    if (timed()) flush_level_stats(sts.data(), 0, 0);
   }
 
   template <typename St>
-  void flush_level_stats(const St* st, int first, int cnt_t)
+  void flush_level_stats(const St* stv, int first, int cnt_t)
   {
     LevelRun& r = lr;
-    HIPC(hipStreamSynchronize(C.stream));
+    HIPC(hipStreamSynchronize(st));
     for (TimedLaunch& tl : g_pending) {
       float ms = 0;
       HIPC(hipEventElapsedTime(&ms, tl.a, tl.b));
@@ -482,7 +487,7 @@ struct Evaluator {
       if (tl.kind == T_LEVEL) {
         double bytes = 0;
         for (int t = first; t < first + cnt_t; t++)
-          if (tl.level <= st[t].lwork)
+          if (tl.level <= stv[t].lwork)
             bytes += (double)n * ((avg_deg + 2.0) * ((r.l[t] + 63) / 64) * 8.0 + 4.0 * avg_deg);
         if (bytes > 0) { g_stats.level_launches++; g_stats.level_ms += ms; g_stats.level_bytes += bytes; }
         else { g_stats.idle_launches++; g_stats.idle_ms += ms; }
@@ -499,20 +504,20 @@ struct Evaluator {
     trace("run_resident");
     prepare((u64*)r.out, (int)(sizeof(ResOut) / 8), (u32*)r.qhead, 1);
     TimedLaunch tl;
-    timed_begin(tl, T_RESIDENT, 0);
-    hipLaunchKernelGGL(k_resident, dim3(r.grid), dim3(r.threads), r.lds, C.stream, r.tasks, r.ntasks, r.qhead, r.out);
+    timed_begin(tl, T_RESIDENT, 0, st);
+    hipLaunchKernelGGL(k_resident, dim3(r.grid), dim3(r.threads), r.lds, st, r.tasks, r.ntasks, r.qhead, r.out);
     HIPC(hipGetLastError());
     timed_end(tl);
     trace("resident launched");
-    HIPC(hipMemcpyAsync(r.h_out, r.out, sizeof(ResOut), hipMemcpyDeviceToHost, C.stream));
-    HIPC(hipMemcpyAsync(h_err, d_err, sizeof(int), hipMemcpyDeviceToHost, C.stream));
-    HIPC(hipStreamSynchronize(C.stream));
+    HIPC(hipMemcpyAsync(r.h_out, r.out, sizeof(ResOut), hipMemcpyDeviceToHost, st));
+    HIPC(hipMemcpyAsync(h_err, d_err, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPC(hipStreamSynchronize(st));
     trace("resident synchronized");
     check_err();
     *excess = r.h_out->sum;
     *diam = std::max(1, r.h_out->diam);
     *disc = (r.h_out->flags & 1) != 0;
-    if (g_stats_on) {
+    if (timed()) {
       for (TimedLaunch& t : g_pending) {
         float ms = 0;
         HIPC(hipEventElapsedTime(&ms, t.a, t.b));
@@ -544,7 +549,7 @@ struct Evaluator {
     *ASPL = lsum / (((double)n - 1) * n);
     lsum /= 2.0;
     *sum = lsum;
-    if (g_stats_on) { g_stats.evals++; g_stats.eval_ms += 1e3 * (wall_time() - t0); }
+    if (timed()) { g_stats.evals++; g_stats.eval_ms += 1e3 * (wall_time() - t0); }
   }
 };
 
@@ -1098,3 +1103,7 @@ extern "C" void ODPX_Batch_release(void)
   B.ng = 0; B.small.clear(); B.large.clear();
   B.single.release();
 }
+
+// ---------------------------------------------------------------------------
+// serving engine
+#include "serve.hpp"

@@ -1,0 +1,95 @@
+"""Serving engine (ODPX_Serve_*): every result is the same alone and in the mix,
+and equals the oracle on the graph the request saw."""
+import numpy as np
+import pytest
+
+import odp_amd as odp
+import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+def same(a, b):
+    return a[0] == b[0] and a[1] == b[1] and np.float64(a[2]).tobytes() == np.float64(b[2]).tobytes()
+
+
+def chain_plan(seed):
+    """(initial adjacency, [swap per request], [graph after each request]) per chain."""
+    specs = [(512, 4), (2000, 8), (4096, 8), (6000, 12), (20000, 4), (18000, 6)]   # the last two are large
+    rng = odp.Rng(seed)
+    chains = []
+    for i, (n, d) in enumerate(specs):
+        adj = odp.generate_regular(n, d, seed * 100 + i)
+        cur = adj.copy()
+        swaps, states = [], []
+        for _ in range(3):
+            swaps.append(odp.two_opt(cur, rng))
+            states.append(cur.copy())
+        chains.append((adj, swaps, states))
+    return chains
+
+
+def run(chains, mixed):
+    srv = odp.Server(small_workers=2, large_workers=2, hbm_budget_gb=16)
+    try:
+        ids = [srv.add_chain(c[0]) for c in chains]
+        results = {i: [] for i in range(len(chains))}
+        if mixed:
+            order = []
+            for k in range(3):
+                for i, (_, swaps, _) in enumerate(chains):
+                    order.append((i, srv.submit(ids[i], [swaps[k]])))
+            srv.drain()
+            got = {r["id"]: r for r in srv.poll()}
+            for i, rid in order:
+                results[i].append(got[rid])
+        else:
+            for i, (_, swaps, _) in enumerate(chains):
+                for k in range(3):
+                    results[i].append(srv.wait(srv.submit(ids[i], [swaps[k]])))
+        return results
+    finally:
+        srv.stop()
+
+
+def test_alone_equals_mixed_and_oracle():
+    chains = chain_plan(3)
+    alone = run(chains, mixed=False)
+    mixed = run(chains, mixed=True)
+    for i, (_, _, states) in enumerate(chains):
+        for k in range(3):
+            a, m = alone[i][k], mixed[i][k]
+            assert a["status"] == 0 and m["status"] == 0
+            ra = (a["diameter"], a["sum"], a["ASPL"])
+            rm = (m["diameter"], m["sum"], m["ASPL"])
+            assert same(ra, rm), (i, k, ra, rm)
+            want = oracle.evaluate(states[k], nodes=states[k].shape[0])
+            assert same(ra, want), (i, k, ra, want)
+            assert m["t_submit"] <= m["t_start"] <= m["t_done"]
+
+
+def test_budget_refuses_chain():
+    srv = odp.Server(small_workers=1, large_workers=1, hbm_budget_gb=0.001)
+    try:
+        with pytest.raises(MemoryError):
+            srv.add_chain(odp.generate_regular(4096, 8, 1))
+    finally:
+        srv.stop()
+
+
+def test_missing_edge_is_reported():
+    srv = odp.Server(small_workers=1, large_workers=0, hbm_budget_gb=4)
+    try:
+        adj = odp.generate_regular(300, 4, 2)
+        cid = srv.add_chain(adj)
+        a = 0
+        b = int(adj[0][0])
+        c = next(v for v in range(1, 300) if v != b and v not in adj[0] and v not in adj[b])
+        r = srv.wait(srv.submit(cid, [(a, b, c, c)]))     # c-c is not an edge
+        assert r["status"] == -1
+        # the refused swap changed nothing: an empty request sees the original graph
+        r = srv.wait(srv.submit(cid, []))
+        assert r["status"] == 0
+        assert same((r["diameter"], r["sum"], r["ASPL"]), oracle.evaluate(adj, nodes=300))
+    finally:
+        srv.stop()
