@@ -69,7 +69,7 @@ def test_alone_equals_mixed_and_oracle():
 
 
 def test_budget_refuses_chain():
-    srv = odp.Server(small_workers=1, large_workers=1, hbm_budget_gb=0.001)
+    srv = odp.Server(small_workers=1, large_workers=1, hbm_budget_gb=0.0001)   # 100 KB < 196 KB needed
     try:
         with pytest.raises(MemoryError):
             srv.add_chain(odp.generate_regular(4096, 8, 1))
