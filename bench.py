@@ -7,10 +7,12 @@ matrix, 512 MiB per buffer, every column block resident).  Also: grid256 (the
 and ASPL) of the adjacency already resident in HBM.  The JSON line's value is
 evaluations per second.
 
-roofline: the dominant kernel (the gather level, k_level) timed with HIP
-events recorded on the library's launch stream around every launch in the
-timed region; achieved = algorithmic bytes of those launches (rows * ((d+2) *
-k * 8 + 4 d) per level, k = 64-bit words per row) / their summed duration.
+The timed region runs with kernel timing off (each evaluation replays its
+captured hipGraph).  roofline: the dominant kernel (the gather level, k_level)
+timed in a second pass with HIP events recorded on the library's launch stream
+around every launch; achieved = algorithmic bytes of those launches (rows *
+((d+2) * k * 8 + 4 d) per level, k = 64-bit words per row) / their summed
+duration.
 traffic: measured HBM bytes per launch, from rocprofv3 counter runs (see
 profiles/), or null.
 
@@ -141,25 +143,31 @@ def served_extra(nchains=64, saturate_s=4.0, poisson_s=6.0, load=0.6, seed=77):
                 samples.append((mirrors[i].copy(), rid))
             return rid
 
+        large = [mirrors[i].shape[0] > 16384 for i in range(nchains)]
+        n_large = sum(large)
         # closed loop: every chain keeps one request in flight
         results = {}
         for i in range(nchains):
             submit(i)
         t = time.perf_counter()
-        done = 0
+        done, done_large = 0, 0
         while time.perf_counter() - t < saturate_s:
             for r in srv.poll():
                 results[r["id"]] = r
                 done += 1
+                done_large += large[r["chain"]]
                 submit(r["chain"])
         srv.drain()
         for r in srv.poll():
             results[r["id"]] = r
         sat_rate = done / saturate_s
-        # open loop: Poisson arrivals at `load` of the saturated rate, chains uniform
-        rate = max(1.0, load * sat_rate)
+        x_large, x_small = done_large / saturate_s, (done - done_large) / saturate_s
+        # open loop: Poisson arrivals, chains uniform; the mix saturates when its slowest
+        # class does, so the rate is `load` of min over classes of capacity / class share
+        cap_mix = min(x_large * nchains / max(1, n_large), x_small * nchains / max(1, nchains - n_large))
+        rate = max(1.0, load * cap_mix)
         arr = np.random.default_rng(seed)
-        lat, starts = [], {}
+        lat, lat_large, starts = [], [], {}
         t = time.perf_counter()
         next_t = arr.exponential(1.0 / rate)
         submitted = 0
@@ -175,7 +183,7 @@ def served_extra(nchains=64, saturate_s=4.0, poisson_s=6.0, load=0.6, seed=77):
             for r in srv.poll():
                 results[r["id"]] = r
                 if r["id"] in starts:
-                    lat.append(r["t_done"] - r["t_submit"])
+                    (lat_large if large[r["chain"]] else lat).append(r["t_done"] - r["t_submit"])
             if next_t - (time.perf_counter() - t) > 2e-4:
                 time.sleep(1e-4)
         srv.drain()
@@ -183,17 +191,23 @@ def served_extra(nchains=64, saturate_s=4.0, poisson_s=6.0, load=0.6, seed=77):
         for r in srv.poll():
             results[r["id"]] = r
             if r["id"] in starts:
-                lat.append(r["t_done"] - r["t_submit"])
+                (lat_large if large[r["chain"]] else lat).append(r["t_done"] - r["t_submit"])
         bad = sum(1 for r in results.values() if r["status"] != 0)
-        lat_ms = np.array(lat) * 1e3
+        lat_ms = np.array(lat + lat_large) * 1e3
+        lat_s_ms = np.array(lat) * 1e3
+        lat_l_ms = np.array(lat_large) * 1e3
         sample_checks = [(g, (results[rid]["diameter"], results[rid]["sum"], results[rid]["ASPL"]))
                          for g, rid in samples]
-        return ({"chains": nchains, "saturated_evals_per_s": sat_rate,
-                 "poisson_rate_per_s": rate, "poisson_evals_per_s": len(lat) / elapsed,
+        return ({"chains": nchains, "large_chains": n_large, "saturated_evals_per_s": sat_rate,
+                 "saturated_large_evals_per_s": x_large, "saturated_small_evals_per_s": x_small,
+                 "poisson_rate_per_s": rate, "poisson_evals_per_s": len(lat_ms) / elapsed,
                  "p50_ms": float(np.percentile(lat_ms, 50)), "p99_ms": float(np.percentile(lat_ms, 99)),
+                 "p50_ms_small": float(np.percentile(lat_s_ms, 50)), "p99_ms_small": float(np.percentile(lat_s_ms, 99)),
+                 "p50_ms_large": float(np.percentile(lat_l_ms, 50)), "p99_ms_large": float(np.percentile(lat_l_ms, 99)),
                  "requests": len(results), "failed": bad,
                  "workload": "64 chains, n in {4096,16384,65536} x d in {8,16,32}, one accepted 2-opt swap per "
-                             "request; closed loop for the saturated rate, then Poisson arrivals at 60% of it"},
+                             "request; closed loop for the saturated rates, then Poisson arrivals over uniform "
+                             "chains at 60% of the mix's capacity"},
                 sample_checks)
     finally:
         srv.stop()
@@ -296,13 +310,17 @@ def main():
 
     for _ in range(args.warmup):
         odp.set_aspl_resident()
-    odp.stats_enable(True)
-    odp.stats_reset()
+    # timed region: kernel timing off, so each evaluation replays its captured hipGraph
     t = time.perf_counter()
     for i in range(args.steps):
         r = odp.set_aspl_resident()
         assert r == first
     el = time.perf_counter() - t
+    # kernel timing pass: HIP events around every launch (eager launches, same kernels)
+    odp.stats_enable(True)
+    odp.stats_reset()
+    for i in range(min(args.steps, 10)):
+        assert odp.set_aspl_resident() == first
     st = odp.stats()
     odp.stats_enable(False)
     log(f"timed {args.steps} evaluations in {el:.3f}s; kernel stats {st}")

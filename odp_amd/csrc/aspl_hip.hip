@@ -165,10 +165,42 @@ struct Evaluator {
   hipStream_t st = nullptr;             // launch stream (the library's main stream unless set)
   bool timed() const { return g_stats_on && st == C.stream; }
 
+  // hipGraph of one evaluation's launch sequence: replayed while the plan, the
+  // buffers, the adjacency pointer and (level plan) the first batch are unchanged
+  hipGraphExec_t gexec = nullptr;
+  int g_key = -1;
+  const int* g_adj = nullptr;
+  void drop_graph()
+  {
+    if (gexec) HIPC(hipGraphExecDestroy(gexec));
+    gexec = nullptr;
+    g_key = -1;
+  }
+  bool graph_on() const
+  {
+    static int env = -1;
+    if (env < 0) { const char* v = getenv("ODP_HIP_GRAPH"); env = v ? atoi(v) != 0 : 1; }
+    return env && !timed();
+  }
+  const int* cur_adj() const { return ext_adj ? ext_adj : d_adj; }
+  template <typename F> void capture(F enqueue, int key)
+  {
+    drop_graph();
+    hipGraph_t g;
+    HIPC(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    enqueue();
+    HIPC(hipStreamEndCapture(st, &g));
+    HIPC(hipGraphInstantiate(&gexec, g, nullptr, nullptr, 0));
+    HIPC(hipGraphDestroy(g));
+    g_key = key;
+    g_adj = cur_adj();
+  }
+
   size_t adj_elems() const { return (size_t)(n / s) * degree; }
 
   void release()
   {
+    drop_graph();
     dfree(&d_adj); dfree(&d_numdeg); dfree(&d_itable); dfree(&d_nbr); dfree(&d_err); dfree(&d_nbr16);
     nbr16_cap = 0;
     hfree(&h_stage); hfree(&h_err);
@@ -184,6 +216,7 @@ struct Evaluator {
   {
     ctx_init();
     if (!st) st = C.stream;
+    drop_graph();
     family = fam; n = nodes; degree = deg; s = sym; width = w; height = h; grid_s = gs;
     nsrc = n / s;
     dpad = round_up(deg > 0 ? deg : 1, 4);
@@ -362,6 +395,7 @@ struct Evaluator {
 
   void apply_plan(const ODPX_Plan& p)
   {
+    drop_graph();
     plan = p;
     if (plan.plan == ODPX_PLAN_LEVEL) setup_level();
     else setup_resident();
@@ -406,7 +440,8 @@ struct Evaluator {
   {
     LevelRun& r = lr;
     trace("run_level");
-    prepare(r.cnt, r.ntasks, r.done, r.ntasks);   // ring slot 0 and the done flags
+    const bool graphed = graph_on() && r.group == r.ntasks;
+    if (!graphed) prepare(r.cnt, r.ntasks, r.done, r.ntasks);   // ring slot 0 and the done flags
     struct St { int running; u64 sum, prev; int lwork; };
     std::vector<St> sts(r.ntasks, St{1, 0, 0, 0});
     bool any_disc = false;
@@ -415,19 +450,16 @@ struct Evaluator {
       const int cnt_t = std::min(r.group, r.ntasks - first);
       const LevelTask* tasks = r.tasks + first;
       const int grid = cnt_t * r.per_task_wg;
-      {
+      auto level0 = [&]() {
         TimedLaunch tl;
         timed_begin(tl, T_OTHER, 0, st);
         hipLaunchKernelGGL(k_level0, dim3(grid), dim3(LEVEL_WG), 0, st,
                            tasks, r.wg_start, cnt_t, RING, r.upw, r.xcd);
         HIPC(hipGetLastError());
         timed_end(tl);
-      }
-      int launched = 1, examined = 0, running = cnt_t;
-      int guess = guess_levels > 0 ? guess_levels : 8;
-      int batch_end = std::min(n, std::max(2, guess));
-      for (;;) {
-        for (int L = launched; L < batch_end; L++) {
+      };
+      auto levels = [&](int from, int to) {
+        for (int L = from; L < to; L++) {
           TimedLaunch tl;
           timed_begin(tl, T_LEVEL, L, st);
           if (r.urow)
@@ -439,9 +471,26 @@ struct Evaluator {
           HIPC(hipGetLastError());
           timed_end(tl);
         }
-        launched = std::max(launched, batch_end);
         HIPC(hipMemcpyAsync(r.h_cnt, r.cnt, sizeof(u64) * RING * r.ntasks, hipMemcpyDeviceToHost, st));
         HIPC(hipMemcpyAsync(h_err, d_err, sizeof(int), hipMemcpyDeviceToHost, st));
+      };
+      int launched = 1, examined = 0, running = cnt_t;
+      int guess = guess_levels > 0 ? guess_levels : 8;
+      int batch_end = std::min(n, std::max(2, guess));
+      bool first_batch = true;
+      for (;;) {
+        if (first_batch && graphed) {
+          // prepare + level 0 + the first batch + the counter copies as one graph
+          const int be = batch_end;
+          if (!gexec || g_key != be || g_adj != cur_adj())
+            capture([&]() { prepare(r.cnt, r.ntasks, r.done, r.ntasks); level0(); levels(1, be); }, be);
+          HIPC(hipGraphLaunch(gexec, st));
+        } else {
+          if (first_batch) level0();
+          levels(launched, batch_end);
+        }
+        first_batch = false;
+        launched = std::max(launched, batch_end);
         HIPC(hipStreamSynchronize(st));
         trace("level batch synchronized");
         check_err();
@@ -502,15 +551,23 @@ struct Evaluator {
   {
     ResRun& r = rr;
     trace("run_resident");
-    prepare((u64*)r.out, (int)(sizeof(ResOut) / 8), (u32*)r.qhead, 1);
-    TimedLaunch tl;
-    timed_begin(tl, T_RESIDENT, 0, st);
-    hipLaunchKernelGGL(k_resident, dim3(r.grid), dim3(r.threads), r.lds, st, r.tasks, r.ntasks, r.qhead, r.out);
-    HIPC(hipGetLastError());
-    timed_end(tl);
+    auto enqueue = [&]() {
+      prepare((u64*)r.out, (int)(sizeof(ResOut) / 8), (u32*)r.qhead, 1);
+      TimedLaunch tl;
+      timed_begin(tl, T_RESIDENT, 0, st);
+      hipLaunchKernelGGL(k_resident, dim3(r.grid), dim3(r.threads), r.lds, st, r.tasks, r.ntasks, r.qhead, r.out);
+      HIPC(hipGetLastError());
+      timed_end(tl);
+      HIPC(hipMemcpyAsync(r.h_out, r.out, sizeof(ResOut), hipMemcpyDeviceToHost, st));
+      HIPC(hipMemcpyAsync(h_err, d_err, sizeof(int), hipMemcpyDeviceToHost, st));
+    };
+    if (graph_on()) {
+      if (!gexec || g_key != 0 || g_adj != cur_adj()) capture(enqueue, 0);
+      HIPC(hipGraphLaunch(gexec, st));
+    } else {
+      enqueue();
+    }
     trace("resident launched");
-    HIPC(hipMemcpyAsync(r.h_out, r.out, sizeof(ResOut), hipMemcpyDeviceToHost, st));
-    HIPC(hipMemcpyAsync(h_err, d_err, sizeof(int), hipMemcpyDeviceToHost, st));
     HIPC(hipStreamSynchronize(st));
     trace("resident synchronized");
     check_err();

@@ -299,3 +299,31 @@ def test_invalid_vertex_id_exits():
         "o.set_aspl(a)\n" % os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
     r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300)
     assert r.returncode == 1 and "outside [0, nodes)" in r.stderr
+
+
+@pytest.mark.parametrize("plan", [None, PLANS[7], PLANS[1]])
+def test_graph_replay_matches_eager(plan):
+    """Evaluations replay a captured hipGraph unless kernel timing is on (eager
+    launches); several replays with eager swap kernels in between must equal the
+    eager evaluation of the same graph every time."""
+    n, d = 3000, 6
+    adj = odp.generate_regular(n, d, 31)
+    rng = odp.Rng(9)
+    odp.init_aspl_general(n, d)
+    if plan:
+        odp.set_plan(**plan)
+    host = adj.copy()
+    odp.upload_adjacency(host)
+    try:
+        for _ in range(4):
+            a, b, c, dd = odp.two_opt(host, rng)
+            assert odp.swap_device(a, b, c, dd) == 0
+            replay = [odp.set_aspl_resident() for _ in range(3)]
+            odp.stats_enable(True)
+            eager = odp.set_aspl_resident()
+            odp.stats_enable(False)
+            want = oracle.evaluate(host, nodes=n)
+            assert all(same(r, eager) for r in replay) and same(eager, want)
+    finally:
+        odp.stats_enable(False)
+        odp.finalize_aspl()
