@@ -176,11 +176,15 @@ struct Evaluator {
     gexec = nullptr;
     g_key = -1;
   }
+  // Graphs are captured and replayed on the library's main stream only; the
+  // serving workers launch eagerly: a run in which two worker threads captured
+  // graphs concurrently ended in a host segmentation fault whose cause was not
+  // found, so that path is off.
   bool graph_on() const
   {
     static int env = -1;
     if (env < 0) { const char* v = getenv("ODP_HIP_GRAPH"); env = v ? atoi(v) != 0 : 1; }
-    return env && !timed();
+    return env && !timed() && st == C.stream;
   }
   const int* cur_adj() const { return ext_adj ? ext_adj : d_adj; }
   template <typename F> void capture(F enqueue, int key)
