@@ -8,8 +8,8 @@ and ASPL) of the adjacency already resident in HBM.  The JSON line's value is
 evaluations per second.
 
 The timed region runs with kernel timing off (each evaluation replays its
-captured hipGraph).  roofline: the dominant kernel (the gather level, k_level)
-timed in a second pass with HIP events recorded on the library's launch stream
+captured hipGraph).  roofline: the dominant kernel (the gather level, k_level;
+k_resident when the graph's blocks fit LDS) timed in a second pass with HIP events recorded on the library's launch stream
 around every launch; achieved = algorithmic bytes of those launches (rows *
 ((d+2) * k * 8 + 4 d) per level, k = 64-bit words per row) / their summed
 duration.
@@ -256,8 +256,8 @@ def main():
     ap.add_argument("--workload", default="random65536", choices=sorted(WORKLOADS))
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--plan", default="auto",
-                    help="auto (the library's choice) | matrix | matrix-rows | level:<kw>[:upw:seg_major:xcd:group] "
-                         "| resident:<wb>:<threads>")
+                    help="auto (the library's choice) | matrix | matrix-rows | "
+                         "level:<kw>[:upw:seg_major:xcd:group:direct] | resident:<wb>:<threads>")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--served", action="store_true",
@@ -293,15 +293,15 @@ def main():
     log(f"{args.workload}: graph generated in {time.perf_counter() - t0:.2f}s")
     if args.plan == "matrix":
         # every column block resident (the full bit matrix), 16-word slabs, XCD-owned
-        odp.set_plan(plan=odp.PLAN_LEVEL, kw=16, upw=8, seg_major=0, xcd=1, group=0)
+        odp.set_plan(plan=odp.PLAN_LEVEL, kw=16, upw=8, seg_major=0, xcd=1, group=0, direct=-1)
     elif args.plan == "matrix-rows":
-        # the full bit matrix with whole 8 KiB rows (the reference's layout)
-        odp.set_plan(plan=odp.PLAN_LEVEL, kw=0, upw=4, seg_major=1, xcd=0, group=0)
+        # the full bit matrix with whole 8 KiB rows (the reference's layout), every level gathered
+        odp.set_plan(plan=odp.PLAN_LEVEL, kw=0, upw=4, seg_major=1, xcd=0, group=0, direct=0)
     elif args.plan.startswith("level:"):
         f = args.plan.split(":")
         odp.set_plan(plan=odp.PLAN_LEVEL, kw=int(f[1]), upw=int(f[2]) if len(f) > 2 else 8,
                      seg_major=int(f[3]) if len(f) > 3 else 0, xcd=int(f[4]) if len(f) > 4 else 1,
-                     group=int(f[5]) if len(f) > 5 else 0)
+                     group=int(f[5]) if len(f) > 5 else 0, direct=int(f[6]) if len(f) > 6 else -1)
     elif args.plan.startswith("resident:"):
         f = args.plan.split(":")
         odp.set_plan(plan=odp.PLAN_RESIDENT, wb=int(f[1]), threads=int(f[2]))
@@ -336,8 +336,12 @@ def main():
     pcie_rate = 3 / (time.perf_counter() - t)
     odp.finalize_aspl()
 
-    level_ms = st["level_ms"] / max(1, st["level_launches"])
-    achieved = (st["level_bytes"] / max(1, st["level_launches"])) / (level_ms * 1e-3) / 1e9 if level_ms > 0 else 0.0
+    # the dominant kernel: the gather level, or the LDS-resident kernel when the plan is resident
+    kern = "k_level" if st["level_launches"] > 0 else "k_resident"
+    pre = "level" if kern == "k_level" else "resident"
+    nl = max(1, st[pre + "_launches"])
+    level_ms = st[pre + "_ms"] / nl
+    achieved = (st[pre + "_bytes"] / nl) / (level_ms * 1e-3) / 1e9 if level_ms > 0 else 0.0
     traffic = None
     if os.path.exists(args.traffic):
         tr = json.load(open(args.traffic)).get(args.workload)
@@ -345,9 +349,9 @@ def main():
             traffic = tr["bytes_per_launch"]
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": "k_level", "launch_ms": round(level_ms, 4),
-                "alg_bytes_per_launch": st["level_bytes"] / max(1, st["level_launches"]),
-                "launches": st["level_launches"]}
+                "kernel": kern, "launch_ms": round(level_ms, 4),
+                "alg_bytes_per_launch": st[pre + "_bytes"] / nl,
+                "launches": st[pre + "_launches"]}
 
     out = {"metric": METRIC, "value": round(args.steps / el, 3), "unit": "evals/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * el / args.steps, 4),

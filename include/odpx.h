@@ -49,6 +49,8 @@ typedef struct {
   int seg_major;  /* level: 1 = all rows of one column segment before the next        */
   int xcd;        /* level: 1 = each XCD takes one contiguous range of workgroups     */
   int group;      /* level: column blocks resident at once; 0 = all (MATRIX)          */
+  int direct;     /* level: levels 0..K-1 by direct walk enumeration in one launch:
+                     0 = off, -1 = the library picks K, K >= 2 = that K (at most 6)     */
 } ODPX_Plan;
 
 /* Plan of the current evaluator (after Init).  Set: 0 on success, -1 if invalid.

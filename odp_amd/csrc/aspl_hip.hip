@@ -34,6 +34,8 @@ static const int GPU_CHUNK_WORDS = 64;          // column-block width of MATRIX_
 static const int SAVING_GROUP = 32;             // MATRIX_SAVING blocks resident at once (4 per XCD)
 static const int LEVEL_BLOCK_WORDS = 16;        // default column-block width of the level plan
 static const int RES_WB_MAX = 8;                // widest resident block: 256 sources
+static const long DIRECT_TREE_MAX = 8192;       // walk-tree nodes per row the automatic direct K allows
+static const long DIRECT_TREE_CAP = 65536;      // ... and any requested K (larger K is lowered)
 static const double MEM_THRESHOLD_MB = 2048.0;  // MATRIX -> MATRIX_SAVING switch (parameter.h:4)
 static const size_t LDS_LIMIT = 163840;         // LDS a gfx950 workgroup may allocate
 static const size_t LDS_STATIC = 512;           // k_resident's own __shared__ data, rounded up
@@ -75,6 +77,7 @@ static void ctx_init()
   C.ncu = prop.multiProcessorCount;
   HIPC(hipStreamCreateWithFlags(&C.stream, hipStreamNonBlocking));
   HIPC(hipFuncSetAttribute((const void*)k_resident, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_DYN_MAX));
+  HIPC(hipFuncSetAttribute((const void*)k_direct, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_DYN_MAX));
   C.ready = true;
   trace("ctx ready");
 }
@@ -130,6 +133,8 @@ static void timed_end(TimedLaunch& t)
 
 struct LevelRun {
   int n = 0, ntasks = 0, group = 0, kw = 0, kwp = 0, upw = 4, seg_major = 1, xcd = 0, urow = 0, per_task_wg = 0;
+  int direct = 0, dnw = 4, dgrid = 0;   // levels done by k_direct (0: k_level0), its waves and grid
+  size_t dlds = 0;
   size_t buf_words = 0;                 // capacity of bufs (64-bit words)
   size_t task_cap = 0;
   u64* bufs = nullptr; u64* cnt = nullptr; u32* done = nullptr;
@@ -170,11 +175,15 @@ struct Evaluator {
   hipGraphExec_t gexec = nullptr;
   int g_key = -1;
   const int* g_adj = nullptr;
+  int g_seen = -1;                      // key of the last eager evaluation, for the repeat test
+  const int* g_seen_adj = nullptr;
   void drop_graph()
   {
     if (gexec) HIPC(hipGraphExecDestroy(gexec));
     gexec = nullptr;
     g_key = -1;
+    g_seen = -1;
+    g_seen_adj = nullptr;
   }
   // Graphs are captured and replayed on the library's main stream only; the
   // serving workers launch eagerly: a run in which two worker threads captured
@@ -187,6 +196,18 @@ struct Evaluator {
     return env && !timed() && st == C.stream;
   }
   const int* cur_adj() const { return ext_adj ? ext_adj : d_adj; }
+  bool graph_ready(int key) const { return gexec && g_key == key && g_adj == cur_adj(); }
+  // Replay a graph only for a launch sequence that repeats: the first evaluation
+  // of a key runs eagerly, the second captures, later ones replay.  One-off
+  // evaluations (a new shape or plan every call) never pay for a capture.
+  bool use_graph(int key)
+  {
+    if (!graph_on()) return false;
+    if (graph_ready(key) || (g_seen == key && g_seen_adj == cur_adj())) return true;
+    g_seen = key;
+    g_seen_adj = cur_adj();
+    return false;
+  }
   template <typename F> void capture(F enqueue, int key)
   {
     drop_graph();
@@ -303,6 +324,7 @@ struct Evaluator {
     p.seg_major = 0;
     p.xcd = 1;
     p.group = (kind == KIND_MATRIX) ? 0 : SAVING_GROUP;
+    p.direct = -1;
     return p;
   }
 
@@ -313,8 +335,40 @@ struct Evaluator {
       if (p.threads != 256 && p.threads != 512 && p.threads != 1024) return false;
       return n <= 65535 && d_nbr16 && (size_t)n * p.wb * 4 <= LDS_DYN_MAX && n <= res_rpt(p.wb) * p.threads;
     }
-    if (p.plan == ODPX_PLAN_LEVEL) return p.kw >= 0 && p.upw >= 1 && p.upw <= 64 && p.group >= 0;
+    if (p.plan == ODPX_PLAN_LEVEL) {
+      if (!(p.kw >= 0 && p.upw >= 1 && p.upw <= 64 && p.group >= 0)) return false;
+      // direct K that does not apply to this graph's blocks falls back to k_level0
+      return p.direct == 0 || p.direct == -1 || (p.direct >= 2 && p.direct <= DIRECT_KMAX);
+    }
     return false;
+  }
+
+  // ---- direct enumeration of the first levels (k_direct) ----
+  long direct_tree(int K) const
+  {
+    long t = 0, m = 1;
+    for (int i = 0; i <= K; i++) { t += m; m *= dpad; if (m > (1L << 40)) m = 1L << 40; }
+    return t;
+  }
+  size_t direct_lds(int K, int kw, int group, int nw) const
+  {
+    return (size_t)nw * group * kw * 8 + (size_t)K * group * 8;
+  }
+  // waves per workgroup k_direct can run with (0: not applicable): the block
+  // rows must be a power of two of 2..128 words and one row bitmap must fit LDS
+  int direct_waves(int K, int kw, int group) const
+  {
+    if (K < 2 || K > DIRECT_KMAX || kw < 2 || kw > 128 || (kw & (kw - 1))) return 0;
+    for (int nw = 4; nw >= 1; nw >>= 1)
+      if (direct_lds(K, kw, group, nw) <= LDS_DYN_MAX) return nw;
+    return 0;
+  }
+  int direct_auto(int kw, int group) const
+  {
+    int best = 0;
+    for (int K = 2; K <= DIRECT_KMAX && K < n; K++)
+      if (direct_tree(K) <= DIRECT_TREE_MAX && direct_waves(K, kw, group) > 0) best = K;
+    return best;
   }
 
   // ---- level plan: buffers and task table ----
@@ -347,6 +401,18 @@ struct Evaluator {
     r.n = n; r.ntasks = ntasks; r.group = group; r.kw = kw; r.kwp = kwp;
     r.upw = plan.upw; r.seg_major = plan.seg_major; r.xcd = plan.xcd; r.urow = (lpr_log2 == 6);
     r.per_task_wg = (units + 4 * r.upw - 1) / (4 * r.upw);
+    int K = plan.direct;
+    while (K >= 2 && direct_tree(K) > DIRECT_TREE_CAP) K--;
+    r.direct = plan.direct == -1 ? direct_auto(kw, group) : (K >= 2 && direct_waves(K, kw, group) ? K : 0);
+    if (r.direct >= n) r.direct = 0;
+    plan.direct = r.direct;            // the plan reports the K that runs
+    if (r.direct) {
+      r.dnw = direct_waves(r.direct, kw, group);
+      r.dlds = direct_lds(r.direct, kw, group, r.dnw);
+      int per_cu = (int)std::max<size_t>(1, LDS_LIMIT / r.dlds);
+      per_cu = std::min(per_cu, 32 / r.dnw);
+      r.dgrid = std::max(1, std::min((n + r.dnw - 1) / r.dnw, per_cu * C.ncu));
+    }
     std::vector<LevelTask> ht(ntasks);
     std::vector<int> ws(ntasks + 1);
     r.l.assign(ntasks, 0);
@@ -444,8 +510,12 @@ struct Evaluator {
   {
     LevelRun& r = lr;
     trace("run_level");
-    const bool graphed = graph_on() && r.group == r.ntasks;
-    if (!graphed) prepare(r.cnt, r.ntasks, r.done, r.ntasks);   // ring slot 0 and the done flags
+    const int K = r.direct;            // levels 0..K-1 come from k_direct (0: k_level0 does level 0)
+    const int L1 = K ? K : 1;           // first level the gather kernel runs
+    const int nzero = (K ? K + 1 : 1) * r.ntasks;   // ring slots 0..L1 start at zero
+    const int be0 = std::min(n, std::max(2, guess_levels > 0 ? guess_levels : 8));   // first batch
+    const bool graphed = r.group == r.ntasks && use_graph(be0);
+    if (!graphed) prepare(r.cnt, nzero, r.done, r.ntasks);   // the first ring slots and the done flags
     struct St { int running; u64 sum, prev; int lwork; };
     std::vector<St> sts(r.ntasks, St{1, 0, 0, 0});
     bool any_disc = false;
@@ -457,8 +527,23 @@ struct Evaluator {
       auto level0 = [&]() {
         TimedLaunch tl;
         timed_begin(tl, T_OTHER, 0, st);
-        hipLaunchKernelGGL(k_level0, dim3(grid), dim3(LEVEL_WG), 0, st,
-                           tasks, r.wg_start, cnt_t, RING, r.upw, r.xcd);
+        if (K) {
+          DirectArgs a;
+          a.nbr = d_nbr;
+          a.out = r.bufs + ((K & 1) ? (size_t)n * r.kwp : 0);     // the buffer level K reads
+          a.tstride = (int64_t)2 * n * r.kwp;
+          a.cnt = r.cnt + first; a.cstride = r.ntasks;
+          a.n = n; a.dpad = dpad; a.kw = r.kw;
+          a.src_lo = first * r.kw * 64;
+          a.nsrc = std::min(cnt_t * r.kw * 64, nsrc - a.src_lo);
+          a.ntask = cnt_t; a.words = cnt_t * r.kw; a.K = K;
+          long m = 1;
+          for (int i = 0; i < 8; i++) { a.pw[i] = (int)m; m = std::min<long>(m * dpad, 1L << 30); }
+          hipLaunchKernelGGL(k_direct, dim3(r.dgrid), dim3(64 * r.dnw), r.dlds, st, a);
+        } else {
+          hipLaunchKernelGGL(k_level0, dim3(grid), dim3(LEVEL_WG), 0, st,
+                             tasks, r.wg_start, cnt_t, RING, r.upw, r.xcd);
+        }
         HIPC(hipGetLastError());
         timed_end(tl);
       };
@@ -478,16 +563,15 @@ struct Evaluator {
         HIPC(hipMemcpyAsync(r.h_cnt, r.cnt, sizeof(u64) * RING * r.ntasks, hipMemcpyDeviceToHost, st));
         HIPC(hipMemcpyAsync(h_err, d_err, sizeof(int), hipMemcpyDeviceToHost, st));
       };
-      int launched = 1, examined = 0, running = cnt_t;
-      int guess = guess_levels > 0 ? guess_levels : 8;
-      int batch_end = std::min(n, std::max(2, guess));
+      int launched = L1, examined = 0, running = cnt_t;
+      int batch_end = first == 0 ? be0 : std::min(n, std::max(2, guess_levels > 0 ? guess_levels : 8));
       bool first_batch = true;
       for (;;) {
         if (first_batch && graphed) {
           // prepare + level 0 + the first batch + the counter copies as one graph
           const int be = batch_end;
-          if (!gexec || g_key != be || g_adj != cur_adj())
-            capture([&]() { prepare(r.cnt, r.ntasks, r.done, r.ntasks); level0(); levels(1, be); }, be);
+          if (!graph_ready(be))
+            capture([&]() { prepare(r.cnt, nzero, r.done, r.ntasks); level0(); levels(L1, be); }, be);
           HIPC(hipGraphLaunch(gexec, st));
         } else {
           if (first_batch) level0();
@@ -565,8 +649,8 @@ struct Evaluator {
       HIPC(hipMemcpyAsync(r.h_out, r.out, sizeof(ResOut), hipMemcpyDeviceToHost, st));
       HIPC(hipMemcpyAsync(h_err, d_err, sizeof(int), hipMemcpyDeviceToHost, st));
     };
-    if (graph_on()) {
-      if (!gexec || g_key != 0 || g_adj != cur_adj()) capture(enqueue, 0);
+    if (use_graph(0)) {
+      if (!graph_ready(0)) capture(enqueue, 0);
       HIPC(hipGraphLaunch(gexec, st));
     } else {
       enqueue();
@@ -907,7 +991,9 @@ extern "C" int ODPX_Autotune(void* adjacency, int repeats)
         ODPX_Plan p{};
         p.plan = ODPX_PLAN_LEVEL; p.kw = kw; p.upw = kw == 0 ? 4 : 8; p.seg_major = sm; p.xcd = x;
         p.group = v.plan.plan == ODPX_PLAN_LEVEL ? v.plan.group : 0;
+        p.direct = 0;
         cand.push_back(p);
+        if (v.direct_auto(kw == 0 ? W : std::min(kw, W), 1) > 0) { p.direct = -1; cand.push_back(p); }
       }
   if (repeats < 1) repeats = 1;
   v.upload(adjacency);

@@ -308,6 +308,99 @@ __global__ __launch_bounds__(LEVEL_WG) void k_level(const LevelTask* __restrict_
 }
 
 // ---------------------------------------------------------------------------
+// direct kernel: the first K levels of a group of column blocks in one launch
+//
+// Row v of level L holds the sources reachable from v by a walk of at most
+// L+1 steps through the row-neighbour table (level 0: v and its neighbours;
+// each level ORs in the neighbours' rows).  While those sets are small they
+// are cheaper to enumerate than to gather: one wave per row walks every path
+// of length <= K from v (sum of dpad^i, i <= K, table reads), sets the bits in
+// an LDS bitmap of the group's row, popcounts it per block after each depth
+// (the counts of levels 0..K-1) and writes the depth-K row to the buffer level
+// K reads.  The sets are exactly those the level kernels build, so every count
+// and every word is identical to running k_level0 and levels 1..K-1.
+
+struct DirectArgs {
+  const int* nbr;        // [n][dpad] row-neighbour table
+  u64* out;              // block t, row v, word w: out[t * tstride + v * kw + w]
+  int64_t tstride;       // words between the slabs of consecutive blocks
+  u64* cnt;              // count of block t at level L: cnt[L * cstride + t], L < K
+  int cstride;
+  int n, dpad, kw;       // kw: words per block row, a power of two in [2, 128]
+  int src_lo, nsrc;      // sources of the group: [src_lo, src_lo + nsrc)
+  int ntask, words;      // blocks in the group; words = ntask * kw
+  int K;                 // levels computed (2..DIRECT_KMAX)
+  int pw[8];             // dpad^i
+};
+
+#define DIRECT_KMAX 6
+
+__device__ __forceinline__ void direct_set(u32* bm, int id, int src_lo, int nsrc)
+{
+  const int c = id - src_lo;
+  if ((unsigned)c < (unsigned)nsrc) atomicOr(&bm[c >> 5], 1u << (c & 31));
+}
+
+__global__ __launch_bounds__(256) void k_direct(DirectArgs a)
+{
+  extern __shared__ __attribute__((aligned(16))) u64 dlds[];
+  const int nw = blockDim.x >> 6;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  u64* bm = dlds + (int64_t)wave * a.words;
+  u32* bm32 = (u32*)bm;
+  u64* lcnt = dlds + (int64_t)nw * a.words;        // [K][ntask] counts of this workgroup
+  for (int i = threadIdx.x; i < a.K * a.ntask; i += blockDim.x) lcnt[i] = 0;
+  const int half = a.kw >> 1;                      // lanes per block row (two words a lane)
+  for (int base = blockIdx.x * nw; base < a.n; base += gridDim.x * nw) {   // uniform per workgroup
+    const int v = base + wave;
+    const bool act = v < a.n;
+    for (int w = 2 * lane; w < a.words; w += 128) { u64x2 z; z.x = 0; z.y = 0; *(u64x2*)(bm + w) = z; }
+    __syncthreads();
+    if (act && lane == 0) direct_set(bm32, v, a.src_lo, a.nsrc);
+    for (int d = 1; d <= a.K; d++) {
+      // depth d: the neighbours of every node at depth d-1 (path p, base-dpad digits)
+      if (act) {
+        const int m = a.pw[d - 1];
+        for (int p = lane; p < m; p += 64) {
+          int x = v;
+          for (int k = d - 2; k >= 0; k--) {
+            const int digit = (p / a.pw[k]) % a.dpad;
+            x = a.nbr[(int64_t)x * a.dpad + digit];
+          }
+          const int* row = a.nbr + (int64_t)x * a.dpad;
+          for (int j = 0; j < a.dpad; j += 4) {
+            const int4 q = *(const int4*)(row + j);
+            direct_set(bm32, q.x, a.src_lo, a.nsrc);
+            direct_set(bm32, q.y, a.src_lo, a.nsrc);
+            direct_set(bm32, q.z, a.src_lo, a.nsrc);
+            direct_set(bm32, q.w, a.src_lo, a.nsrc);
+          }
+        }
+      }
+      __syncthreads();
+      // level d-1: per-block popcounts; after the last depth, the row itself
+      if (act) {
+        for (int w0 = 0; w0 < a.words; w0 += 128) {      // wave-uniform trip count
+          const int w = w0 + 2 * lane;
+          u64x2 x; x.x = 0; x.y = 0;
+          if (w < a.words) x = *(const u64x2*)(bm + w);
+          u64 c = (u64)(__popcll(x.x) + __popcll(x.y));
+          for (int o = 1; o < half; o <<= 1) c += __shfl_xor(c, o, 64);
+          const int t = w / a.kw;
+          if (w < a.words && (lane & (half - 1)) == 0 && c) atomicAdd(&lcnt[(d - 1) * a.ntask + t], c);
+          if (d == a.K && w < a.words)
+            *(u64x2*)(a.out + (int64_t)t * a.tstride + (int64_t)v * a.kw + (w & (a.kw - 1))) = x;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  for (int i = threadIdx.x; i < a.K * a.ntask; i += blockDim.x)
+    if (lcnt[i]) atomicAdd(&a.cnt[(int64_t)(i / a.ntask) * a.cstride + (i % a.ntask)], lcnt[i]);
+}
+
+// ---------------------------------------------------------------------------
 // resident kernel
 //
 // One workgroup runs every level of one block of sources with the block's

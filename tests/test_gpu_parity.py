@@ -34,6 +34,10 @@ PLANS = [
     dict(plan=LEV, kw=2, upw=2, seg_major=0, xcd=1, group=2),
     dict(plan=LEV, kw=1, upw=16, seg_major=1, xcd=0, group=1),
     dict(plan=RES, wb=4, threads=1024),
+    dict(plan=LEV, kw=16, upw=8, seg_major=0, xcd=1, direct=-1),
+    dict(plan=LEV, kw=2, upw=2, seg_major=1, xcd=1, direct=2),
+    dict(plan=LEV, kw=4, upw=4, seg_major=0, xcd=1, group=2, direct=3),
+    dict(plan=LEV, kw=128, upw=8, seg_major=0, xcd=1, direct=6),
 ]
 
 
@@ -327,3 +331,59 @@ def test_graph_replay_matches_eager(plan):
     finally:
         odp.stats_enable(False)
         odp.finalize_aspl()
+
+
+def direct_eval(adj, K, kw, group=0, **init):
+    """Evaluate with the first K levels enumerated by k_direct; returns (result, K that ran)."""
+    nodes = init.pop("nodes", None)
+    width, height = init.pop("width", None), init.pop("height", None)
+    s = init.pop("symmetries", 1)
+    nd = init.pop("num_degrees", None)
+    if width is None:
+        (odp.init_aspl_general(nodes, adj.shape[1], nd) if s == 1
+         else odp.init_aspl_general_s(nodes, adj.shape[1], nd, s))
+    else:
+        (odp.init_aspl_grid(width, height, adj.shape[1], nd) if s == 1
+         else odp.init_aspl_grid_s(width, height, adj.shape[1], nd, s))
+    try:
+        odp.set_plan(plan=LEV, kw=kw, upw=4, seg_major=0, xcd=1, group=group, direct=K)
+        ran = odp.get_plan()["direct"]
+        return odp.set_aspl(adj), ran
+    finally:
+        odp.finalize_aspl()
+
+
+@pytest.mark.parametrize("K,kw,group", [(2, 2, 0), (3, 16, 0), (4, 4, 3), (5, 8, 1), (6, 2, 0), (3, 32, 2),
+                                        (2, 128, 0)])
+def test_direct_levels(K, kw, group):
+    """k_direct (levels 0..K-1 by walk enumeration) against the oracle: regular,
+    symmetric (circulant and grid), non-regular, disconnected and asymmetric
+    tables; every count and word must equal the gather levels'."""
+    cases = []
+    adj = odp.generate_regular(2400, 5, 7)
+    cases.append((adj, dict(nodes=2400), oracle.evaluate(adj, nodes=2400)))
+    big = odp.generate_regular(8192, 4, 11)                       # 128 words per row
+    cases.append((big, dict(nodes=8192), oracle.evaluate(big, nodes=8192)))
+    base = odp.generate_general_s(1200, 4, 4, 5)
+    cases.append((base, dict(nodes=1200, symmetries=4),
+                  oracle.evaluate(oracle.expand(base, nodes=1200, symmetries=4), nodes=1200)))
+    full = odp.generate_grid(40, 40, 6, 4, 4, 3)
+    gb = odp.grid_base_table(40, 40, 6, 4, full)
+    cases.append((gb, dict(width=40, height=40, symmetries=4), oracle.evaluate(full, width=40, height=40)))
+    pa, nd = path_graph(300)
+    cases.append((pa, dict(nodes=300, num_degrees=nd), oracle.evaluate(pa, nodes=300, num_degrees=nd)))
+    dis = np.concatenate([cycle_graph(500), cycle_graph(700) + 500]).astype(np.int32)
+    cases.append((dis, dict(nodes=1200), (INT_MAX, LONG_MAX, DBL_MAX)))
+    rng = np.random.default_rng(K * 100 + kw)
+    asym = rng.integers(0, 900, size=(900, 3)).astype(np.int32)    # a directed table, rows need not agree
+    asym[:, 0] = (np.arange(900) + 1) % 900                         # with a directed cycle through all rows
+    cases.append((asym, dict(nodes=900), oracle.evaluate(asym, nodes=900)))
+    ran_any = False
+    for adj, init, want in cases:
+        got, ran = direct_eval(adj, K, kw, group, **dict(init))
+        assert same(got, want), (K, kw, group, init, got, want)
+        ref, _ = direct_eval(adj, 0, kw, group, **dict(init))
+        assert same(ref, want)
+        ran_any = ran_any or ran >= 2
+        assert ran == 0 or 2 <= ran <= K
+    assert ran_any, "k_direct never applied"
