@@ -350,9 +350,16 @@ struct Evaluator {
     for (int i = 0; i <= K; i++) { t += m; m *= dpad; if (m > (1L << 40)) m = 1L << 40; }
     return t;
   }
+  // ints of walk-tree ids k_direct keeps per wave: 4 for depth 0, dpad^d for 0 < d < K
+  long direct_list(int K) const
+  {
+    long t = 4, m = 1;
+    for (int i = 1; i < K; i++) { m *= dpad; t += m; }
+    return t;
+  }
   size_t direct_lds(int K, int kw, int group, int nw) const
   {
-    return (size_t)nw * group * kw * 8 + (size_t)K * group * 8;
+    return (size_t)nw * ((size_t)group * kw * 8 + (size_t)direct_list(K) * 4) + (size_t)K * group * 8;
   }
   // waves per workgroup k_direct can run with (0: not applicable): the block
   // rows must be a power of two of 2..128 words and one row bitmap must fit LDS
@@ -402,8 +409,8 @@ struct Evaluator {
     r.upw = plan.upw; r.seg_major = plan.seg_major; r.xcd = plan.xcd; r.urow = (lpr_log2 == 6);
     r.per_task_wg = (units + 4 * r.upw - 1) / (4 * r.upw);
     int K = plan.direct;
-    while (K >= 2 && direct_tree(K) > DIRECT_TREE_CAP) K--;
-    r.direct = plan.direct == -1 ? direct_auto(kw, group) : (K >= 2 && direct_waves(K, kw, group) ? K : 0);
+    while (K >= 2 && (direct_tree(K) > DIRECT_TREE_CAP || !direct_waves(K, kw, group))) K--;
+    r.direct = plan.direct == -1 ? direct_auto(kw, group) : (K >= 2 ? K : 0);
     if (r.direct >= n) r.direct = 0;
     plan.direct = r.direct;            // the plan reports the K that runs
     if (r.direct) {
@@ -536,7 +543,7 @@ struct Evaluator {
           a.n = n; a.dpad = dpad; a.kw = r.kw;
           a.src_lo = first * r.kw * 64;
           a.nsrc = std::min(cnt_t * r.kw * 64, nsrc - a.src_lo);
-          a.ntask = cnt_t; a.words = cnt_t * r.kw; a.K = K;
+          a.ntask = cnt_t; a.words = cnt_t * r.kw; a.K = K; a.list = (int)direct_list(K);
           long m = 1;
           for (int i = 0; i < 8; i++) { a.pw[i] = (int)m; m = std::min<long>(m * dpad, 1L << 30); }
           hipLaunchKernelGGL(k_direct, dim3(r.dgrid), dim3(64 * r.dnw), r.dlds, st, a);

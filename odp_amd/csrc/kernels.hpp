@@ -313,12 +313,15 @@ __global__ __launch_bounds__(LEVEL_WG) void k_level(const LevelTask* __restrict_
 // Row v of level L holds the sources reachable from v by a walk of at most
 // L+1 steps through the row-neighbour table (level 0: v and its neighbours;
 // each level ORs in the neighbours' rows).  While those sets are small they
-// are cheaper to enumerate than to gather: one wave per row walks every path
-// of length <= K from v (sum of dpad^i, i <= K, table reads), sets the bits in
-// an LDS bitmap of the group's row, popcounts it per block after each depth
-// (the counts of levels 0..K-1) and writes the depth-K row to the buffer level
-// K reads.  The sets are exactly those the level kernels build, so every count
-// and every word is identical to running k_level0 and levels 1..K-1.
+// are cheaper to enumerate than to gather.  One wave per row expands the walk
+// tree of v breadth-first: the ids at depth d-1 sit in an LDS list, the wave
+// loads their table rows (16 B per load, DIRECT_UNROLL independent loads per
+// lane in flight), sets the bits in an LDS bitmap of the group's row and, for
+// d < K, appends the ids to the depth-d list.  After each depth the bitmap is
+// popcounted per block (the count of level d-1); after depth K the row is
+// written to the buffer level K reads.  The sets are exactly those the level
+// kernels build, so every count and every word equals k_level0 followed by
+// levels 1..K-1.
 
 struct DirectArgs {
   const int* nbr;        // [n][dpad] row-neighbour table
@@ -330,10 +333,12 @@ struct DirectArgs {
   int src_lo, nsrc;      // sources of the group: [src_lo, src_lo + nsrc)
   int ntask, words;      // blocks in the group; words = ntask * kw
   int K;                 // levels computed (2..DIRECT_KMAX)
+  int list;              // ints of walk-tree ids kept per wave: 4 + sum of dpad^i, 0 < i < K
   int pw[8];             // dpad^i
 };
 
 #define DIRECT_KMAX 6
+#define DIRECT_UNROLL 8
 
 __device__ __forceinline__ void direct_set(u32* bm, int id, int src_lo, int nsrc)
 {
@@ -349,35 +354,53 @@ __global__ __launch_bounds__(256) void k_direct(DirectArgs a)
   const int lane = threadIdx.x & 63;
   u64* bm = dlds + (int64_t)wave * a.words;
   u32* bm32 = (u32*)bm;
-  u64* lcnt = dlds + (int64_t)nw * a.words;        // [K][ntask] counts of this workgroup
+  // LDS: nw row bitmaps, nw id lists, then the [K][ntask] counts of this workgroup.
+  // Depth 0 takes 4 ints of a list and depth d >= 1 dpad^d, so every depth
+  // starts 16-byte aligned (dpad is a multiple of 4).
+  int* list = (int*)(dlds + (int64_t)nw * a.words) + wave * a.list;
+  u64* lcnt = (u64*)((int*)(dlds + (int64_t)nw * a.words) + nw * a.list);
   for (int i = threadIdx.x; i < a.K * a.ntask; i += blockDim.x) lcnt[i] = 0;
   const int half = a.kw >> 1;                      // lanes per block row (two words a lane)
+  const int Q = a.dpad >> 2;                       // 16-byte loads per table row
   for (int base = blockIdx.x * nw; base < a.n; base += gridDim.x * nw) {   // uniform per workgroup
     const int v = base + wave;
     const bool act = v < a.n;
     for (int w = 2 * lane; w < a.words; w += 128) { u64x2 z; z.x = 0; z.y = 0; *(u64x2*)(bm + w) = z; }
+    if (act && lane == 0) list[0] = v;
     __syncthreads();
     if (act && lane == 0) direct_set(bm32, v, a.src_lo, a.nsrc);
+    int off = 0;                                   // start of the depth d-1 list
     for (int d = 1; d <= a.K; d++) {
-      // depth d: the neighbours of every node at depth d-1 (path p, base-dpad digits)
+      const int m = a.pw[d - 1];                   // ids at depth d-1
+      const int jobs = m * Q;
+      const int* from = list + off;
+      const int step = d == 1 ? 4 : m;             // ints the depth d-1 list takes
+      int* to = list + off + step;                 // the depth-d list follows
       if (act) {
-        const int m = a.pw[d - 1];
-        for (int p = lane; p < m; p += 64) {
-          int x = v;
-          for (int k = d - 2; k >= 0; k--) {
-            const int digit = (p / a.pw[k]) % a.dpad;
-            x = a.nbr[(int64_t)x * a.dpad + digit];
+        for (int j0 = 0; j0 < jobs; j0 += 64 * DIRECT_UNROLL) {
+          int4 q[DIRECT_UNROLL];
+#pragma unroll
+          for (int u = 0; u < DIRECT_UNROLL; u++) {
+            const int idx = j0 + u * 64 + lane;
+            if (idx < jobs) {
+              const int r = idx / Q;
+              q[u] = *(const int4*)(a.nbr + (int64_t)from[r] * a.dpad + 4 * (idx - r * Q));
+            }
           }
-          const int* row = a.nbr + (int64_t)x * a.dpad;
-          for (int j = 0; j < a.dpad; j += 4) {
-            const int4 q = *(const int4*)(row + j);
-            direct_set(bm32, q.x, a.src_lo, a.nsrc);
-            direct_set(bm32, q.y, a.src_lo, a.nsrc);
-            direct_set(bm32, q.z, a.src_lo, a.nsrc);
-            direct_set(bm32, q.w, a.src_lo, a.nsrc);
+#pragma unroll
+          for (int u = 0; u < DIRECT_UNROLL; u++) {
+            const int idx = j0 + u * 64 + lane;
+            if (idx < jobs) {
+              direct_set(bm32, q[u].x, a.src_lo, a.nsrc);
+              direct_set(bm32, q[u].y, a.src_lo, a.nsrc);
+              direct_set(bm32, q[u].z, a.src_lo, a.nsrc);
+              direct_set(bm32, q[u].w, a.src_lo, a.nsrc);
+              if (d < a.K) *(int4*)(to + 4 * idx) = q[u];
+            }
           }
         }
       }
+      off += step;
       __syncthreads();
       // level d-1: per-block popcounts; after the last depth, the row itself
       if (act) {
