@@ -51,6 +51,7 @@ typedef struct {
   int group;      /* level: column blocks resident at once; 0 = all (MATRIX)          */
   int direct;     /* level: levels 0..K-1 by direct walk enumeration in one launch:
                      0 = off, -1 = the library picks K, K >= 2 = that K (at most 6)     */
+  int batch;      /* level: neighbour rows gathered per batch: 4 (also 0), 8 or 16     */
 } ODPX_Plan;
 
 /* Plan of the current evaluator (after Init).  Set: 0 on success, -1 if invalid.

@@ -77,7 +77,8 @@ static void ctx_init()
   C.ncu = prop.multiProcessorCount;
   HIPC(hipStreamCreateWithFlags(&C.stream, hipStreamNonBlocking));
   HIPC(hipFuncSetAttribute((const void*)k_resident, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_DYN_MAX));
-  HIPC(hipFuncSetAttribute((const void*)k_direct, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_DYN_MAX));
+  HIPC(hipFuncSetAttribute((const void*)k_direct<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_DYN_MAX));
+  HIPC(hipFuncSetAttribute((const void*)k_direct<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_DYN_MAX));
   C.ready = true;
   trace("ctx ready");
 }
@@ -134,6 +135,7 @@ static void timed_end(TimedLaunch& t)
 struct LevelRun {
   int n = 0, ntasks = 0, group = 0, kw = 0, kwp = 0, upw = 4, seg_major = 1, xcd = 0, urow = 0, per_task_wg = 0;
   int direct = 0, dnw = 4, dgrid = 0;   // levels done by k_direct (0: k_level0), its waves and grid
+  int batch = 4;                        // neighbour rows k_level gathers per batch
   size_t dlds = 0;
   size_t buf_words = 0;                 // capacity of bufs (64-bit words)
   size_t task_cap = 0;
@@ -338,6 +340,7 @@ struct Evaluator {
     if (p.plan == ODPX_PLAN_LEVEL) {
       if (!(p.kw >= 0 && p.upw >= 1 && p.upw <= 64 && p.group >= 0)) return false;
       // direct K that does not apply to this graph's blocks falls back to k_level0
+      if (p.batch != 0 && p.batch != 4 && p.batch != 8 && p.batch != 16) return false;
       return p.direct == 0 || p.direct == -1 || (p.direct >= 2 && p.direct <= DIRECT_KMAX);
     }
     return false;
@@ -359,13 +362,14 @@ struct Evaluator {
   }
   size_t direct_lds(int K, int kw, int group, int nw) const
   {
-    return (size_t)nw * ((size_t)group * kw * 8 + (size_t)direct_list(K) * 4) + (size_t)K * group * 8;
+    const size_t words = (size_t)group * kw, chunks = (words + 127) / 128;
+    return (size_t)nw * (words * 8 + (size_t)direct_list(K) * 4) + (size_t)K * chunks * 64 * 4;
   }
   // waves per workgroup k_direct can run with (0: not applicable): the block
   // rows must be a power of two of 2..128 words and one row bitmap must fit LDS
   int direct_waves(int K, int kw, int group) const
   {
-    if (K < 2 || K > DIRECT_KMAX || kw < 2 || kw > 128 || (kw & (kw - 1))) return 0;
+    if (K < 2 || K > DIRECT_KMAX || kw < 2 || kw > 128 || (kw & (kw - 1)) || dpad > 1024) return 0;
     for (int nw = 4; nw >= 1; nw >>= 1)
       if (direct_lds(K, kw, group, nw) <= LDS_DYN_MAX) return nw;
     return 0;
@@ -407,6 +411,7 @@ struct Evaluator {
     // the previous contents are never read: level 0 rewrites every word it uses
     r.n = n; r.ntasks = ntasks; r.group = group; r.kw = kw; r.kwp = kwp;
     r.upw = plan.upw; r.seg_major = plan.seg_major; r.xcd = plan.xcd; r.urow = (lpr_log2 == 6);
+    r.batch = plan.batch == 8 || plan.batch == 16 ? plan.batch : 4;
     r.per_task_wg = (units + 4 * r.upw - 1) / (4 * r.upw);
     int K = plan.direct;
     while (K >= 2 && (direct_tree(K) > DIRECT_TREE_CAP || !direct_waves(K, kw, group))) K--;
@@ -540,13 +545,16 @@ struct Evaluator {
           a.out = r.bufs + ((K & 1) ? (size_t)n * r.kwp : 0);     // the buffer level K reads
           a.tstride = (int64_t)2 * n * r.kwp;
           a.cnt = r.cnt + first; a.cstride = r.ntasks;
-          a.n = n; a.dpad = dpad; a.kw = r.kw;
+          a.n = n; a.dpad = dpad; a.kw = r.kw; a.kwl = ilog2(r.kw);
           a.src_lo = first * r.kw * 64;
           a.nsrc = std::min(cnt_t * r.kw * 64, nsrc - a.src_lo);
-          a.ntask = cnt_t; a.words = cnt_t * r.kw; a.K = K; a.list = (int)direct_list(K);
+          a.ntask = cnt_t; a.words = cnt_t * r.kw; a.chunks = (a.words + 127) / 128;
+          a.K = K; a.list = (int)direct_list(K);
+          a.qmul = dpad > 4 ? (unsigned)((1ULL << 32) / (unsigned)(dpad / 4) + 1) : 0u;
           long m = 1;
           for (int i = 0; i < 8; i++) { a.pw[i] = (int)m; m = std::min<long>(m * dpad, 1L << 30); }
-          hipLaunchKernelGGL(k_direct, dim3(r.dgrid), dim3(64 * r.dnw), r.dlds, st, a);
+          const bool all = a.src_lo == 0 && a.nsrc == n;
+          hipLaunchKernelGGL(all ? k_direct<true> : k_direct<false>, dim3(r.dgrid), dim3(64 * r.dnw), r.dlds, st, a);
         } else {
           hipLaunchKernelGGL(k_level0, dim3(grid), dim3(LEVEL_WG), 0, st,
                              tasks, r.wg_start, cnt_t, RING, r.upw, r.xcd);
@@ -558,12 +566,10 @@ struct Evaluator {
         for (int L = from; L < to; L++) {
           TimedLaunch tl;
           timed_begin(tl, T_LEVEL, L, st);
-          if (r.urow)
-            hipLaunchKernelGGL(k_level<true>, dim3(grid), dim3(LEVEL_WG), 0, st,
-                               tasks, r.wg_start, cnt_t, L, RING, r.upw, r.xcd);
-          else
-            hipLaunchKernelGGL(k_level<false>, dim3(grid), dim3(LEVEL_WG), 0, st,
-                               tasks, r.wg_start, cnt_t, L, RING, r.upw, r.xcd);
+          auto kern = r.urow ? (r.batch == 16 ? k_level<true, 16> : r.batch == 8 ? k_level<true, 8> : k_level<true, 4>)
+                             : (r.batch == 16 ? k_level<false, 16> : r.batch == 8 ? k_level<false, 8> : k_level<false, 4>);
+          hipLaunchKernelGGL(kern, dim3(grid), dim3(LEVEL_WG), 0, st,
+                             tasks, r.wg_start, cnt_t, L, RING, r.upw, r.xcd);
           HIPC(hipGetLastError());
           timed_end(tl);
         }

@@ -247,8 +247,9 @@ __global__ __launch_bounds__(LEVEL_WG) void k_level0(const LevelTask* __restrict
 }
 
 // One gather level L >= 1 of every task.  UROW: one row per wave, so the row's
-// neighbour ids are wave-uniform (scalar loads).
-template <bool UROW>
+// neighbour ids are wave-uniform (scalar loads).  G: neighbour rows gathered
+// per batch (in flight together) before the wave checks for saturation.
+template <bool UROW, int G>
 __global__ __launch_bounds__(LEVEL_WG) void k_level(const LevelTask* __restrict__ tasks,
                                                     const int* __restrict__ wg_start, int ntasks,
                                                     int level, int ring, int upw, int xcd)
@@ -284,19 +285,32 @@ __global__ __launch_bounds__(LEVEL_WG) void k_level(const LevelTask* __restrict_
     if (q.r < T.n) {
       const u64 f0 = full_word(T.l, q.w), f1 = full_word(T.l, q.w + 1);
       u64x2 x = *(const u64x2*)(cur + q.r * kwp + q.w);
-      // Once every lane's words are saturated the remaining neighbours cannot
-      // add a bit, so the wave stops gathering (exact: OR is monotone).
-      if (!__all(x.x == f0 && x.y == f1)) {
+      // Once a lane's words are saturated the remaining neighbours cannot add
+      // a bit (OR is monotone): the lane stops gathering (its loads are masked
+      // off, so they cost no memory requests) and the wave stops once every
+      // lane has.
+      bool need = !(x.x == f0 && x.y == f1);
+      if (__any(need)) {
         const int* nb = T.nbr + (int64_t)q.r * T.dpad;
-        for (int j = 0; j < T.dpad; j += 4) {
-          const int4 v = *(const int4*)(nb + j);
-          const u64x2 a = *(const u64x2*)(cur + (int64_t)v.x * kwp + q.w);
-          const u64x2 b = *(const u64x2*)(cur + (int64_t)v.y * kwp + q.w);
-          const u64x2 c = *(const u64x2*)(cur + (int64_t)v.z * kwp + q.w);
-          const u64x2 d = *(const u64x2*)(cur + (int64_t)v.w * kwp + q.w);
-          x.x |= (a.x | b.x) | (c.x | d.x);
-          x.y |= (a.y | b.y) | (c.y | d.y);
-          if (__all(x.x == f0 && x.y == f1)) break;
+        for (int j = 0; j < T.dpad; j += G) {
+          int id[G];
+#pragma unroll
+          for (int k = 0; k < G; k += 4) {
+            int4 v;
+            if (need && (k == 0 || j + k < T.dpad)) v = *(const int4*)(nb + j + k);
+            else { v.x = q.r; v.y = q.r; v.z = q.r; v.w = q.r; }
+            id[k] = v.x; id[k + 1] = v.y; id[k + 2] = v.z; id[k + 3] = v.w;
+          }
+          u64x2 g[G];
+#pragma unroll
+          for (int k = 0; k < G; k++) {
+            if (need) g[k] = *(const u64x2*)(cur + (int64_t)id[k] * kwp + q.w);
+            else { g[k].x = 0; g[k].y = 0; }
+          }
+#pragma unroll
+          for (int k = 0; k < G; k++) { x.x |= g[k].x; x.y |= g[k].y; }
+          need = !(x.x == f0 && x.y == f1);
+          if (!__any(need)) break;
         }
       }
       *(u64x2*)(nxt + q.r * kwp + q.w) = x;
@@ -329,23 +343,29 @@ struct DirectArgs {
   int64_t tstride;       // words between the slabs of consecutive blocks
   u64* cnt;              // count of block t at level L: cnt[L * cstride + t], L < K
   int cstride;
-  int n, dpad, kw;       // kw: words per block row, a power of two in [2, 128]
+  int n, dpad, kw, kwl;  // kw: words per block row, a power of two in [2, 128]; kwl = log2(kw)
   int src_lo, nsrc;      // sources of the group: [src_lo, src_lo + nsrc)
   int ntask, words;      // blocks in the group; words = ntask * kw
+  int chunks;            // ceil(words / 128): 128-word chunks a wave sweeps per row
   int K;                 // levels computed (2..DIRECT_KMAX)
   int list;              // ints of walk-tree ids kept per wave: 4 + sum of dpad^i, 0 < i < K
+  unsigned qmul;         // floor(2^32 / q) + 1 for q = dpad / 4 > 1: idx / q == umulhi(idx, qmul)
+                         // for idx < 2^32 / q (idx < 2^24 and q <= 256 here)
   int pw[8];             // dpad^i
 };
 
 #define DIRECT_KMAX 6
 #define DIRECT_UNROLL 8
 
+// ALL: every id is a source of the group (src_lo == 0, nsrc == n), no range test
+template <bool ALL>
 __device__ __forceinline__ void direct_set(u32* bm, int id, int src_lo, int nsrc)
 {
   const int c = id - src_lo;
-  if ((unsigned)c < (unsigned)nsrc) atomicOr(&bm[c >> 5], 1u << (c & 31));
+  if (ALL || (unsigned)c < (unsigned)nsrc) atomicOr(&bm[c >> 5], 1u << (c & 31));
 }
 
+template <bool ALL>
 __global__ __launch_bounds__(256) void k_direct(DirectArgs a)
 {
   extern __shared__ __attribute__((aligned(16))) u64 dlds[];
@@ -354,21 +374,25 @@ __global__ __launch_bounds__(256) void k_direct(DirectArgs a)
   const int lane = threadIdx.x & 63;
   u64* bm = dlds + (int64_t)wave * a.words;
   u32* bm32 = (u32*)bm;
-  // LDS: nw row bitmaps, nw id lists, then the [K][ntask] counts of this workgroup.
+  // LDS: nw row bitmaps, nw id lists, then per-lane counts lc[K][chunks][64] of
+  // the workgroup (lane l of chunk j holds words 128 j + 2 l, 128 j + 2 l + 1).
   // Depth 0 takes 4 ints of a list and depth d >= 1 dpad^d, so every depth
   // starts 16-byte aligned (dpad is a multiple of 4).
   int* list = (int*)(dlds + (int64_t)nw * a.words) + wave * a.list;
-  u64* lcnt = (u64*)((int*)(dlds + (int64_t)nw * a.words) + nw * a.list);
-  for (int i = threadIdx.x; i < a.K * a.ntask; i += blockDim.x) lcnt[i] = 0;
-  const int half = a.kw >> 1;                      // lanes per block row (two words a lane)
+  u32* lc = (u32*)((int*)(dlds + (int64_t)nw * a.words) + nw * a.list);
+  const int ncnt = a.K * a.chunks * 64;
+  for (int i = threadIdx.x; i < ncnt; i += blockDim.x) lc[i] = 0;
+  __syncthreads();
   const int Q = a.dpad >> 2;                       // 16-byte loads per table row
-  for (int base = blockIdx.x * nw; base < a.n; base += gridDim.x * nw) {   // uniform per workgroup
-    const int v = base + wave;
-    const bool act = v < a.n;
+  // Each wave works on its own rows, bitmap and lists: between the phases of a
+  // row only the wave's own LDS accesses must be ordered, which LDS guarantees
+  // for one wave's instructions, so a wavefront fence (compiler ordering) is
+  // enough and the waves of a workgroup never wait for each other.
+  for (int v = blockIdx.x * nw + wave; v < a.n; v += gridDim.x * nw) {   // wave-uniform
     for (int w = 2 * lane; w < a.words; w += 128) { u64x2 z; z.x = 0; z.y = 0; *(u64x2*)(bm + w) = z; }
-    if (act && lane == 0) list[0] = v;
-    __syncthreads();
-    if (act && lane == 0) direct_set(bm32, v, a.src_lo, a.nsrc);
+    if (lane == 0) list[0] = v;
+    __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+    if (lane == 0) direct_set<ALL>(bm32, v, a.src_lo, a.nsrc);
     int off = 0;                                   // start of the depth d-1 list
     for (int d = 1; d <= a.K; d++) {
       const int m = a.pw[d - 1];                   // ids at depth d-1
@@ -376,51 +400,56 @@ __global__ __launch_bounds__(256) void k_direct(DirectArgs a)
       const int* from = list + off;
       const int step = d == 1 ? 4 : m;             // ints the depth d-1 list takes
       int* to = list + off + step;                 // the depth-d list follows
-      if (act) {
-        for (int j0 = 0; j0 < jobs; j0 += 64 * DIRECT_UNROLL) {
-          int4 q[DIRECT_UNROLL];
+      for (int j0 = 0; j0 < jobs; j0 += 64 * DIRECT_UNROLL) {
+        int4 q[DIRECT_UNROLL];
 #pragma unroll
-          for (int u = 0; u < DIRECT_UNROLL; u++) {
-            const int idx = j0 + u * 64 + lane;
-            if (idx < jobs) {
-              const int r = idx / Q;
-              q[u] = *(const int4*)(a.nbr + (int64_t)from[r] * a.dpad + 4 * (idx - r * Q));
-            }
+        for (int u = 0; u < DIRECT_UNROLL; u++) {
+          const unsigned idx = j0 + u * 64 + lane;
+          if (idx < (unsigned)jobs) {
+            const unsigned r = Q == 1 ? idx : __umulhi(idx, a.qmul);
+            q[u] = *(const int4*)(a.nbr + (int64_t)from[r] * a.dpad + 4 * (idx - r * Q));
           }
+        }
 #pragma unroll
-          for (int u = 0; u < DIRECT_UNROLL; u++) {
-            const int idx = j0 + u * 64 + lane;
-            if (idx < jobs) {
-              direct_set(bm32, q[u].x, a.src_lo, a.nsrc);
-              direct_set(bm32, q[u].y, a.src_lo, a.nsrc);
-              direct_set(bm32, q[u].z, a.src_lo, a.nsrc);
-              direct_set(bm32, q[u].w, a.src_lo, a.nsrc);
-              if (d < a.K) *(int4*)(to + 4 * idx) = q[u];
-            }
+        for (int u = 0; u < DIRECT_UNROLL; u++) {
+          const int idx = j0 + u * 64 + lane;
+          if (idx < jobs) {
+            direct_set<ALL>(bm32, q[u].x, a.src_lo, a.nsrc);
+            direct_set<ALL>(bm32, q[u].y, a.src_lo, a.nsrc);
+            direct_set<ALL>(bm32, q[u].z, a.src_lo, a.nsrc);
+            direct_set<ALL>(bm32, q[u].w, a.src_lo, a.nsrc);
+            if (d < a.K) *(int4*)(to + 4 * idx) = q[u];
           }
         }
       }
       off += step;
-      __syncthreads();
-      // level d-1: per-block popcounts; after the last depth, the row itself
-      if (act) {
-        for (int w0 = 0; w0 < a.words; w0 += 128) {      // wave-uniform trip count
-          const int w = w0 + 2 * lane;
-          u64x2 x; x.x = 0; x.y = 0;
-          if (w < a.words) x = *(const u64x2*)(bm + w);
-          u64 c = (u64)(__popcll(x.x) + __popcll(x.y));
-          for (int o = 1; o < half; o <<= 1) c += __shfl_xor(c, o, 64);
-          const int t = w / a.kw;
-          if (w < a.words && (lane & (half - 1)) == 0 && c) atomicAdd(&lcnt[(d - 1) * a.ntask + t], c);
-          if (d == a.K && w < a.words)
-            *(u64x2*)(a.out + (int64_t)t * a.tstride + (int64_t)v * a.kw + (w & (a.kw - 1))) = x;
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+      // level d-1: popcounts into the lane's counters; after the last depth, the row itself
+      u32* lcd = lc + (d - 1) * a.chunks * 64 + lane;
+      for (int j = 0; j < a.chunks; j++) {
+        const int w = 128 * j + 2 * lane;
+        if (w < a.words) {
+          const u64x2 x = *(const u64x2*)(bm + w);
+          const u32 c = (u32)(__popcll(x.x) + __popcll(x.y));
+          if (c) atomicAdd(&lcd[64 * j], c);
+          if (d == a.K)
+            *(u64x2*)(a.out + (int64_t)(w >> a.kwl) * a.tstride + (int64_t)v * a.kw + (w & (a.kw - 1))) = x;
         }
       }
-      __syncthreads();
+      __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
     }
   }
-  for (int i = threadIdx.x; i < a.K * a.ntask; i += blockDim.x)
-    if (lcnt[i]) atomicAdd(&a.cnt[(int64_t)(i / a.ntask) * a.cstride + (i % a.ntask)], lcnt[i]);
+  __syncthreads();
+  // per block: the kw/2 lanes of a chunk that share a block row add up
+  const int half = a.kw >> 1;
+  for (int e = wave; e < a.K * a.chunks; e += nw) {            // wave-uniform
+    const int L = e / a.chunks, j = e - L * a.chunks;
+    u64 c = lc[e * 64 + lane];
+    for (int o = 1; o < half; o <<= 1) c += __shfl_xor(c, o, 64);
+    const int w = 128 * j + 2 * lane;
+    if (w < a.words && (lane & (half - 1)) == 0 && c)
+      atomicAdd(&a.cnt[(int64_t)L * a.cstride + (w >> a.kwl)], c);
+  }
 }
 
 // ---------------------------------------------------------------------------
