@@ -128,7 +128,7 @@ def served_extra(nchains=64, saturate_s=4.0, poisson_s=6.0, load=0.6, seed=77):
     mirrors = [odp.generate_regular(ns[i % 3], ds[(i // 3) % 3], 5000 + i) for i in range(nchains)]
     log(f"served: generated {nchains} chains in {time.perf_counter() - t0:.1f}s")
     rng = odp.Rng(seed)
-    srv = odp.Server(small_workers=2, large_workers=4, hbm_budget_gb=64)
+    srv = odp.Server(small_slots=2, large_slots=4, hbm_budget_gb=64)
     samples = []            # (graph copy, request id) checked against the reference later
     want_sample = {2: None, 3: None, 5: None}    # chains 2 (n=65536), 3 (n=4096), 5 (n=65536)
     try:

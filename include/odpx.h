@@ -122,7 +122,7 @@ void ODPX_Batch_release(void);
  * of 2-opt swaps (a-b, c-d -> a-c, b-d, each as {a, b, c, d}) to its chain and
  * evaluates the result exactly.  Requests of one chain run in submission order;
  * requests of different chains are batched and overlapped on the GPU as
- * workers free up.  Results equal ODP_Set_aspl on the same graph. */
+ * slots free up.  Results equal ODP_Set_aspl on the same graph. */
 typedef struct {
   long long id;               /* request id returned by ODPX_Serve_submit            */
   int    chain;
@@ -132,10 +132,13 @@ typedef struct {
   double ASPL;
   double t_submit, t_start, t_done;   /* seconds on the host's monotonic clock     */
 } ODPX_Result;
-/* small_workers: launchers of LDS-resident batches; large_workers: evaluators of
- * graphs too large for LDS, each holding its own bit buffers; hbm_budget_gb:
- * bound on chain tables plus those buffers.  0 on success. */
-int  ODPX_Serve_start(int small_workers, int large_workers, double hbm_budget_gb);
+/* small_slots: concurrent LDS-resident batch launches (each its own stream);
+ * large_slots: concurrent evaluations of graphs too large for LDS, each holding
+ * its own bit buffers and stream; hbm_budget_gb: bound on chain tables plus
+ * those buffers.  0 on success.  The server is advanced by its caller: submit,
+ * poll, wait and drain collect finished work and admit waiting requests, so
+ * every HIP call is made by the calling thread (calls are serialised). */
+int  ODPX_Serve_start(int small_slots, int large_slots, double hbm_budget_gb);
 /* adjacency int[nodes][degree]; returns the chain id, or -1 (over the HBM budget,
  * invalid graph, or no server) */
 int  ODPX_Serve_add_chain(int nodes, int degree, const int *adjacency);
@@ -147,7 +150,7 @@ int  ODPX_Serve_poll(ODPX_Result *out, int max);
 int  ODPX_Serve_wait(long long id, ODPX_Result *out);
 /* blocks until every submitted request has completed */
 void ODPX_Serve_drain(void);
-/* drains, stops the workers and frees the chains */
+/* drains, frees the slots and the chains */
 void ODPX_Serve_stop(void);
 
 #ifdef __cplusplus

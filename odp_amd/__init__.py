@@ -407,8 +407,8 @@ class Server:
     """ODPX_Serve_*: continuous batching of 2-opt evaluation requests over
     chains whose graphs stay resident in HBM."""
 
-    def __init__(self, small_workers=2, large_workers=4, hbm_budget_gb=64.0):
-        if load().ODPX_Serve_start(small_workers, large_workers, hbm_budget_gb) != 0:
+    def __init__(self, small_slots=2, large_slots=4, hbm_budget_gb=64.0):
+        if load().ODPX_Serve_start(small_slots, large_slots, hbm_budget_gb) != 0:
             raise RuntimeError("a server is already running")
 
     def add_chain(self, adjacency):

@@ -515,114 +515,164 @@ struct Evaluator {
     }
   }
 
-  // returns the ordered-pair excess over n(n-1) and the diameter; disconnected -> *disc.
-  // Column blocks run `group` at a time (all at once for MATRIX); every launch
-  // advances each block of the group by one level.
-  void run_level(u64* excess, int* diam, bool* disc)
+  // ---- level plan as a resumable sequence ----
+  // lv_begin() enqueues prepare, the first levels and the first batch of
+  // gather levels of the first group, ending with copies of the counter ring
+  // and the error flag to the host.  Once the stream has passed those copies,
+  // lv_step() examines the counters and enqueues the next batch (or the next
+  // group), returning true when the evaluation is complete; lv_result() then
+  // gives the ordered-pair excess over n(n-1), the diameter and whether the
+  // graph is disconnected.  Column blocks run `group` at a time (all at once
+  // for MATRIX); every launch advances each block of the group by one level.
+  struct LvTask { int running; u64 sum, prev; int lwork; };
+  struct LvState {
+    int K = 0, L1 = 1, nzero = 0, be0 = 2;
+    bool graphed = false;
+    int first = 0, cnt_t = 0;
+    int launched = 0, examined = 0, running = 0, batch_end = 0;
+    std::vector<LvTask> sts;
+    bool any_disc = false;
+    int max_d = 1, lmax = 0;
+  } lv;
+
+  void lv_level0()
+  {
+    LevelRun& r = lr;
+    const int K = lv.K, first = lv.first, cnt_t = lv.cnt_t;
+    TimedLaunch tl;
+    timed_begin(tl, T_OTHER, 0, st);
+    if (K) {
+      DirectArgs a;
+      a.nbr = d_nbr;
+      a.out = r.bufs + ((K & 1) ? (size_t)n * r.kwp : 0);     // the buffer level K reads
+      a.tstride = (int64_t)2 * n * r.kwp;
+      a.cnt = r.cnt + first; a.cstride = r.ntasks;
+      a.n = n; a.dpad = dpad; a.kw = r.kw; a.kwl = ilog2(r.kw);
+      a.src_lo = first * r.kw * 64;
+      a.nsrc = std::min(cnt_t * r.kw * 64, nsrc - a.src_lo);
+      a.ntask = cnt_t; a.words = cnt_t * r.kw; a.chunks = (a.words + 127) / 128;
+      a.K = K; a.list = (int)direct_list(K);
+      a.qmul = dpad > 4 ? (unsigned)((1ULL << 32) / (unsigned)(dpad / 4) + 1) : 0u;
+      long m = 1;
+      for (int i = 0; i < 8; i++) { a.pw[i] = (int)m; m = std::min<long>(m * dpad, 1L << 30); }
+      const bool all = a.src_lo == 0 && a.nsrc == n;
+      hipLaunchKernelGGL(all ? k_direct<true> : k_direct<false>, dim3(r.dgrid), dim3(64 * r.dnw), r.dlds, st, a);
+    } else {
+      hipLaunchKernelGGL(k_level0, dim3(cnt_t * r.per_task_wg), dim3(LEVEL_WG), 0, st,
+                         r.tasks + first, r.wg_start, cnt_t, RING, r.upw, r.xcd);
+    }
+    HIPC(hipGetLastError());
+    timed_end(tl);
+  }
+
+  // gather levels [from, to) of the current group, then the counter and error copies
+  void lv_levels(int from, int to)
+  {
+    LevelRun& r = lr;
+    const int grid = lv.cnt_t * r.per_task_wg;
+    for (int L = from; L < to; L++) {
+      TimedLaunch tl;
+      timed_begin(tl, T_LEVEL, L, st);
+      auto kern = r.urow ? (r.batch == 16 ? k_level<true, 16> : r.batch == 8 ? k_level<true, 8> : k_level<true, 4>)
+                         : (r.batch == 16 ? k_level<false, 16> : r.batch == 8 ? k_level<false, 8> : k_level<false, 4>);
+      hipLaunchKernelGGL(kern, dim3(grid), dim3(LEVEL_WG), 0, st,
+                         r.tasks + lv.first, r.wg_start, lv.cnt_t, L, RING, r.upw, r.xcd);
+      HIPC(hipGetLastError());
+      timed_end(tl);
+    }
+    HIPC(hipMemcpyAsync(r.h_cnt, r.cnt, sizeof(u64) * RING * r.ntasks, hipMemcpyDeviceToHost, st));
+    HIPC(hipMemcpyAsync(h_err, d_err, sizeof(int), hipMemcpyDeviceToHost, st));
+  }
+
+  // the first levels and the first batch of group lv.first
+  void lv_group_start()
+  {
+    LevelRun& r = lr;
+    lv.cnt_t = std::min(r.group, r.ntasks - lv.first);
+    lv.launched = lv.L1; lv.examined = 0; lv.running = lv.cnt_t;
+    lv.batch_end = lv.first == 0 ? lv.be0 : std::min(n, std::max(2, guess_levels > 0 ? guess_levels : 8));
+    if (lv.first == 0 && lv.graphed) {
+      // prepare + the first levels + the first batch + the counter copies as one graph
+      const int be = lv.batch_end, L1 = lv.L1, nz = lv.nzero;
+      if (!graph_ready(be))
+        capture([&]() { prepare(r.cnt, nz, r.done, r.ntasks); lv_level0(); lv_levels(L1, be); }, be);
+      HIPC(hipGraphLaunch(gexec, st));
+    } else {
+      lv_level0();
+      lv_levels(lv.launched, lv.batch_end);
+    }
+    lv.launched = std::max(lv.launched, lv.batch_end);
+  }
+
+  void lv_begin()
   {
     LevelRun& r = lr;
     trace("run_level");
-    const int K = r.direct;            // levels 0..K-1 come from k_direct (0: k_level0 does level 0)
-    const int L1 = K ? K : 1;           // first level the gather kernel runs
-    const int nzero = (K ? K + 1 : 1) * r.ntasks;   // ring slots 0..L1 start at zero
-    const int be0 = std::min(n, std::max(2, guess_levels > 0 ? guess_levels : 8));   // first batch
-    const bool graphed = r.group == r.ntasks && use_graph(be0);
-    if (!graphed) prepare(r.cnt, nzero, r.done, r.ntasks);   // the first ring slots and the done flags
-    struct St { int running; u64 sum, prev; int lwork; };
-    std::vector<St> sts(r.ntasks, St{1, 0, 0, 0});
-    bool any_disc = false;
-    int max_d = 1, lmax = 0;
-    for (int first = 0; first < r.ntasks; first += r.group) {
-      const int cnt_t = std::min(r.group, r.ntasks - first);
-      const LevelTask* tasks = r.tasks + first;
-      const int grid = cnt_t * r.per_task_wg;
-      auto level0 = [&]() {
-        TimedLaunch tl;
-        timed_begin(tl, T_OTHER, 0, st);
-        if (K) {
-          DirectArgs a;
-          a.nbr = d_nbr;
-          a.out = r.bufs + ((K & 1) ? (size_t)n * r.kwp : 0);     // the buffer level K reads
-          a.tstride = (int64_t)2 * n * r.kwp;
-          a.cnt = r.cnt + first; a.cstride = r.ntasks;
-          a.n = n; a.dpad = dpad; a.kw = r.kw; a.kwl = ilog2(r.kw);
-          a.src_lo = first * r.kw * 64;
-          a.nsrc = std::min(cnt_t * r.kw * 64, nsrc - a.src_lo);
-          a.ntask = cnt_t; a.words = cnt_t * r.kw; a.chunks = (a.words + 127) / 128;
-          a.K = K; a.list = (int)direct_list(K);
-          a.qmul = dpad > 4 ? (unsigned)((1ULL << 32) / (unsigned)(dpad / 4) + 1) : 0u;
-          long m = 1;
-          for (int i = 0; i < 8; i++) { a.pw[i] = (int)m; m = std::min<long>(m * dpad, 1L << 30); }
-          const bool all = a.src_lo == 0 && a.nsrc == n;
-          hipLaunchKernelGGL(all ? k_direct<true> : k_direct<false>, dim3(r.dgrid), dim3(64 * r.dnw), r.dlds, st, a);
-        } else {
-          hipLaunchKernelGGL(k_level0, dim3(grid), dim3(LEVEL_WG), 0, st,
-                             tasks, r.wg_start, cnt_t, RING, r.upw, r.xcd);
-        }
-        HIPC(hipGetLastError());
-        timed_end(tl);
-      };
-      auto levels = [&](int from, int to) {
-        for (int L = from; L < to; L++) {
-          TimedLaunch tl;
-          timed_begin(tl, T_LEVEL, L, st);
-          auto kern = r.urow ? (r.batch == 16 ? k_level<true, 16> : r.batch == 8 ? k_level<true, 8> : k_level<true, 4>)
-                             : (r.batch == 16 ? k_level<false, 16> : r.batch == 8 ? k_level<false, 8> : k_level<false, 4>);
-          hipLaunchKernelGGL(kern, dim3(grid), dim3(LEVEL_WG), 0, st,
-                             tasks, r.wg_start, cnt_t, L, RING, r.upw, r.xcd);
-          HIPC(hipGetLastError());
-          timed_end(tl);
-        }
-        HIPC(hipMemcpyAsync(r.h_cnt, r.cnt, sizeof(u64) * RING * r.ntasks, hipMemcpyDeviceToHost, st));
-        HIPC(hipMemcpyAsync(h_err, d_err, sizeof(int), hipMemcpyDeviceToHost, st));
-      };
-      int launched = L1, examined = 0, running = cnt_t;
-      int batch_end = first == 0 ? be0 : std::min(n, std::max(2, guess_levels > 0 ? guess_levels : 8));
-      bool first_batch = true;
-      for (;;) {
-        if (first_batch && graphed) {
-          // prepare + level 0 + the first batch + the counter copies as one graph
-          const int be = batch_end;
-          if (!graph_ready(be))
-            capture([&]() { prepare(r.cnt, nzero, r.done, r.ntasks); level0(); levels(L1, be); }, be);
-          HIPC(hipGraphLaunch(gexec, st));
-        } else {
-          if (first_batch) level0();
-          levels(launched, batch_end);
-        }
-        first_batch = false;
-        launched = std::max(launched, batch_end);
-        HIPC(hipStreamSynchronize(st));
-        trace("level batch synchronized");
-        check_err();
-        for (int t = first; t < first + cnt_t; t++) {
-          St& q = sts[t];
-          if (!q.running) continue;
-          const u64 target = (u64)n * (u64)r.l[t];
-          for (int L = examined; L < launched; L++) {
-            const u64 c = r.h_cnt[(size_t)(L % RING) * r.ntasks + t];
-            if (c == target) { q.running = 0; q.lwork = L; max_d = std::max(max_d, L + 1); break; }
-            if ((L >= 1 && c == q.prev) || L + 1 >= n) { q.running = 0; q.lwork = L; any_disc = true; break; }
-            q.sum += (target - c) * (u64)s;
-            q.prev = c;
-          }
-          if (!q.running) running--;
-        }
-        examined = launched;
-        if (running == 0) break;
-        if (launched >= n) { any_disc = true; break; }
-        int step = std::min(MAX_BATCH_LEVELS, std::max(launched, 8));
-        batch_end = std::min(n, launched + step);
+    lv.K = r.direct;                      // levels 0..K-1 come from k_direct (0: k_level0 does level 0)
+    lv.L1 = lv.K ? lv.K : 1;              // first level the gather kernel runs
+    lv.nzero = (lv.K ? lv.K + 1 : 1) * r.ntasks;   // ring slots 0..L1 start at zero
+    lv.be0 = std::min(n, std::max(2, guess_levels > 0 ? guess_levels : 8));   // first batch
+    lv.graphed = r.group == r.ntasks && use_graph(lv.be0);
+    if (!lv.graphed) prepare(r.cnt, lv.nzero, r.done, r.ntasks);   // the first ring slots and the done flags
+    lv.sts.assign(r.ntasks, LvTask{1, 0, 0, 0});
+    lv.any_disc = false;
+    lv.max_d = 1; lv.lmax = 0;
+    lv.first = 0;
+    lv_group_start();
+  }
+
+  bool lv_step()
+  {
+    LevelRun& r = lr;
+    for (int t = lv.first; t < lv.first + lv.cnt_t; t++) {
+      LvTask& q = lv.sts[t];
+      if (!q.running) continue;
+      const u64 target = (u64)n * (u64)r.l[t];
+      for (int L = lv.examined; L < lv.launched; L++) {
+        const u64 c = r.h_cnt[(size_t)(L % RING) * r.ntasks + t];
+        if (c == target) { q.running = 0; q.lwork = L; lv.max_d = std::max(lv.max_d, L + 1); break; }
+        if ((L >= 1 && c == q.prev) || L + 1 >= n) { q.running = 0; q.lwork = L; lv.any_disc = true; break; }
+        q.sum += (target - c) * (u64)s;
+        q.prev = c;
       }
-      if (any_disc) break;          // the result is the sentinel whatever the other blocks give
-      for (int t = first; t < first + cnt_t; t++) lmax = std::max(lmax, sts[t].lwork);
-      guess_levels = lmax + 1;
-      if (timed()) flush_level_stats(sts.data(), first, cnt_t);
+      if (!q.running) lv.running--;
     }
+    lv.examined = lv.launched;
+    if (lv.running > 0 && lv.launched < n) {
+      const int step = std::min(MAX_BATCH_LEVELS, std::max(lv.launched, 8));
+      lv.batch_end = std::min(n, lv.launched + step);
+      lv_levels(lv.launched, lv.batch_end);
+      lv.launched = std::max(lv.launched, lv.batch_end);
+      return false;
+    }
+    if (lv.running > 0) lv.any_disc = true;
+    if (lv.any_disc) return true;          // the result is the sentinel whatever the other blocks give
+    for (int t = lv.first; t < lv.first + lv.cnt_t; t++) lv.lmax = std::max(lv.lmax, lv.sts[t].lwork);
+    guess_levels = lv.lmax + 1;
+    if (timed()) flush_level_stats(lv.sts.data(), lv.first, lv.cnt_t);
+    lv.first += r.group;
+    if (lv.first < r.ntasks) { lv_group_start(); return false; }
+    return true;
+  }
+
+  void lv_result(u64* excess, int* diam, bool* disc)
+  {
     u64 tot = 0;
-    for (int t = 0; t < r.ntasks; t++) tot += sts[t].sum;
-    *excess = tot; *diam = max_d; *disc = any_disc;
-    if (timed()) flush_level_stats(sts.data(), 0, 0);
+    for (const LvTask& q : lv.sts) tot += q.sum;
+    *excess = tot; *diam = lv.max_d; *disc = lv.any_disc;
+    if (timed()) flush_level_stats(lv.sts.data(), 0, 0);
+  }
+
+  void run_level(u64* excess, int* diam, bool* disc)
+  {
+    lv_begin();
+    for (;;) {
+      HIPC(hipStreamSynchronize(st));
+      trace("level batch synchronized");
+      check_err();
+      if (lv_step()) break;
+    }
+    lv_result(excess, diam, disc);
   }
 
   template <typename St>
@@ -692,6 +742,18 @@ struct Evaluator {
     }
   }
 
+  // the reference's final arithmetic (aspl.c:73-74): ordered-pair sum, ASPL, halved sum;
+  // a disconnected graph reports diameter n+1 (turned into the sentinels by the caller)
+  void result_of(u64 excess, int d, bool disc, int* diameter, long* sum, double* ASPL) const
+  {
+    long lsum = (long)n * (n - 1) + (long)excess;
+    if (disc) d = n + 1;
+    *diameter = d;
+    *ASPL = lsum / (((double)n - 1) * n);
+    lsum /= 2.0;
+    *sum = lsum;
+  }
+
   // full evaluation of the adjacency on the device, reference arithmetic
   void evaluate(int* diameter, long* sum, double* ASPL)
   {
@@ -701,12 +763,7 @@ struct Evaluator {
     bool disc = false;
     if (plan.plan == ODPX_PLAN_LEVEL) run_level(&excess, &d, &disc);
     else run_resident(&excess, &d, &disc);
-    long lsum = (long)n * (n - 1) + (long)excess;
-    if (disc) d = n + 1;
-    *diameter = d;
-    *ASPL = lsum / (((double)n - 1) * n);
-    lsum /= 2.0;
-    *sum = lsum;
+    result_of(excess, d, disc, diameter, sum, ASPL);
     if (timed()) { g_stats.evals++; g_stats.eval_ms += 1e3 * (wall_time() - t0); }
   }
 };

@@ -7,24 +7,29 @@
 // and small chains also keep their 16-bit neighbour table there.  A request
 // is a list of 2-opt swaps followed by an exact evaluation.  The requests of
 // one chain run one at a time in submission order (each is applied to the
-// graph the previous one left).  Worker threads, each with its own HIP stream,
-// take work as soon as they are free:
-//   * small workers pull every admissible small-chain request (up to
-//     SV_MAX_BATCH) into ONE persistent k_resident launch;
-//   * large workers own one set of level-plan bit buffers each (the HBM slot)
-//     and run one large-chain request at a time.
-// Requests that arrive while others are on the GPU are admitted the moment a
-// worker -- and with it its slot and memory -- frees up; launches on different
-// streams overlap on the device.  Results never depend on what else is in
-// flight: every request's integers come from its own blocks and counters.
-#include <thread>
+// graph the previous one left).
+//
+// The engine is a progress engine driven by its caller: every ODPX_Serve_*
+// call (submit, poll, wait, drain) first advances it -- collects the launches
+// whose completion event has passed, then admits waiting requests into free
+// slots -- so all HIP calls come from the calling thread, one call at a time
+// (a mutex serialises callers).  Slots own a stream each, so their work
+// overlaps on the device:
+//   * a small slot runs every admissible small-chain request (up to
+//     SV_MAX_BATCH) in ONE persistent k_resident launch;
+//   * a large slot owns one set of level-plan bit buffers (an Evaluator) and
+//     steps one large-chain request through its level batches.
+// Requests that arrive while others are on the GPU are admitted as soon as a
+// slot -- and with it its memory -- frees up.  Results never depend on what
+// else is in flight: every request's integers come from its own blocks and
+// counters.
 #include <mutex>
-#include <condition_variable>
 #include <deque>
+#include <memory>
 #include <unordered_map>
 #include <chrono>
 
-static const int SV_MAX_BATCH = 64;       // requests per small-worker launch
+static const int SV_MAX_BATCH = 64;       // requests per small-slot launch
 
 static double mono_time()
 {
@@ -76,7 +81,7 @@ struct SvChain {
   unsigned short* d_nbr16 = nullptr;
   size_t bytes = 0;
   std::deque<long long> queue;      // pending requests, FIFO
-  bool busy = false;
+  bool busy = false;                // its front request is in a slot
 };
 
 struct SvReq {
@@ -88,36 +93,12 @@ struct SvReq {
   double aspl = 0;
 };
 
-static struct Server {
-  bool running = false, stopping = false;
-  std::mutex mu;
-  std::condition_variable cv_work, cv_done;
-  std::vector<SvChain> chains;
-  std::unordered_map<long long, SvReq> reqs;   // submitted, not yet handed back
-  std::deque<long long> completed;             // done, not yet polled
-  long long next_id = 1;
-  long long outstanding = 0;                   // submitted and not completed
-  size_t budget = 0, used = 0;
-  int large_slots = 0;
-  std::vector<std::thread> workers;
-} S;
-
-static bool sv_chain_ready(const SvChain& c, bool small) { return c.small == small && !c.busy && !c.queue.empty(); }
-
-static void sv_complete(long long id, const SvReq& r)
-{
-  // caller holds S.mu
-  SvChain& c = S.chains[r.chain];
-  c.busy = false;
-  c.queue.pop_front();
-  S.completed.push_back(id);
-  S.outstanding--;
-}
-
-static void sv_small_worker()
-{
-  hipStream_t st;
-  HIPC(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+struct SvSmallSlot {
+  hipStream_t st = nullptr;
+  hipEvent_t done = nullptr;
+  bool busy = false;
+  std::vector<long long> ids;
+  std::vector<int> rn;
   std::vector<ResTask> tasks;
   std::vector<SwapJob> jobs;
   std::vector<int> swaps;
@@ -126,163 +107,230 @@ static void sv_small_worker()
   SwapJob* d_jobs = nullptr;
   int *d_swaps = nullptr, *d_status = nullptr, *h_status = nullptr, *d_qhead = nullptr;
   ResOut *d_out = nullptr, *h_out = nullptr;
-  dalloc(&d_jobs, SV_MAX_BATCH); dalloc(&d_status, SV_MAX_BATCH); halloc(&h_status, SV_MAX_BATCH);
-  dalloc(&d_out, SV_MAX_BATCH); halloc(&h_out, SV_MAX_BATCH); dalloc(&d_qhead, 1);
-  for (;;) {
-    std::vector<long long> ids;
-    {
-      std::unique_lock<std::mutex> lk(S.mu);
-      S.cv_work.wait(lk, [] {
-        if (S.stopping) return true;
-        for (const SvChain& c : S.chains) if (sv_chain_ready(c, true)) return true;
-        return false;
-      });
-      if (S.stopping) break;
-      double now = mono_time();
-      for (size_t i = 0; i < S.chains.size() && (int)ids.size() < SV_MAX_BATCH; i++) {
-        SvChain& c = S.chains[i];
-        if (!sv_chain_ready(c, true)) continue;
-        c.busy = true;
-        long long id = c.queue.front();
-        S.reqs[id].t_start = now;
-        ids.push_back(id);
-      }
-    }
-    // snapshot what the launch needs (the request map is only mutated under the lock)
-    jobs.clear(); swaps.clear(); tasks.clear();
-    std::vector<int> rn(ids.size());
-    size_t lds = 0;
-    {
-      std::lock_guard<std::mutex> lk(S.mu);
-      for (size_t k = 0; k < ids.size(); k++) {
-        const SvReq& r = S.reqs[ids[k]];
-        const SvChain& c = S.chains[r.chain];
-        SwapJob J;
-        J.adj = c.d_adj; J.nbr16 = c.d_nbr16; J.n = c.n; J.degree = c.degree; J.dp16 = c.dp16;
-        J.first = (int)(swaps.size() / 4); J.nswaps = (int)(r.swaps.size() / 4);
-        swaps.insert(swaps.end(), r.swaps.begin(), r.swaps.end());
-        jobs.push_back(J);
-        rn[k] = c.n;
-        const int b = 32 * c.wb;
-        lds = std::max(lds, (size_t)c.n * c.wb * 4);
-        for (int t = 0; t * b < c.n; t++) {
-          ResTask T;
-          T.nbr = c.d_nbr16; T.n = c.n; T.dp = c.dp16; T.src0 = t * b; T.l = std::min(b, c.n - t * b);
-          T.s = 1; T.graph = (int)k; T.wb = c.wb; T.pad = 0;
-          tasks.push_back(T);
-        }
-      }
-    }
-    if (tasks.size() > t_cap) { dfree(&d_t); dalloc(&d_t, tasks.size()); t_cap = tasks.size(); }
-    if (swaps.size() > s_cap) { dfree(&d_swaps); dalloc(&d_swaps, std::max<size_t>(swaps.size(), 4)); s_cap = swaps.size(); }
-    if (!swaps.empty())
-      HIPC(hipMemcpyAsync(d_swaps, swaps.data(), sizeof(int) * swaps.size(), hipMemcpyHostToDevice, st));
-    HIPC(hipMemcpyAsync(d_jobs, jobs.data(), sizeof(SwapJob) * jobs.size(), hipMemcpyHostToDevice, st));
-    HIPC(hipMemcpyAsync(d_t, tasks.data(), sizeof(ResTask) * tasks.size(), hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_apply_swaps, dim3(1), dim3(SV_MAX_BATCH), 0, st, d_jobs, (int)jobs.size(), d_swaps, d_status);
-    HIPC(hipGetLastError());
-    HIPC(hipMemsetAsync(d_out, 0, sizeof(ResOut) * ids.size(), st));
-    HIPC(hipMemsetAsync(d_qhead, 0, sizeof(int), st));
-    int grid = std::max(1, std::min((int)tasks.size(), C.ncu));
-    hipLaunchKernelGGL(k_resident, dim3(grid), dim3(1024), lds, st, d_t, (int)tasks.size(), d_qhead, d_out);
-    HIPC(hipGetLastError());
-    HIPC(hipMemcpyAsync(h_out, d_out, sizeof(ResOut) * ids.size(), hipMemcpyDeviceToHost, st));
-    HIPC(hipMemcpyAsync(h_status, d_status, sizeof(int) * ids.size(), hipMemcpyDeviceToHost, st));
-    HIPC(hipStreamSynchronize(st));
-    double now = mono_time();
-    {
-      std::lock_guard<std::mutex> lk(S.mu);
-      for (size_t k = 0; k < ids.size(); k++) {
-        SvReq& r = S.reqs[ids[k]];
-        const ResOut& o = h_out[k];
-        finish_one(rn[k], o.sum, std::max(1, o.diam), (o.flags & 1) != 0, &r.diameter, &r.sum, &r.aspl);
-        r.status = h_status[k] ? -1 : 0;
-        r.t_done = now;
-        sv_complete(ids[k], r);
-      }
-    }
-    S.cv_done.notify_all();
-    S.cv_work.notify_all();
-  }
-  dfree(&d_t); dfree(&d_jobs); dfree(&d_swaps); dfree(&d_status); hfree(&h_status);
-  dfree(&d_out); hfree(&h_out); dfree(&d_qhead);
-  HIPC(hipStreamDestroy(st));
-}
+};
 
-static void sv_large_worker()
-{
-  Evaluator ev;
-  HIPC(hipStreamCreateWithFlags(&ev.st, hipStreamNonBlocking));
-  SwapJob* d_job = nullptr;
+struct SvLargeSlot {
+  Evaluator ev;                     // its stream is ev.st
+  hipEvent_t done = nullptr;
+  bool busy = false;
+  long long id = 0;
+  int chain = -1;
+  int shaped_n = -1, shaped_d = -1;
+  SwapJob *d_job = nullptr, *h_job = nullptr;    // h_job: pinned, read by the async copy
   int *d_swaps = nullptr, *d_status = nullptr, *h_status = nullptr;
   size_t s_cap = 0;
-  dalloc(&d_job, 1); dalloc(&d_status, 1); halloc(&h_status, 1);
-  int shaped_n = -1, shaped_d = -1;
-  for (;;) {
-    long long id = 0;
-    SvReq r;
-    SvChain c;
-    {
-      std::unique_lock<std::mutex> lk(S.mu);
-      S.cv_work.wait(lk, [] {
-        if (S.stopping) return true;
-        for (const SvChain& x : S.chains) if (sv_chain_ready(x, false)) return true;
-        return false;
-      });
-      if (S.stopping) break;
-      for (size_t i = 0; i < S.chains.size(); i++) {
-        SvChain& x = S.chains[i];
-        if (!sv_chain_ready(x, false)) continue;
-        x.busy = true;
-        id = x.queue.front();
-        S.reqs[id].t_start = mono_time();
-        r = S.reqs[id];
-        c.n = x.n; c.degree = x.degree; c.d_adj = x.d_adj;
-        break;
-      }
+};
+
+static struct Server {
+  bool running = false;
+  std::mutex mu;                               // one caller at a time drives the engine
+  std::vector<SvChain> chains;
+  std::unordered_map<long long, SvReq> reqs;   // submitted, not yet handed back
+  std::deque<long long> completed;             // done, not yet polled
+  long long next_id = 1;
+  long long outstanding = 0;                   // submitted and not completed
+  size_t budget = 0, used = 0;
+  int nsmall = 0, nlarge = 0;                  // slots asked for at start
+  std::vector<std::unique_ptr<SvSmallSlot>> small;
+  std::vector<std::unique_ptr<SvLargeSlot>> large;
+} S;
+
+static bool sv_chain_ready(const SvChain& c, bool small) { return c.small == small && !c.busy && !c.queue.empty(); }
+
+static void sv_complete(long long id)
+{
+  SvChain& c = S.chains[S.reqs[id].chain];
+  c.busy = false;
+  c.queue.pop_front();
+  S.completed.push_back(id);
+  S.outstanding--;
+}
+
+// true once the work recorded on `e` has finished
+static bool sv_event_done(hipEvent_t e)
+{
+  const hipError_t q = hipEventQuery(e);
+  if (q == hipSuccess) return true;
+  if (q != hipErrorNotReady) HIPC(q);
+  return false;
+}
+
+// ---- small slots: one k_resident launch over a batch of small chains ----
+
+static void sv_small_launch(SvSmallSlot& sl)
+{
+  sl.ids.clear(); sl.rn.clear(); sl.jobs.clear(); sl.swaps.clear(); sl.tasks.clear();
+  const double now = mono_time();
+  size_t lds = 0;
+  for (size_t i = 0; i < S.chains.size() && (int)sl.ids.size() < SV_MAX_BATCH; i++) {
+    SvChain& c = S.chains[i];
+    if (!sv_chain_ready(c, true)) continue;
+    c.busy = true;
+    const long long id = c.queue.front();
+    SvReq& r = S.reqs[id];
+    r.t_start = now;
+    const int k = (int)sl.ids.size();
+    sl.ids.push_back(id);
+    SwapJob J;
+    J.adj = c.d_adj; J.nbr16 = c.d_nbr16; J.n = c.n; J.degree = c.degree; J.dp16 = c.dp16;
+    J.first = (int)(sl.swaps.size() / 4); J.nswaps = (int)(r.swaps.size() / 4);
+    sl.swaps.insert(sl.swaps.end(), r.swaps.begin(), r.swaps.end());
+    sl.jobs.push_back(J);
+    sl.rn.push_back(c.n);
+    const int b = 32 * c.wb;
+    lds = std::max(lds, (size_t)c.n * c.wb * 4);
+    for (int t = 0; t * b < c.n; t++) {
+      ResTask T;
+      T.nbr = c.d_nbr16; T.n = c.n; T.dp = c.dp16; T.src0 = t * b; T.l = std::min(b, c.n - t * b);
+      T.s = 1; T.graph = k; T.wb = c.wb; T.pad = 0;
+      sl.tasks.push_back(T);
     }
-    if (!r.swaps.empty()) {
-      if (r.swaps.size() > s_cap) { dfree(&d_swaps); dalloc(&d_swaps, r.swaps.size()); s_cap = r.swaps.size(); }
-      SwapJob J;
-      J.adj = c.d_adj; J.nbr16 = nullptr; J.n = c.n; J.degree = c.degree; J.dp16 = 0; J.first = 0;
-      J.nswaps = (int)(r.swaps.size() / 4);
-      HIPC(hipMemcpyAsync(d_swaps, r.swaps.data(), sizeof(int) * r.swaps.size(), hipMemcpyHostToDevice, ev.st));
-      HIPC(hipMemcpyAsync(d_job, &J, sizeof(SwapJob), hipMemcpyHostToDevice, ev.st));
-      hipLaunchKernelGGL(k_apply_swaps, dim3(1), dim3(64), 0, ev.st, d_job, 1, d_swaps, d_status);
-      HIPC(hipGetLastError());
-      HIPC(hipMemcpyAsync(h_status, d_status, sizeof(int), hipMemcpyDeviceToHost, ev.st));
-      HIPC(hipStreamSynchronize(ev.st));
-    } else {
-      *h_status = 0;
-    }
-    if (c.n != shaped_n || c.degree != shaped_d) {
-      ev.shape(0, c.n, c.degree, 1, 0, -1, nullptr, 0);
-      ODPX_Plan p{};
-      p.plan = ODPX_PLAN_LEVEL; p.kw = LEVEL_BLOCK_WORDS; p.upw = 8; p.seg_major = 0; p.xcd = 1; p.group = 0;
-      ev.apply_plan(p);
-      shaped_n = c.n; shaped_d = c.degree;
-    }
-    ev.ext_adj = c.d_adj;
-    int d; long sm; double as;
-    ev.evaluate(&d, &sm, &as);
-    if (d > c.n) { d = INT_MAX; sm = LONG_MAX; as = DBL_MAX; }
-    double now = mono_time();
-    {
-      std::lock_guard<std::mutex> lk(S.mu);
-      SvReq& q = S.reqs[id];
-      q.diameter = d; q.sum = sm; q.aspl = as; q.status = *h_status ? -1 : 0; q.t_done = now;
-      sv_complete(id, q);
-    }
-    S.cv_done.notify_all();
-    S.cv_work.notify_all();
   }
-  dfree(&d_job); dfree(&d_swaps); dfree(&d_status); hfree(&h_status);
-  hipStream_t st = ev.st;
-  ev.release();
+  if (sl.ids.empty()) return;
+  if (sl.tasks.size() > sl.t_cap) { dfree(&sl.d_t); dalloc(&sl.d_t, sl.tasks.size()); sl.t_cap = sl.tasks.size(); }
+  if (sl.swaps.size() > sl.s_cap) { dfree(&sl.d_swaps); dalloc(&sl.d_swaps, sl.swaps.size()); sl.s_cap = sl.swaps.size(); }
+  const int nb = (int)sl.ids.size();
+  if (!sl.swaps.empty())
+    HIPC(hipMemcpyAsync(sl.d_swaps, sl.swaps.data(), sizeof(int) * sl.swaps.size(), hipMemcpyHostToDevice, sl.st));
+  HIPC(hipMemcpyAsync(sl.d_jobs, sl.jobs.data(), sizeof(SwapJob) * nb, hipMemcpyHostToDevice, sl.st));
+  HIPC(hipMemcpyAsync(sl.d_t, sl.tasks.data(), sizeof(ResTask) * sl.tasks.size(), hipMemcpyHostToDevice, sl.st));
+  hipLaunchKernelGGL(k_apply_swaps, dim3(1), dim3(SV_MAX_BATCH), 0, sl.st, sl.d_jobs, nb, sl.d_swaps, sl.d_status);
+  HIPC(hipGetLastError());
+  HIPC(hipMemsetAsync(sl.d_out, 0, sizeof(ResOut) * nb, sl.st));
+  HIPC(hipMemsetAsync(sl.d_qhead, 0, sizeof(int), sl.st));
+  const int grid = std::max(1, std::min((int)sl.tasks.size(), C.ncu));
+  hipLaunchKernelGGL(k_resident, dim3(grid), dim3(1024), lds, sl.st, sl.d_t, (int)sl.tasks.size(), sl.d_qhead, sl.d_out);
+  HIPC(hipGetLastError());
+  HIPC(hipMemcpyAsync(sl.h_out, sl.d_out, sizeof(ResOut) * nb, hipMemcpyDeviceToHost, sl.st));
+  HIPC(hipMemcpyAsync(sl.h_status, sl.d_status, sizeof(int) * nb, hipMemcpyDeviceToHost, sl.st));
+  HIPC(hipEventRecord(sl.done, sl.st));
+  sl.busy = true;
+}
+
+static void sv_small_finish(SvSmallSlot& sl)
+{
+  const double now = mono_time();
+  for (size_t k = 0; k < sl.ids.size(); k++) {
+    SvReq& r = S.reqs[sl.ids[k]];
+    const ResOut& o = sl.h_out[k];
+    finish_one(sl.rn[k], o.sum, std::max(1, o.diam), (o.flags & 1) != 0, &r.diameter, &r.sum, &r.aspl);
+    r.status = sl.h_status[k] ? -1 : 0;
+    r.t_done = now;
+    sv_complete(sl.ids[k]);
+  }
+  sl.busy = false;
+}
+
+// ---- large slots: one level-plan evaluation stepped through its batches ----
+
+static void sv_large_launch(SvLargeSlot& sl, int ci)
+{
+  SvChain& c = S.chains[ci];
+  c.busy = true;
+  const long long id = c.queue.front();
+  SvReq& r = S.reqs[id];
+  r.t_start = mono_time();
+  sl.id = id; sl.chain = ci;
+  Evaluator& ev = sl.ev;
+  if (!r.swaps.empty()) {
+    if (r.swaps.size() > sl.s_cap) { dfree(&sl.d_swaps); dalloc(&sl.d_swaps, r.swaps.size()); sl.s_cap = r.swaps.size(); }
+    SwapJob& J = *sl.h_job;
+    J.adj = c.d_adj; J.nbr16 = nullptr; J.n = c.n; J.degree = c.degree; J.dp16 = 0; J.first = 0;
+    J.nswaps = (int)(r.swaps.size() / 4);
+    HIPC(hipMemcpyAsync(sl.d_swaps, r.swaps.data(), sizeof(int) * r.swaps.size(), hipMemcpyHostToDevice, ev.st));
+    HIPC(hipMemcpyAsync(sl.d_job, sl.h_job, sizeof(SwapJob), hipMemcpyHostToDevice, ev.st));
+    hipLaunchKernelGGL(k_apply_swaps, dim3(1), dim3(64), 0, ev.st, sl.d_job, 1, sl.d_swaps, sl.d_status);
+    HIPC(hipGetLastError());
+    HIPC(hipMemcpyAsync(sl.h_status, sl.d_status, sizeof(int), hipMemcpyDeviceToHost, ev.st));
+  } else {
+    *sl.h_status = 0;
+  }
+  if (c.n != sl.shaped_n || c.degree != sl.shaped_d) {
+    ev.shape(0, c.n, c.degree, 1, 0, -1, nullptr, 0);
+    ODPX_Plan p{};
+    p.plan = ODPX_PLAN_LEVEL; p.kw = LEVEL_BLOCK_WORDS; p.upw = 8; p.seg_major = 0; p.xcd = 1; p.group = 0;
+    p.direct = -1; p.batch = 0;
+    ev.apply_plan(p);
+    sl.shaped_n = c.n; sl.shaped_d = c.degree;
+  }
+  ev.ext_adj = c.d_adj;
+  ev.lv_begin();
+  HIPC(hipEventRecord(sl.done, ev.st));
+  sl.busy = true;
+}
+
+// the slot's recorded work has finished: the next batch, or the result
+static void sv_large_advance(SvLargeSlot& sl)
+{
+  Evaluator& ev = sl.ev;
+  if (*ev.h_err == 0 && !ev.lv_step()) {
+    HIPC(hipEventRecord(sl.done, ev.st));
+    return;
+  }
+  SvReq& r = S.reqs[sl.id];
+  if (*ev.h_err) {                 // cannot happen for a chain validated at registration
+    *ev.h_err = 0;
+    HIPC(hipMemsetAsync(ev.d_err, 0, sizeof(int), ev.st));
+    HIPC(hipStreamSynchronize(ev.st));
+    r.status = -2;
+  } else {
+    u64 excess; int d; bool disc;
+    ev.lv_result(&excess, &d, &disc);
+    long sm; double as;
+    ev.result_of(excess, d, disc, &d, &sm, &as);
+    if (d > ev.n) { d = INT_MAX; sm = LONG_MAX; as = DBL_MAX; }
+    r.diameter = d; r.sum = sm; r.aspl = as;
+    r.status = *sl.h_status ? -1 : 0;
+  }
+  r.t_done = mono_time();
+  sv_complete(sl.id);
+  sl.busy = false;
+}
+
+// collect finished work, then admit waiting requests into free slots
+static void sv_progress()
+{
+  for (auto& p : S.small)
+    if (p->busy && sv_event_done(p->done)) sv_small_finish(*p);
+  for (auto& p : S.large)
+    if (p->busy && sv_event_done(p->done)) sv_large_advance(*p);
+  for (auto& p : S.small)
+    if (!p->busy) sv_small_launch(*p);
+  for (auto& p : S.large) {
+    if (p->busy) continue;
+    for (size_t i = 0; i < S.chains.size(); i++)
+      if (sv_chain_ready(S.chains[i], false)) { sv_large_launch(*p, (int)i); break; }
+  }
+}
+
+// blocks until some slot's recorded work has finished; false if none is busy
+static bool sv_block()
+{
+  for (auto& p : S.small)
+    if (p->busy) { HIPC(hipEventSynchronize(p->done)); return true; }
+  for (auto& p : S.large)
+    if (p->busy) { HIPC(hipEventSynchronize(p->done)); return true; }
+  return false;
+}
+
+static void sv_small_free(SvSmallSlot& sl)
+{
+  dfree(&sl.d_t); dfree(&sl.d_jobs); dfree(&sl.d_swaps); dfree(&sl.d_status); hfree(&sl.h_status);
+  dfree(&sl.d_out); hfree(&sl.h_out); dfree(&sl.d_qhead);
+  HIPC(hipEventDestroy(sl.done));
+  HIPC(hipStreamDestroy(sl.st));
+}
+
+static void sv_large_free(SvLargeSlot& sl)
+{
+  dfree(&sl.d_job); hfree(&sl.h_job); dfree(&sl.d_swaps); dfree(&sl.d_status); hfree(&sl.h_status);
+  HIPC(hipEventDestroy(sl.done));
+  hipStream_t st = sl.ev.st;
+  sl.ev.release();
   HIPC(hipStreamDestroy(st));
 }
 
-// bytes of level-plan bit buffers one large worker holds for the largest chain size
+// bytes of level-plan bit buffers one large slot holds for a chain of n vertices
 static size_t sv_level_bytes(int n)
 {
   int W = (n + 63) / 64;
@@ -291,36 +339,58 @@ static size_t sv_level_bytes(int n)
   return blocks * 2 * (size_t)n * kw * 8 + (size_t)n * round_up(64, 4) * 4;
 }
 
-extern "C" int ODPX_Serve_start(int small_workers, int large_workers, double hbm_budget_gb)
+extern "C" int ODPX_Serve_start(int small_slots, int large_slots, double hbm_budget_gb)
 {
   ctx_init();
+  std::lock_guard<std::mutex> lk(S.mu);
   if (S.running) return -1;
-  S.stopping = false;
   S.budget = (size_t)(hbm_budget_gb * 1e9);
   S.used = 0;
   S.next_id = 1;
   S.outstanding = 0;
-  S.large_slots = std::max(0, large_workers);
-  for (int i = 0; i < std::max(1, small_workers); i++) S.workers.emplace_back(sv_small_worker);
+  S.nsmall = std::max(1, small_slots);
+  S.nlarge = std::max(0, large_slots);
+  for (int i = 0; i < S.nsmall; i++) {
+    std::unique_ptr<SvSmallSlot> p(new SvSmallSlot());
+    HIPC(hipStreamCreateWithFlags(&p->st, hipStreamNonBlocking));
+    HIPC(hipEventCreateWithFlags(&p->done, hipEventDisableTiming));
+    dalloc(&p->d_jobs, SV_MAX_BATCH); dalloc(&p->d_status, SV_MAX_BATCH); halloc(&p->h_status, SV_MAX_BATCH);
+    dalloc(&p->d_out, SV_MAX_BATCH); halloc(&p->h_out, SV_MAX_BATCH); dalloc(&p->d_qhead, 1);
+    S.small.push_back(std::move(p));
+  }
   S.running = true;
   return 0;
 }
 
+// the large slots, created with the first large chain
+static void sv_make_large_slots()
+{
+  for (int i = 0; i < S.nlarge; i++) {
+    std::unique_ptr<SvLargeSlot> p(new SvLargeSlot());
+    HIPC(hipStreamCreateWithFlags(&p->ev.st, hipStreamNonBlocking));
+    HIPC(hipEventCreateWithFlags(&p->done, hipEventDisableTiming));
+    dalloc(&p->d_job, 1); halloc(&p->h_job, 1); dalloc(&p->d_status, 1); halloc(&p->h_status, 1);
+    S.large.push_back(std::move(p));
+  }
+}
+
 extern "C" int ODPX_Serve_add_chain(int nodes, int degree, const int* adjacency)
 {
-  if (!S.running || nodes < 2 || degree < 1) return -1;
+  std::lock_guard<std::mutex> lk(S.mu);
+  if (!S.running || nodes < 2 || degree < 1 || !adjacency) return -1;
+  for (long i = 0; i < (long)nodes * degree; i++)
+    if (adjacency[i] < 0 || adjacency[i] >= nodes) return -1;     // every entry a vertex id
   SvChain c;
   c.n = nodes; c.degree = degree; c.dp16 = round_up(degree, 8);
   c.wb = batch_wb(nodes);
   c.small = c.wb > 0;
   c.bytes = (size_t)nodes * degree * 4 + (c.small ? (size_t)nodes * c.dp16 * 2 : 0);
   size_t extra = 0;
-  std::lock_guard<std::mutex> lk(S.mu);
   if (!c.small) {
-    // the first large chain of a size brings the large workers and their buffers
+    // the first large chain of a size brings the large slots' buffers
     bool have = false;
     for (const SvChain& x : S.chains) if (!x.small && x.n >= nodes) have = true;
-    if (!have && S.large_slots > 0) extra = (size_t)S.large_slots * sv_level_bytes(nodes);
+    if (!have && S.nlarge > 0) extra = (size_t)S.nlarge * sv_level_bytes(nodes);
   }
   if (S.used + c.bytes + extra > S.budget) return -1;      // admission by HBM footprint
   dalloc(&c.d_adj, (size_t)nodes * degree);
@@ -343,33 +413,25 @@ extern "C" int ODPX_Serve_add_chain(int nodes, int degree, const int* adjacency)
     if (h_err) { dfree(&c.d_adj); dfree(&c.d_nbr16); return -1; }
   }
   S.used += c.bytes + extra;
-  if (extra) {
-    int have_workers = 0;
-    for (const SvChain& x : S.chains) if (!x.small) have_workers = 1;
-    if (!have_workers)
-      for (int i = 0; i < S.large_slots; i++) S.workers.emplace_back(sv_large_worker);
-  }
+  if (!c.small && S.large.empty()) sv_make_large_slots();
   S.chains.push_back(c);
   return (int)S.chains.size() - 1;
 }
 
 extern "C" long long ODPX_Serve_submit(int chain, int nswaps, const int* swaps)
 {
-  long long id;
-  {
-    std::lock_guard<std::mutex> lk(S.mu);
-    if (!S.running || chain < 0 || chain >= (int)S.chains.size() || nswaps < 0) return -1;
-    if (!S.chains[chain].small && S.large_slots == 0) return -1;
-    id = S.next_id++;
-    SvReq r;
-    r.chain = chain;
-    r.swaps.assign(swaps, swaps + 4 * nswaps);
-    r.t_submit = mono_time();
-    S.reqs[id] = r;
-    S.chains[chain].queue.push_back(id);
-    S.outstanding++;
-  }
-  S.cv_work.notify_all();
+  std::lock_guard<std::mutex> lk(S.mu);
+  if (!S.running || chain < 0 || chain >= (int)S.chains.size() || nswaps < 0) return -1;
+  if (!S.chains[chain].small && S.nlarge == 0) return -1;
+  const long long id = S.next_id++;
+  SvReq r;
+  r.chain = chain;
+  if (nswaps) r.swaps.assign(swaps, swaps + 4 * nswaps);
+  r.t_submit = mono_time();
+  S.reqs[id] = r;
+  S.chains[chain].queue.push_back(id);
+  S.outstanding++;
+  sv_progress();
   return id;
 }
 
@@ -383,6 +445,8 @@ static void sv_fill(long long id, const SvReq& r, ODPX_Result* o)
 extern "C" int ODPX_Serve_poll(ODPX_Result* out, int max)
 {
   std::lock_guard<std::mutex> lk(S.mu);
+  if (!S.running) return 0;
+  sv_progress();
   int k = 0;
   while (k < max && !S.completed.empty()) {
     long long id = S.completed.front();
@@ -396,38 +460,45 @@ extern "C" int ODPX_Serve_poll(ODPX_Result* out, int max)
 
 extern "C" int ODPX_Serve_wait(long long id, ODPX_Result* out)
 {
-  std::unique_lock<std::mutex> lk(S.mu);
-  auto done = [&] {
-    for (long long c : S.completed) if (c == id) return true;
-    return false;
-  };
-  if (S.reqs.find(id) == S.reqs.end()) return -1;
-  S.cv_done.wait(lk, done);
-  for (auto it = S.completed.begin(); it != S.completed.end(); ++it)
-    if (*it == id) { S.completed.erase(it); break; }
-  auto it = S.reqs.find(id);
-  sv_fill(id, it->second, out);
-  S.reqs.erase(it);
-  return 0;
+  std::lock_guard<std::mutex> lk(S.mu);
+  if (!S.running || S.reqs.find(id) == S.reqs.end()) return -1;
+  for (;;) {
+    sv_progress();
+    for (auto it = S.completed.begin(); it != S.completed.end(); ++it)
+      if (*it == id) {
+        S.completed.erase(it);
+        auto r = S.reqs.find(id);
+        sv_fill(id, r->second, out);
+        S.reqs.erase(r);
+        return 0;
+      }
+    if (!sv_block()) return -1;      // nothing in flight: the request cannot advance
+  }
+}
+
+static void sv_drain_locked()
+{
+  for (;;) {
+    sv_progress();
+    if (S.outstanding == 0 || !sv_block()) return;
+  }
 }
 
 extern "C" void ODPX_Serve_drain(void)
 {
-  std::unique_lock<std::mutex> lk(S.mu);
-  S.cv_done.wait(lk, [] { return S.outstanding == 0; });
+  std::lock_guard<std::mutex> lk(S.mu);
+  if (S.running) sv_drain_locked();
 }
 
 extern "C" void ODPX_Serve_stop(void)
 {
+  std::lock_guard<std::mutex> lk(S.mu);
   if (!S.running) return;
-  ODPX_Serve_drain();
-  {
-    std::lock_guard<std::mutex> lk(S.mu);
-    S.stopping = true;
-  }
-  S.cv_work.notify_all();
-  for (std::thread& t : S.workers) t.join();
-  S.workers.clear();
+  sv_drain_locked();
+  for (auto& p : S.small) sv_small_free(*p);
+  for (auto& p : S.large) sv_large_free(*p);
+  S.small.clear();
+  S.large.clear();
   for (SvChain& c : S.chains) { dfree(&c.d_adj); dfree(&c.d_nbr16); }
   S.chains.clear();
   S.reqs.clear();
