@@ -21,11 +21,11 @@ the reference sources) timed on this host on the same graph.
 
 Extra objects (random65536, unless --no-extras): "batched" (1024 seeded 2-opt
 candidates with n in [1024, 16384] and d in {4, 8, 12, 16, 24}, resident in HBM,
-one persistent launch, against looping the single-graph GPU path) and, with
---served only, "served" (64 annealing chains through the serving engine:
-saturated rate, then Poisson arrivals with p50/p99 latency).  Sampled batched
-and served results are checked bit for bit against the reference in the
-cpu_baseline leg.
+one persistent launch, against looping the single-graph GPU path) and
+"served" (64 annealing chains through the serving engine: saturated rate,
+then Poisson arrivals with p50/p99 latency; --no-served skips it).  Sampled
+batched and served results are checked bit for bit against the reference in
+the cpu_baseline leg.
 """
 import argparse
 import json
@@ -260,9 +260,7 @@ def main():
                          "level:<kw>[:upw:seg_major:xcd:group:direct:batch] | resident:<wb>:<threads>")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
-    ap.add_argument("--served", action="store_true",
-                    help="also run the serving-engine workload (off by default: its mixed "
-                         "small/large worker run is not cleared on the GPU)")
+    ap.add_argument("--no-served", action="store_true", help="skip the serving-engine workload")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic.json"))
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first run one device-to-device copy of MIB MiB (a plain copy kernel of known "
@@ -367,7 +365,7 @@ def main():
     if not args.no_extras and args.workload == "random65536":
         bchecks, out["batched"] = batched_extra(1024, 3)
         checks = bchecks
-        if args.served:
+        if not args.no_served:
             out["served"], schecks = served_extra()
             checks = bchecks + schecks
     if not args.no_cpu:
