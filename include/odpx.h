@@ -114,6 +114,17 @@ int ODPX_Batch_eval(int ngraphs, const int *nodes, const int *degrees, const lon
 /* ODPX_Batch_load + ODPX_Loop_run. */
 int ODPX_Loop_eval(int ngraphs, const int *nodes, const int *degrees, const long long *offsets,
                    const int *packed_adjacency, int plan, int *diameter, long *sum, double *ASPL);
+/* Launch policy of ODPX_Batch_run: wbcap[5] = the widest LDS block (32-bit
+ * words per row: 1, 2, 4 or 8) for graphs with n <= 2048, 4096, 8192, 16384
+ * and larger; per_cu = resident workgroups per CU (1 or 2).  Set returns 0, or
+ * -1 for an invalid policy; it rebuilds the loaded batch's task table. */
+void ODPX_Batch_get_policy(int *wbcap, int *per_cu);
+int  ODPX_Batch_set_policy(const int *wbcap, int per_cu);
+/* Autotunes the policy on the loaded batch: times candidates (workgroups per
+ * CU, then the block width per size class, coordinate descent), checks that
+ * all return identical results (exit(1) otherwise) and keeps the fastest.
+ * Returns the number of candidates timed. */
+int  ODPX_Batch_autotune(int repeats);
 /* Frees the device buffers the batched and looped paths keep between calls. */
 void ODPX_Batch_release(void);
 

@@ -125,6 +125,9 @@ def load():
     _sig(lib, "ODPX_Loop_run", _I, [_I, _P, _P, _P])
     _sig(lib, "ODPX_Loop_eval", _I, [_I, _P, _P, _P, _P, _I, _P, _P, _P])
     _sig(lib, "ODPX_Batch_release", None, [])
+    _sig(lib, "ODPX_Batch_get_policy", None, [_P, _P])
+    _sig(lib, "ODPX_Batch_set_policy", _I, [_P, _I])
+    _sig(lib, "ODPX_Batch_autotune", _I, [_I])
     _sig(lib, "ODPX_Serve_start", _I, [_I, _I, _D])
     _sig(lib, "ODPX_Serve_add_chain", _I, [_I, _I, _P])
     _sig(lib, "ODPX_Serve_submit", ctypes.c_longlong, [_I, _I, _P])
@@ -296,6 +299,25 @@ def loop_run(ngraphs, plan=PLAN_LEVEL):
 
 def batch_release():
     load().ODPX_Batch_release()
+
+
+def batch_policy():
+    """ODPX_Batch_get_policy -> (widest block per size class [5], workgroups per CU)."""
+    caps = np.zeros(5, np.int32)
+    pc = np.zeros(1, np.int32)
+    load().ODPX_Batch_get_policy(_ptr(caps), _ptr(pc))
+    return [int(c) for c in caps], int(pc[0])
+
+
+def set_batch_policy(wbcap, per_cu=2):
+    caps = np.ascontiguousarray(wbcap, dtype=np.int32)
+    if caps.shape != (5,) or load().ODPX_Batch_set_policy(_ptr(caps), int(per_cu)) != 0:
+        raise ValueError(f"invalid batch policy {wbcap} {per_cu}")
+
+
+def batch_autotune(repeats=2):
+    """ODPX_Batch_autotune on the loaded batch; returns the number of candidates timed."""
+    return load().ODPX_Batch_autotune(int(repeats))
 
 
 # ---- seeded graphs ------------------------------------------------------------

@@ -34,6 +34,7 @@ static const int GPU_CHUNK_WORDS = 64;          // column-block width of MATRIX_
 static const int SAVING_GROUP = 32;             // MATRIX_SAVING blocks resident at once (4 per XCD)
 static const int LEVEL_BLOCK_WORDS = 16;        // default column-block width of the level plan
 static const int RES_WB_MAX = 8;                // widest resident block: 256 sources
+static const int BATCH_CLASSES = 5;             // size classes of the batch launch policy
 static const long DIRECT_TREE_MAX = 8192;       // walk-tree nodes per row the automatic direct K allows
 static const long DIRECT_TREE_CAP = 65536;      // ... and any requested K (larger K is lowered)
 static const double MEM_THRESHOLD_MB = 2048.0;  // MATRIX -> MATRIX_SAVING switch (parameter.h:4)
@@ -95,6 +96,14 @@ template <typename T> static void halloc(T** p, size_t count)
   if (count) HIPC(hipHostMalloc((void**)p, count * sizeof(T), hipHostMallocDefault));
 }
 template <typename T> static void hfree(T** p) { if (*p) HIPC(hipHostFree(*p)); *p = nullptr; }
+
+// hipGraph capture and replay of repeated launch sequences (env ODP_HIP_GRAPH=0 turns it off)
+static bool graphs_env()
+{
+  static int env = -1;
+  if (env < 0) { const char* v = getenv("ODP_HIP_GRAPH"); env = v ? atoi(v) != 0 : 1; }
+  return env != 0;
+}
 
 // ---------------------------------------------------------------------------
 // kernel timing: HIP events recorded on the launch stream around launches
@@ -191,12 +200,7 @@ struct Evaluator {
   // serving workers launch eagerly: a run in which two worker threads captured
   // graphs concurrently ended in a host segmentation fault whose cause was not
   // found, so that path is off.
-  bool graph_on() const
-  {
-    static int env = -1;
-    if (env < 0) { const char* v = getenv("ODP_HIP_GRAPH"); env = v ? atoi(v) != 0 : 1; }
-    return env && !timed() && st == C.stream;
-  }
+  bool graph_on() const { return graphs_env() && !timed() && st == C.stream; }
   const int* cur_adj() const { return ext_adj ? ext_adj : d_adj; }
   bool graph_ready(int key) const { return gexec && g_key == key && g_adj == cur_adj(); }
   // Replay a graph only for a launch sequence that repeats: the first evaluation
@@ -1134,7 +1138,20 @@ static struct {
   ResOut *d_out = nullptr, *h_out = nullptr;
   ResTask* d_t = nullptr;
   Evaluator single;          // graphs too large for the LDS path, and the looped path
+  // launch policy, set by ODPX_Batch_autotune: the widest block (32-bit words per
+  // row) per size class, and resident workgroups per CU
+  int wbcap[BATCH_CLASSES] = {RES_WB_MAX, RES_WB_MAX, RES_WB_MAX, RES_WB_MAX, RES_WB_MAX};
+  int per_cu = 2;
+  hipGraphExec_t gexec = nullptr;   // the resident launch sequence, captured once per load and policy
+  int runs = 0;                     // runs since the task table was built (capture from the second)
 } B;
+
+static void batch_drop_graph()
+{
+  if (B.gexec) HIPC(hipGraphExecDestroy(B.gexec));
+  B.gexec = nullptr;
+  B.runs = 0;
+}
 
 // widest block for a graph of n rows in a 1024-thread resident workgroup; 0 if none
 static int batch_wb(int n)
@@ -1143,6 +1160,12 @@ static int batch_wb(int n)
   for (int wb = RES_WB_MAX; wb >= 1; wb >>= 1)
     if ((size_t)n * wb * 4 <= LDS_DYN_MAX && n <= res_rpt(wb) * 1024) return wb;
   return 0;
+}
+
+// size class of a graph for the batch policy: n <= 2048, 4096, 8192, 16384, larger
+static int batch_class(int n)
+{
+  return n <= 2048 ? 0 : n <= 4096 ? 1 : n <= 8192 ? 2 : n <= 16384 ? 3 : 4;
 }
 
 static void finish_one(int n, u64 excess, int d, bool disc, int* diameter, long* sum, double* ASPL)
@@ -1172,9 +1195,44 @@ static void single_eval_dev(Evaluator& v, int n, int degree, const int* dev_adj,
   if (*diameter > n) { *diameter = INT_MAX; *sum = LONG_MAX; *ASPL = DBL_MAX; }
 }
 
+// the resident launch's task table under the current policy: per graph the
+// widest block its size class allows, longest blocks first (short queue tail)
+static void batch_build_tasks()
+{
+  batch_drop_graph();
+  std::vector<ResTask> tasks;
+  std::vector<double> cost;
+  B.lds = 0;
+  for (size_t k = 0; k < B.small.size(); k++) {
+    const BatchGraph& G = B.hg[k];
+    int wb = batch_wb(G.n);
+    while (wb > B.wbcap[batch_class(G.n)]) wb >>= 1;      // narrower blocks always fit
+    const int b = 32 * wb, nt = (G.n + b - 1) / b;
+    B.wb[k] = wb;
+    B.lds = std::max(B.lds, (size_t)G.n * wb * 4);
+    for (int t = 0; t < nt; t++) {
+      ResTask T;
+      T.nbr = B.d_nbr + G.nbr_off; T.n = G.n; T.dp = G.dp; T.src0 = t * b; T.l = std::min(b, G.n - t * b);
+      T.s = 1; T.graph = (int)k; T.wb = wb; T.pad = 0;
+      tasks.push_back(T);
+      cost.push_back((double)G.n * G.dp * (1.0 + log((double)G.n) / log(std::max(2.0, G.degree - 1.0))));
+    }
+  }
+  std::vector<int> ord(tasks.size());
+  for (size_t i = 0; i < ord.size(); i++) ord[i] = (int)i;
+  std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+  std::vector<ResTask> sorted(tasks.size());
+  for (size_t i = 0; i < ord.size(); i++) sorted[i] = tasks[ord[i]];
+  if (sorted.size() > B.t_cap) { dfree(&B.d_t); dalloc(&B.d_t, sorted.size()); B.t_cap = sorted.size(); }
+  B.ntasks = (int)sorted.size();
+  HIPC(hipMemcpyAsync(B.d_t, sorted.data(), sizeof(ResTask) * sorted.size(), hipMemcpyHostToDevice, C.stream));
+  HIPC(hipStreamSynchronize(C.stream));
+}
+
 extern "C" int ODPX_Batch_load(int ngraphs, const int* nodes, const int* degrees, const long long* offsets,
                                const int* packed)
 {
+  batch_drop_graph();
   ctx_init();
   B.ng = ngraphs;
   B.nodes.assign(nodes, nodes + ngraphs);
@@ -1196,8 +1254,6 @@ extern "C" int ODPX_Batch_load(int ngraphs, const int* nodes, const int* degrees
     long long nbr_tot = 0;
     B.hg.resize(B.small.size());
     B.wb.resize(B.small.size());
-    std::vector<ResTask> tasks;
-    std::vector<double> cost;
     for (size_t k = 0; k < B.small.size(); k++) {
       int g = B.small[k];
       BatchGraph& G = B.hg[k];
@@ -1211,29 +1267,8 @@ extern "C" int ODPX_Batch_load(int ngraphs, const int* nodes, const int* degrees
       dalloc(&B.d_g, B.small.size()); dalloc(&B.d_out, B.small.size()); halloc(&B.h_out, B.small.size());
       B.g_cap = B.small.size();
     }
-    for (size_t k = 0; k < B.small.size(); k++) {
-      const BatchGraph& G = B.hg[k];
-      int wb = batch_wb(G.n), b = 32 * wb, nt = (G.n + b - 1) / b;
-      B.wb[k] = wb;
-      B.lds = std::max(B.lds, (size_t)G.n * wb * 4);
-      for (int t = 0; t < nt; t++) {
-        ResTask T;
-        T.nbr = B.d_nbr + G.nbr_off; T.n = G.n; T.dp = G.dp; T.src0 = t * b; T.l = std::min(b, G.n - t * b);
-        T.s = 1; T.graph = (int)k; T.wb = wb; T.pad = 0;
-        tasks.push_back(T);
-        cost.push_back((double)G.n * G.dp * (1.0 + log((double)G.n) / log(std::max(2.0, G.degree - 1.0))));
-      }
-    }
-    // longest blocks first, so the queue's tail is short
-    std::vector<int> ord(tasks.size());
-    for (size_t i = 0; i < ord.size(); i++) ord[i] = (int)i;
-    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return cost[a] > cost[b]; });
-    std::vector<ResTask> sorted(tasks.size());
-    for (size_t i = 0; i < ord.size(); i++) sorted[i] = tasks[ord[i]];
-    if (sorted.size() > B.t_cap) { dfree(&B.d_t); dalloc(&B.d_t, sorted.size()); B.t_cap = sorted.size(); }
-    B.ntasks = (int)sorted.size();
     HIPC(hipMemcpyAsync(B.d_g, B.hg.data(), sizeof(BatchGraph) * B.hg.size(), hipMemcpyHostToDevice, C.stream));
-    HIPC(hipMemcpyAsync(B.d_t, sorted.data(), sizeof(ResTask) * sorted.size(), hipMemcpyHostToDevice, C.stream));
+    batch_build_tasks();
     int pgrid = (int)std::min<long long>(std::max<long long>(1, (nbr_tot + 255) / 256), 8192);
     hipLaunchKernelGGL(k_prepare_batch, dim3(pgrid), dim3(256), 0, C.stream, B.d_g, (int)B.hg.size(), nbr_tot,
                        B.d_adj, B.d_nbr, B.d_err);
@@ -1245,22 +1280,52 @@ extern "C" int ODPX_Batch_load(int ngraphs, const int* nodes, const int* degrees
   return 0;
 }
 
+// zero the per-graph accumulators and the queue head (a kernel, so the
+// sequence can live in a hipGraph)
+__global__ void k_clear(u64* a, int na, int* q)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < na) a[i] = 0;
+  if (i == 0) *q = 0;
+}
+
+// clear, the persistent resident launch over every small graph, results to the host
+static void batch_enqueue()
+{
+  const int na = (int)(B.small.size() * sizeof(ResOut) / 8);
+  hipLaunchKernelGGL(k_clear, dim3((na + 255) / 256), dim3(256), 0, C.stream, (u64*)B.d_out, na, B.d_qhead);
+  HIPC(hipGetLastError());
+  int per_cu = (int)std::max<size_t>(1, LDS_LIMIT / (B.lds + LDS_STATIC));
+  per_cu = std::min(per_cu, B.per_cu);
+  const int grid = std::max(1, std::min(B.ntasks, per_cu * C.ncu));
+  TimedLaunch tr;
+  timed_begin(tr, T_RESIDENT, 0);
+  hipLaunchKernelGGL(k_resident, dim3(grid), dim3(1024), B.lds, C.stream, B.d_t, B.ntasks, B.d_qhead, B.d_out);
+  HIPC(hipGetLastError());
+  timed_end(tr);
+  HIPC(hipMemcpyAsync(B.h_out, B.d_out, sizeof(ResOut) * B.small.size(), hipMemcpyDeviceToHost, C.stream));
+}
+
 extern "C" int ODPX_Batch_run(int* diameter, long* sum, double* ASPL)
 {
   ctx_init();
   double t0 = wall_time();
   if (!B.small.empty()) {
-    HIPC(hipMemsetAsync(B.d_out, 0, sizeof(ResOut) * B.small.size(), C.stream));
-    HIPC(hipMemsetAsync(B.d_qhead, 0, sizeof(int), C.stream));
-    int per_cu = (int)std::max<size_t>(1, LDS_LIMIT / (B.lds + LDS_STATIC));
-    per_cu = std::min(per_cu, 2);
-    int grid = std::max(1, std::min(B.ntasks, per_cu * C.ncu));
-    TimedLaunch tr;
-    timed_begin(tr, T_RESIDENT, 0);
-    hipLaunchKernelGGL(k_resident, dim3(grid), dim3(1024), B.lds, C.stream, B.d_t, B.ntasks, B.d_qhead, B.d_out);
-    HIPC(hipGetLastError());
-    timed_end(tr);
-    HIPC(hipMemcpyAsync(B.h_out, B.d_out, sizeof(ResOut) * B.small.size(), hipMemcpyDeviceToHost, C.stream));
+    if (graphs_env() && !g_stats_on && B.runs++ > 0) {
+      // replayed graph of the launch sequence, captured on the second run after a
+      // load or policy change (a batch run once is launched eagerly)
+      if (!B.gexec) {
+        hipGraph_t g;
+        HIPC(hipStreamBeginCapture(C.stream, hipStreamCaptureModeThreadLocal));
+        batch_enqueue();
+        HIPC(hipStreamEndCapture(C.stream, &g));
+        HIPC(hipGraphInstantiate(&B.gexec, g, nullptr, nullptr, 0));
+        HIPC(hipGraphDestroy(g));
+      }
+      HIPC(hipGraphLaunch(B.gexec, C.stream));
+    } else {
+      batch_enqueue();
+    }
     HIPC(hipStreamSynchronize(C.stream));
     for (size_t k = 0; k < B.small.size(); k++) {
       int g = B.small[k];
@@ -1312,8 +1377,81 @@ extern "C" int ODPX_Loop_eval(int ngraphs, const int* nodes, const int* degrees,
   return ODPX_Loop_run(plan, diameter, sum, ASPL);
 }
 
+extern "C" void ODPX_Batch_get_policy(int* wbcap, int* per_cu)
+{
+  for (int c = 0; c < BATCH_CLASSES; c++) wbcap[c] = B.wbcap[c];
+  *per_cu = B.per_cu;
+}
+
+extern "C" int ODPX_Batch_set_policy(const int* wbcap, int per_cu)
+{
+  for (int c = 0; c < BATCH_CLASSES; c++)
+    if (wbcap[c] != 1 && wbcap[c] != 2 && wbcap[c] != 4 && wbcap[c] != 8) return -1;
+  if (per_cu != 1 && per_cu != 2) return -1;
+  for (int c = 0; c < BATCH_CLASSES; c++) B.wbcap[c] = wbcap[c];
+  B.per_cu = per_cu;
+  if (!B.small.empty()) batch_build_tasks();
+  return 0;
+}
+
+// Times the loaded batch's resident launch under candidate policies --
+// workgroups per CU, then per size class the widest block -- keeping each
+// change that makes it faster (coordinate descent from the current policy).
+// Every candidate must return identical results (exit(1) otherwise).
+extern "C" int ODPX_Batch_autotune(int repeats)
+{
+  ctx_init();
+  if (B.small.empty()) return 0;
+  if (repeats < 1) repeats = 1;
+  const int ng = B.ng;
+  std::vector<int> d0(ng), d1(ng);
+  std::vector<long> s0(ng), s1(ng);
+  std::vector<double> a0(ng), a1(ng);
+  const std::vector<int> large = B.large;
+  B.large.clear();                          // time the resident launch alone
+  auto timed_run = [&](std::vector<int>& d, std::vector<long>& sm, std::vector<double>& as) {
+    ODPX_Batch_run(d.data(), sm.data(), as.data());      // warm-up (and the capture)
+    double t = wall_time();
+    for (int i = 0; i < repeats; i++) ODPX_Batch_run(d.data(), sm.data(), as.data());
+    return (wall_time() - t) / repeats;
+  };
+  double best = timed_run(d0, s0, a0);
+  int tried = 1;
+  auto consider = [&]() {
+    batch_build_tasks();
+    const double t = timed_run(d1, s1, a1);
+    tried++;
+    for (int k = 0; k < ng; k++)
+      if (d1[k] != d0[k] || s1[k] != s0[k] || memcmp(&a1[k], &a0[k], sizeof(double)) != 0)
+        ODP_ERROR("libodp_hip: batch policies disagree on graph %d\n", k);
+    if (t < best) { best = t; return true; }
+    return false;
+  };
+  for (int pc : {1, 2}) {
+    if (pc == B.per_cu) continue;
+    const int old = B.per_cu;
+    B.per_cu = pc;
+    if (!consider()) B.per_cu = old;
+  }
+  bool present[BATCH_CLASSES] = {false, false, false, false, false};
+  for (int g : B.small) present[batch_class(B.nodes[g])] = true;
+  for (int c = 0; c < BATCH_CLASSES; c++) {
+    if (!present[c]) continue;
+    for (int cap = RES_WB_MAX; cap >= 1; cap >>= 1) {
+      if (cap == B.wbcap[c]) continue;
+      const int old = B.wbcap[c];
+      B.wbcap[c] = cap;
+      if (!consider()) B.wbcap[c] = old;
+    }
+  }
+  batch_build_tasks();
+  B.large = large;
+  return tried;
+}
+
 extern "C" void ODPX_Batch_release(void)
 {
+  batch_drop_graph();
   dfree(&B.d_adj); dfree(&B.d_nbr); dfree(&B.d_err); dfree(&B.d_qhead); hfree(&B.h_err);
   dfree(&B.d_g); dfree(&B.d_out); hfree(&B.h_out); dfree(&B.d_t);
   B.adj_cap = B.nbr_cap = B.g_cap = B.t_cap = 0;

@@ -287,6 +287,44 @@ def test_batch_vs_oracle():
         assert one[0][0] == d[g] and one[1][0] == s[g] and one[2][0] == a[g]
 
 
+def test_batch_policy_autotune_and_graph_replay():
+    """Every batch launch policy (block width per size class, workgroups per CU)
+    gives the oracle's results; the autotuner keeps one of them; the captured
+    launch replays identically with eager launches and host work in between."""
+    graphs = _batch_graphs(3, 24, nmax=12000)
+    graphs.append(two_triangles())
+    packed = odp.pack(graphs)
+    want = [oracle.evaluate(g, nodes=g.shape[0]) for g in graphs]
+    ng = odp.batch_load(packed)
+    try:
+        for caps, pc in (([8] * 5, 2), ([1] * 5, 1), ([2, 4, 1, 2, 8], 2), ([4, 1, 8, 1, 2], 1)):
+            odp.set_batch_policy(caps, pc)
+            assert odp.batch_policy() == (caps, pc)
+            d, sm, a = odp.batch_run(ng)
+            for g in range(ng):
+                assert same((int(d[g]), int(sm[g]), float(a[g])), want[g]), (caps, pc, g)
+        with pytest.raises(ValueError):
+            odp.set_batch_policy([3] * 5, 2)
+        tried = odp.batch_autotune(1)
+        assert tried >= 2
+        caps, pc = odp.batch_policy()
+        assert all(c in (1, 2, 4, 8) for c in caps) and pc in (1, 2)
+        ref = odp.batch_run(ng)
+        for _ in range(4):                     # replays, with eager work and syncs between them
+            odp.stats_enable(True)
+            eager = odp.batch_run(ng)
+            odp.stats_enable(False)
+            rep = odp.batch_run(ng)
+            for x, y, z in zip(ref, eager, rep):
+                assert (x.view(np.int64) == y.view(np.int64)).all() and (x.view(np.int64) == z.view(np.int64)).all()
+        for g in range(ng):
+            assert same((int(ref[0][g]), int(ref[1][g]), float(ref[2][g])), want[g])
+    finally:
+        odp.stats_enable(False)
+        odp.set_batch_policy([8] * 5, 2)
+        odp.batch_release()
+
+
 def test_batch_large_graph_falls_back():
     graphs = [odp.generate_regular(24000, 4, 3), odp.generate_regular(500, 6, 4)]
     d, s, a = odp.batch_eval(odp.pack(graphs))
