@@ -144,11 +144,15 @@ static void timed_end(TimedLaunch& t)
 struct LevelRun {
   int n = 0, ntasks = 0, group = 0, kw = 0, kwp = 0, upw = 4, seg_major = 1, xcd = 0, urow = 0, per_task_wg = 0;
   int direct = 0, dnw = 4, dgrid = 0;   // levels done by k_direct (0: k_level0), its waves and grid
+  int segs = 1;                         // 2 x lanes-per-row word segments per block row
   int batch = 4;                        // neighbour rows k_level gathers per batch
   size_t dlds = 0;
   size_t buf_words = 0;                 // capacity of bufs (64-bit words)
   size_t task_cap = 0;
   u64* bufs = nullptr; u64* cnt = nullptr; u32* done = nullptr;
+  unsigned char* flags = nullptr;       // changed flags: per block of the group 2 x segs x n bytes
+  u64* h_acnt = nullptr;                // processed-unit counts on the host (kernel timing only)
+  size_t flag_bytes = 0;
   LevelTask* tasks = nullptr; int* wg_start = nullptr;
   u64* h_cnt = nullptr;
   std::vector<int> l;                   // sources per task
@@ -236,6 +240,7 @@ struct Evaluator {
     nbr16_cap = 0;
     hfree(&h_stage); hfree(&h_err);
     dfree(&lr.bufs); dfree(&lr.cnt); dfree(&lr.done); dfree(&lr.tasks); dfree(&lr.wg_start); hfree(&lr.h_cnt);
+    dfree(&lr.flags); hfree(&lr.h_acnt);
     dfree(&rr.tasks); dfree(&rr.out); dfree(&rr.qhead); hfree(&rr.h_out);
     lr = LevelRun(); rr = ResRun();
     adj_cap = nbr_cap = stage_cap = 0;
@@ -405,15 +410,18 @@ struct Evaluator {
     // bit buffers for `group` column blocks; group g of blocks reuses them
     size_t words = (size_t)group * 2 * n * kwp;
     if (words > r.buf_words) { dfree(&r.bufs); dalloc(&r.bufs, words); r.buf_words = words; }
+    const size_t fbytes = (size_t)group * 2 * segs * n;
+    if (fbytes > r.flag_bytes) { dfree(&r.flags); dalloc(&r.flags, fbytes); r.flag_bytes = fbytes; }
     if ((size_t)ntasks > r.task_cap) {
       dfree(&r.cnt); dfree(&r.done); dfree(&r.tasks); dfree(&r.wg_start); hfree(&r.h_cnt);
-      dalloc(&r.cnt, (size_t)RING * ntasks); dalloc(&r.done, (size_t)ntasks);
+      dalloc(&r.cnt, (size_t)3 * RING * ntasks); dalloc(&r.done, (size_t)ntasks);   // counts, changed, processed
+      hfree(&r.h_acnt); halloc(&r.h_acnt, (size_t)RING * ntasks);
       dalloc(&r.tasks, (size_t)ntasks); dalloc(&r.wg_start, (size_t)ntasks + 1);
       halloc(&r.h_cnt, (size_t)RING * ntasks);
       r.task_cap = ntasks;
     }
     // the previous contents are never read: level 0 rewrites every word it uses
-    r.n = n; r.ntasks = ntasks; r.group = group; r.kw = kw; r.kwp = kwp;
+    r.n = n; r.ntasks = ntasks; r.group = group; r.kw = kw; r.kwp = kwp; r.segs = segs;
     r.upw = plan.upw; r.seg_major = plan.seg_major; r.xcd = plan.xcd; r.urow = (lpr_log2 == 6);
     r.batch = plan.batch == 8 || plan.batch == 16 ? plan.batch : 4;
     r.per_task_wg = (units + 4 * r.upw - 1) / (4 * r.upw);
@@ -444,6 +452,10 @@ struct Evaluator {
       T.l = std::min(kw * 64, nsrc - T.src0);
       T.lpr_log2 = lpr_log2; T.segs = segs; T.rgroups = rgroups; T.units = units;
       T.seg_major = r.seg_major; T.cstride = ntasks; T.pad = 0;
+      T.ccnt = r.cnt + (size_t)RING * ntasks + t;
+      T.acnt = r.cnt + (size_t)2 * RING * ntasks + t;
+      T.chg0 = r.flags + (size_t)(t % group) * 2 * segs * n;
+      T.chg1 = T.chg0 + (size_t)segs * n;
       r.l[t] = T.l;
       ws[t] = t * r.per_task_wg;     // blocks are equal: the starts of a group rebase to 0
     }
@@ -580,12 +592,15 @@ struct Evaluator {
       auto kern = r.urow ? (r.batch == 16 ? k_level<true, 16> : r.batch == 8 ? k_level<true, 8> : k_level<true, 4>)
                          : (r.batch == 16 ? k_level<false, 16> : r.batch == 8 ? k_level<false, 8> : k_level<false, 4>);
       hipLaunchKernelGGL(kern, dim3(grid), dim3(LEVEL_WG), 0, st,
-                         r.tasks + lv.first, r.wg_start, lv.cnt_t, L, RING, r.upw, r.xcd);
+                         r.tasks + lv.first, r.wg_start, lv.cnt_t, L, RING, r.upw, r.xcd, lv.L1);
       HIPC(hipGetLastError());
       timed_end(tl);
     }
     HIPC(hipMemcpyAsync(r.h_cnt, r.cnt, sizeof(u64) * RING * r.ntasks, hipMemcpyDeviceToHost, st));
     HIPC(hipMemcpyAsync(h_err, d_err, sizeof(int), hipMemcpyDeviceToHost, st));
+    if (timed())
+      HIPC(hipMemcpyAsync(r.h_acnt, r.cnt + (size_t)2 * RING * r.ntasks, sizeof(u64) * RING * r.ntasks,
+                          hipMemcpyDeviceToHost, st));
   }
 
   // the first levels and the first batch of group lv.first
@@ -614,7 +629,7 @@ struct Evaluator {
     trace("run_level");
     lv.K = r.direct;                      // levels 0..K-1 come from k_direct (0: k_level0 does level 0)
     lv.L1 = lv.K ? lv.K : 1;              // first level the gather kernel runs
-    lv.nzero = (lv.K ? lv.K + 1 : 1) * r.ntasks;   // ring slots 0..L1 start at zero
+    lv.nzero = 3 * RING * r.ntasks;       // the rings (counts, changed, processed) start at zero
     lv.be0 = std::min(n, std::max(2, guess_levels > 0 ? guess_levels : 8));   // first batch
     lv.graphed = r.group == r.ntasks && use_graph(lv.be0);
     if (!lv.graphed) prepare(r.cnt, lv.nzero, r.done, r.ntasks);   // the first ring slots and the done flags
@@ -690,9 +705,13 @@ struct Evaluator {
       g_evpool.push_back(tl.a); g_evpool.push_back(tl.b);
       if (tl.kind == T_LEVEL) {
         double bytes = 0;
+        // a level processes a share of the (row, segment) units: all of them, or
+        // in frontier mode the active ones; bytes are that share of the full level's
         for (int t = first; t < first + cnt_t; t++)
-          if (tl.level <= stv[t].lwork)
-            bytes += (double)n * ((avg_deg + 2.0) * ((r.l[t] + 63) / 64) * 8.0 + 4.0 * avg_deg);
+          if (tl.level <= stv[t].lwork) {
+            const double share = (double)r.h_acnt[(size_t)(tl.level % RING) * r.ntasks + t] / ((double)n * r.segs);
+            bytes += share * (double)n * ((avg_deg + 2.0) * ((r.l[t] + 63) / 64) * 8.0 + 4.0 * avg_deg);
+          }
         if (bytes > 0) { g_stats.level_launches++; g_stats.level_ms += ms; g_stats.level_bytes += bytes; }
         else { g_stats.idle_launches++; g_stats.idle_ms += ms; }
       } else {

@@ -168,8 +168,12 @@ struct LevelTask {
   int rgroups;         // ceil(n / rows per wave)
   int units;           // rgroups * segs wave-units
   int seg_major;       // unit order: segment-major (1) or row-major (0)
-  int cstride;         // level stride of cnt
+  int cstride;         // level stride of cnt and ccnt
   int pad;
+  u64* ccnt;           // per-level count of changed (row, segment) units: ccnt[(L % ring) * cstride]
+  u64* acnt;           // per-level count of processed (row, segment) units, same layout
+  unsigned char* chg0; // changed flag of (segment s, row r) at [s * n + r]: levels of even parity
+  unsigned char* chg1; //   ... and of odd parity
 };
 
 __device__ __forceinline__ int find_task(const int* __restrict__ wg_start, int ntasks, int g)
@@ -182,7 +186,7 @@ __device__ __forceinline__ int find_task(const int* __restrict__ wg_start, int n
   return lo;
 }
 
-struct UnitGeom { int r, w; };
+struct UnitGeom { int r, w, seg; };
 
 __device__ __forceinline__ UnitGeom unit_geom(const LevelTask& T, int u, int lane)
 {
@@ -193,6 +197,7 @@ __device__ __forceinline__ UnitGeom unit_geom(const LevelTask& T, int u, int lan
   UnitGeom g;
   g.r = (rg << (6 - T.lpr_log2)) + (lane >> T.lpr_log2);
   g.w = seg * (2 * lpr) + 2 * (lane & (lpr - 1));
+  g.seg = seg;
   return g;
 }
 
@@ -249,10 +254,19 @@ __global__ __launch_bounds__(LEVEL_WG) void k_level0(const LevelTask* __restrict
 // One gather level L >= 1 of every task.  UROW: one row per wave, so the row's
 // neighbour ids are wave-uniform (scalar loads).  G: neighbour rows gathered
 // per batch (in flight together) before the wave checks for saturation.
+//
+// Every level records, per (row, segment), whether the segment changed and
+// how many did.  When fewer than half changed at level L-1 (and L-1 was not
+// the first gathered level, which wrote every row), level L runs in frontier
+// mode: a segment whose row and neighbours all stayed unchanged at L-1 cannot
+// change at L, and the buffer level L writes already holds its value (the
+// level L-2 value, equal to L-1's), so it is skipped -- no reads, no write.
+// The count of level L is the count of L-1 plus each processed segment's
+// gain, which equals the popcount of the whole level.
 template <bool UROW, int G>
 __global__ __launch_bounds__(LEVEL_WG) void k_level(const LevelTask* __restrict__ tasks,
                                                     const int* __restrict__ wg_start, int ntasks,
-                                                    int level, int ring, int upw, int xcd)
+                                                    int level, int ring, int upw, int xcd, int first_level)
 {
   int g = blockIdx.x;
   if (xcd) g = remap_xcd(g, gridDim.x);
@@ -260,8 +274,8 @@ __global__ __launch_bounds__(LEVEL_WG) void k_level(const LevelTask* __restrict_
   const LevelTask T = tasks[t];
   if (*T.done) return;
   const u64 target = (u64)T.n * (u64)T.l;
+  const u64 c1 = T.cnt[(int64_t)((level - 1) % ring) * T.cstride];
   {
-    const u64 c1 = T.cnt[(int64_t)((level - 1) % ring) * T.cstride];
     const bool stop = (c1 == target) || (level >= 2 && c1 == T.cnt[(int64_t)((level - 2) % ring) * T.cstride]);
     if (stop) {
       if (threadIdx.x == 0) *T.done = 1;
@@ -269,13 +283,24 @@ __global__ __launch_bounds__(LEVEL_WG) void k_level(const LevelTask* __restrict_
     }
   }
   const int wg = g - wg_start[t];
-  if (wg == 0 && threadIdx.x == 0) T.cnt[(int64_t)((level + 1) % ring) * T.cstride] = 0;
+  if (wg == 0 && threadIdx.x == 0) {
+    T.cnt[(int64_t)((level + 1) % ring) * T.cstride] = 0;
+    T.ccnt[(int64_t)((level + 1) % ring) * T.cstride] = 0;
+    T.acnt[(int64_t)((level + 1) % ring) * T.cstride] = 0;
+    atomicAdd(&T.cnt[(int64_t)(level % ring) * T.cstride], c1);      // the gains below add to it
+  }
+  const bool fr = level > first_level &&
+                  2 * T.ccnt[(int64_t)((level - 1) % ring) * T.cstride] < (u64)T.n * (u64)T.segs;
+  const unsigned char* __restrict__ chr = (level & 1) ? T.chg0 : T.chg1;   // written by level L-1
+  unsigned char* __restrict__ chw = (level & 1) ? T.chg1 : T.chg0;
   const u64* __restrict__ cur = (level & 1) ? T.buf1 : T.buf0;
   u64* __restrict__ nxt = (level & 1) ? T.buf0 : T.buf1;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   const int lane = threadIdx.x & 63;
+  const int lpr = 1 << T.lpr_log2;
+  const u64 gmask = lpr == 64 ? ~0ULL : ((1ULL << lpr) - 1);
   const int64_t kwp = T.kwp;
-  u64 pop = 0;
+  u64 pop = 0, nch = 0, nact = 0;
   const int base = wg * 4 * upw;
   for (int i = 0; i < upw; i++) {
     const int u = base + i * 4 + wave;
@@ -283,42 +308,68 @@ __global__ __launch_bounds__(LEVEL_WG) void k_level(const LevelTask* __restrict_
     UnitGeom q = unit_geom(T, u, lane);
     if (UROW) q.r = __builtin_amdgcn_readfirstlane(q.r);
     if (q.r < T.n) {
-      const u64 f0 = full_word(T.l, q.w), f1 = full_word(T.l, q.w + 1);
-      u64x2 x = *(const u64x2*)(cur + q.r * kwp + q.w);
-      // Once a lane's words are saturated the remaining neighbours cannot add
-      // a bit (OR is monotone): the lane stops gathering (its loads are masked
-      // off, so they cost no memory requests) and the wave stops once every
-      // lane has.
-      bool need = !(x.x == f0 && x.y == f1);
-      if (__any(need)) {
-        const int* nb = T.nbr + (int64_t)q.r * T.dpad;
-        for (int j = 0; j < T.dpad; j += G) {
-          int id[G];
-#pragma unroll
-          for (int k = 0; k < G; k += 4) {
-            int4 v;
-            if (need && (k == 0 || j + k < T.dpad)) v = *(const int4*)(nb + j + k);
-            else { v.x = q.r; v.y = q.r; v.z = q.r; v.w = q.r; }
-            id[k] = v.x; id[k + 1] = v.y; id[k + 2] = v.z; id[k + 3] = v.w;
-          }
-          u64x2 g[G];
-#pragma unroll
-          for (int k = 0; k < G; k++) {
-            if (need) g[k] = *(const u64x2*)(cur + (int64_t)id[k] * kwp + q.w);
-            else { g[k].x = 0; g[k].y = 0; }
-          }
-#pragma unroll
-          for (int k = 0; k < G; k++) { x.x |= g[k].x; x.y |= g[k].y; }
-          need = !(x.x == f0 && x.y == f1);
-          if (!__any(need)) break;
+      const int64_t so = (int64_t)q.seg * T.n;
+      const int* nb = T.nbr + (int64_t)q.r * T.dpad;
+      bool act = true;
+      if (fr) {
+        act = chr[so + q.r] != 0;
+        for (int j = 0; j < T.dpad && !act; j += 4) {
+          const int4 v = *(const int4*)(nb + j);
+          act = (chr[so + v.x] | chr[so + v.y] | chr[so + v.z] | chr[so + v.w]) != 0;
         }
       }
-      *(u64x2*)(nxt + q.r * kwp + q.w) = x;
-      pop += (u64)(__popcll(x.x) + __popcll(x.y));
+      bool changed = false;
+      if (act) {
+        const u64 f0 = full_word(T.l, q.w), f1 = full_word(T.l, q.w + 1);
+        const u64x2 x0 = *(const u64x2*)(cur + q.r * kwp + q.w);
+        u64x2 x = x0;
+        // Once a lane's words are saturated the remaining neighbours cannot add
+        // a bit (OR is monotone): the lane stops gathering (its loads are masked
+        // off, so they cost no memory requests) and the wave stops once every
+        // lane has.
+        bool need = !(x.x == f0 && x.y == f1);
+        if (__any(need)) {
+          for (int j = 0; j < T.dpad; j += G) {
+            int id[G];
+#pragma unroll
+            for (int k = 0; k < G; k += 4) {
+              int4 v;
+              if (need && (k == 0 || j + k < T.dpad)) v = *(const int4*)(nb + j + k);
+              else { v.x = q.r; v.y = q.r; v.z = q.r; v.w = q.r; }
+              id[k] = v.x; id[k + 1] = v.y; id[k + 2] = v.z; id[k + 3] = v.w;
+            }
+            u64x2 gv[G];
+#pragma unroll
+            for (int k = 0; k < G; k++) {
+              if (need) gv[k] = *(const u64x2*)(cur + (int64_t)id[k] * kwp + q.w);
+              else { gv[k].x = 0; gv[k].y = 0; }
+            }
+#pragma unroll
+            for (int k = 0; k < G; k++) { x.x |= gv[k].x; x.y |= gv[k].y; }
+            need = !(x.x == f0 && x.y == f1);
+            if (!__any(need)) break;
+          }
+        }
+        *(u64x2*)(nxt + q.r * kwp + q.w) = x;
+        changed = x.x != x0.x || x.y != x0.y;
+        pop += (u64)(__popcll(x.x) + __popcll(x.y) - __popcll(x0.x) - __popcll(x0.y));
+      }
+      // the segment changed if any of its lanes did
+      const u64 m = __ballot(changed);
+      const bool any = ((m >> (lane & ~(lpr - 1))) & gmask) != 0;
+      if ((lane & (lpr - 1)) == 0) {
+        chw[so + q.r] = any ? 1 : 0;
+        nch += any ? 1 : 0;
+        nact += act ? 1 : 0;
+      }
     }
   }
   pop = wave_sum(pop);
+  nch = wave_sum(nch);
+  nact = wave_sum(nact);
   if (lane == 0 && pop) atomicAdd(&T.cnt[(int64_t)(level % ring) * T.cstride], pop);
+  if (lane == 0 && nch) atomicAdd(&T.ccnt[(int64_t)(level % ring) * T.cstride], nch);
+  if (lane == 0 && nact) atomicAdd(&T.acnt[(int64_t)(level % ring) * T.cstride], nact);
 }
 
 // ---------------------------------------------------------------------------
