@@ -35,6 +35,7 @@ static const int SAVING_GROUP = 32;             // MATRIX_SAVING blocks resident
 static const int LEVEL_BLOCK_WORDS = 16;        // default column-block width of the level plan
 static const int RES_WB_MAX = 8;                // widest resident block: 256 sources
 static const int BATCH_CLASSES = 5;             // size classes of the batch launch policy
+static const int FRONTIER_MIN_LEVELS = 16;      // frontier-mode levels when the last evaluation needed more
 static const long DIRECT_TREE_MAX = 8192;       // walk-tree nodes per row the automatic direct K allows
 static const long DIRECT_TREE_CAP = 65536;      // ... and any requested K (larger K is lowered)
 static const double MEM_THRESHOLD_MB = 2048.0;  // MATRIX -> MATRIX_SAVING switch (parameter.h:4)
@@ -543,6 +544,7 @@ struct Evaluator {
   struct LvTask { int running; u64 sum, prev; int lwork; };
   struct LvState {
     int K = 0, L1 = 1, nzero = 0, be0 = 2;
+    bool fr = false;                    // frontier-mode level kernel (graphs with many levels)
     bool graphed = false;
     int first = 0, cnt_t = 0;
     int launched = 0, examined = 0, running = 0, batch_end = 0;
@@ -589,8 +591,11 @@ struct Evaluator {
     for (int L = from; L < to; L++) {
       TimedLaunch tl;
       timed_begin(tl, T_LEVEL, L, st);
-      auto kern = r.urow ? (r.batch == 16 ? k_level<true, 16> : r.batch == 8 ? k_level<true, 8> : k_level<true, 4>)
-                         : (r.batch == 16 ? k_level<false, 16> : r.batch == 8 ? k_level<false, 8> : k_level<false, 4>);
+      auto kern = lv.fr
+          ? (r.urow ? (r.batch == 16 ? k_level<true, 16, true> : r.batch == 8 ? k_level<true, 8, true> : k_level<true, 4, true>)
+                    : (r.batch == 16 ? k_level<false, 16, true> : r.batch == 8 ? k_level<false, 8, true> : k_level<false, 4, true>))
+          : (r.urow ? (r.batch == 16 ? k_level<true, 16, false> : r.batch == 8 ? k_level<true, 8, false> : k_level<true, 4, false>)
+                    : (r.batch == 16 ? k_level<false, 16, false> : r.batch == 8 ? k_level<false, 8, false> : k_level<false, 4, false>));
       hipLaunchKernelGGL(kern, dim3(grid), dim3(LEVEL_WG), 0, st,
                          r.tasks + lv.first, r.wg_start, lv.cnt_t, L, RING, r.upw, r.xcd, lv.L1);
       HIPC(hipGetLastError());
@@ -612,9 +617,9 @@ struct Evaluator {
     lv.batch_end = lv.first == 0 ? lv.be0 : std::min(n, std::max(2, guess_levels > 0 ? guess_levels : 8));
     if (lv.first == 0 && lv.graphed) {
       // prepare + the first levels + the first batch + the counter copies as one graph
-      const int be = lv.batch_end, L1 = lv.L1, nz = lv.nzero;
-      if (!graph_ready(be))
-        capture([&]() { prepare(r.cnt, nz, r.done, r.ntasks); lv_level0(); lv_levels(L1, be); }, be);
+      const int be = lv.batch_end, L1 = lv.L1, nz = lv.nzero, key = 2 * be + (lv.fr ? 1 : 0);
+      if (!graph_ready(key))
+        capture([&]() { prepare(r.cnt, nz, r.done, r.ntasks); lv_level0(); lv_levels(L1, be); }, key);
       HIPC(hipGraphLaunch(gexec, st));
     } else {
       lv_level0();
@@ -631,7 +636,10 @@ struct Evaluator {
     lv.L1 = lv.K ? lv.K : 1;              // first level the gather kernel runs
     lv.nzero = 3 * RING * r.ntasks;       // the rings (counts, changed, processed) start at zero
     lv.be0 = std::min(n, std::max(2, guess_levels > 0 ? guess_levels : 8));   // first batch
-    lv.graphed = r.group == r.ntasks && use_graph(lv.be0);
+    // frontier mode pays off over long level sequences; with few levels every
+    // level is dense and the plain kernel is faster
+    lv.fr = guess_levels == 0 || guess_levels > FRONTIER_MIN_LEVELS;
+    lv.graphed = r.group == r.ntasks && use_graph(2 * lv.be0 + (lv.fr ? 1 : 0));
     if (!lv.graphed) prepare(r.cnt, lv.nzero, r.done, r.ntasks);   // the first ring slots and the done flags
     lv.sts.assign(r.ntasks, LvTask{1, 0, 0, 0});
     lv.any_disc = false;
@@ -709,7 +717,8 @@ struct Evaluator {
         // in frontier mode the active ones; bytes are that share of the full level's
         for (int t = first; t < first + cnt_t; t++)
           if (tl.level <= stv[t].lwork) {
-            const double share = (double)r.h_acnt[(size_t)(tl.level % RING) * r.ntasks + t] / ((double)n * r.segs);
+            const double share = lv.fr ? (double)r.h_acnt[(size_t)(tl.level % RING) * r.ntasks + t] / ((double)n * r.segs)
+                                       : 1.0;
             bytes += share * (double)n * ((avg_deg + 2.0) * ((r.l[t] + 63) / 64) * 8.0 + 4.0 * avg_deg);
           }
         if (bytes > 0) { g_stats.level_launches++; g_stats.level_ms += ms; g_stats.level_bytes += bytes; }

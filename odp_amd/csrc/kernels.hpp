@@ -255,15 +255,15 @@ __global__ __launch_bounds__(LEVEL_WG) void k_level0(const LevelTask* __restrict
 // neighbour ids are wave-uniform (scalar loads).  G: neighbour rows gathered
 // per batch (in flight together) before the wave checks for saturation.
 //
-// Every level records, per (row, segment), whether the segment changed and
-// how many did.  When fewer than half changed at level L-1 (and L-1 was not
+// FR (chosen per evaluation, for graphs with many levels): every level records,
+// per (row, segment), whether the segment changed and how many did.  When fewer than half changed at level L-1 (and L-1 was not
 // the first gathered level, which wrote every row), level L runs in frontier
 // mode: a segment whose row and neighbours all stayed unchanged at L-1 cannot
 // change at L, and the buffer level L writes already holds its value (the
 // level L-2 value, equal to L-1's), so it is skipped -- no reads, no write.
 // The count of level L is the count of L-1 plus each processed segment's
 // gain, which equals the popcount of the whole level.
-template <bool UROW, int G>
+template <bool UROW, int G, bool FR>
 __global__ __launch_bounds__(LEVEL_WG) void k_level(const LevelTask* __restrict__ tasks,
                                                     const int* __restrict__ wg_start, int ntasks,
                                                     int level, int ring, int upw, int xcd, int first_level)
@@ -285,11 +285,13 @@ __global__ __launch_bounds__(LEVEL_WG) void k_level(const LevelTask* __restrict_
   const int wg = g - wg_start[t];
   if (wg == 0 && threadIdx.x == 0) {
     T.cnt[(int64_t)((level + 1) % ring) * T.cstride] = 0;
-    T.ccnt[(int64_t)((level + 1) % ring) * T.cstride] = 0;
-    T.acnt[(int64_t)((level + 1) % ring) * T.cstride] = 0;
-    atomicAdd(&T.cnt[(int64_t)(level % ring) * T.cstride], c1);      // the gains below add to it
+    if (FR) {
+      T.ccnt[(int64_t)((level + 1) % ring) * T.cstride] = 0;
+      T.acnt[(int64_t)((level + 1) % ring) * T.cstride] = 0;
+      atomicAdd(&T.cnt[(int64_t)(level % ring) * T.cstride], c1);    // the gains below add to it
+    }
   }
-  const bool fr = level > first_level &&
+  const bool fr = FR && level > first_level &&
                   2 * T.ccnt[(int64_t)((level - 1) % ring) * T.cstride] < (u64)T.n * (u64)T.segs;
   const unsigned char* __restrict__ chr = (level & 1) ? T.chg0 : T.chg1;   // written by level L-1
   unsigned char* __restrict__ chw = (level & 1) ? T.chg1 : T.chg0;
@@ -351,25 +353,33 @@ __global__ __launch_bounds__(LEVEL_WG) void k_level(const LevelTask* __restrict_
           }
         }
         *(u64x2*)(nxt + q.r * kwp + q.w) = x;
-        changed = x.x != x0.x || x.y != x0.y;
-        pop += (u64)(__popcll(x.x) + __popcll(x.y) - __popcll(x0.x) - __popcll(x0.y));
+        if (FR) {
+          changed = x.x != x0.x || x.y != x0.y;
+          pop += (u64)(__popcll(x.x) + __popcll(x.y) - __popcll(x0.x) - __popcll(x0.y));
+        } else {
+          pop += (u64)(__popcll(x.x) + __popcll(x.y));
+        }
       }
-      // the segment changed if any of its lanes did
-      const u64 m = __ballot(changed);
-      const bool any = ((m >> (lane & ~(lpr - 1))) & gmask) != 0;
-      if ((lane & (lpr - 1)) == 0) {
-        chw[so + q.r] = any ? 1 : 0;
-        nch += any ? 1 : 0;
-        nact += act ? 1 : 0;
+      if (FR) {
+        // the segment changed if any of its lanes did
+        const u64 m = __ballot(changed);
+        const bool any = ((m >> (lane & ~(lpr - 1))) & gmask) != 0;
+        if ((lane & (lpr - 1)) == 0) {
+          chw[so + q.r] = any ? 1 : 0;
+          nch += any ? 1 : 0;
+          nact += act ? 1 : 0;
+        }
       }
     }
   }
   pop = wave_sum(pop);
-  nch = wave_sum(nch);
-  nact = wave_sum(nact);
   if (lane == 0 && pop) atomicAdd(&T.cnt[(int64_t)(level % ring) * T.cstride], pop);
-  if (lane == 0 && nch) atomicAdd(&T.ccnt[(int64_t)(level % ring) * T.cstride], nch);
-  if (lane == 0 && nact) atomicAdd(&T.acnt[(int64_t)(level % ring) * T.cstride], nact);
+  if (FR) {
+    nch = wave_sum(nch);
+    nact = wave_sum(nact);
+    if (lane == 0 && nch) atomicAdd(&T.ccnt[(int64_t)(level % ring) * T.cstride], nch);
+    if (lane == 0 && nact) atomicAdd(&T.acnt[(int64_t)(level % ring) * T.cstride], nact);
+  }
 }
 
 // ---------------------------------------------------------------------------

@@ -287,6 +287,28 @@ def test_batch_vs_oracle():
         assert one[0][0] == d[g] and one[1][0] == s[g] and one[2][0] == a[g]
 
 
+@pytest.mark.parametrize("kw", [1, 4, 16])
+def test_frontier_and_dense_levels_agree(kw):
+    """The first evaluation after Init runs the frontier-mode level kernel; later
+    ones run it only for graphs that needed more than 16 levels, the plain one
+    otherwise.  Every evaluation equals the oracle: small and large diameters,
+    a non-regular table, a disconnected graph."""
+    cases = [(odp.generate_regular(3000, 5, 21), None), (cycle_graph(600), None)]
+    pa, nd = path_graph(257)
+    cases.append((pa, nd))
+    cases.append((np.concatenate([cycle_graph(300), cycle_graph(200) + 300]).astype(np.int32), None))
+    for adj, nd in cases:
+        n = adj.shape[0]
+        want = oracle.evaluate(adj, nodes=n, num_degrees=nd) if nd is not None else oracle.evaluate(adj, nodes=n)
+        odp.init_aspl_general(n, adj.shape[1], nd)
+        try:
+            odp.set_plan(plan=LEV, kw=kw, upw=4, seg_major=0, xcd=1, group=0, direct=0)
+            for _ in range(3):
+                assert same(odp.set_aspl(adj), want), (n, kw)
+        finally:
+            odp.finalize_aspl()
+
+
 def test_batch_policy_autotune_and_graph_replay():
     """Every batch launch policy (block width per size class, workgroups per CU)
     gives the oracle's results; the autotuner keeps one of them; the captured
