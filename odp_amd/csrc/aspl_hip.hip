@@ -614,7 +614,7 @@ struct Evaluator {
     LevelRun& r = lr;
     lv.cnt_t = std::min(r.group, r.ntasks - lv.first);
     lv.launched = lv.L1; lv.examined = 0; lv.running = lv.cnt_t;
-    lv.batch_end = lv.first == 0 ? lv.be0 : std::min(n, std::max(2, guess_levels > 0 ? guess_levels : 8));
+    lv.batch_end = lv.first == 0 ? lv.be0 : first_batch_end();
     if (lv.first == 0 && lv.graphed) {
       // prepare + the first levels + the first batch + the counter copies as one graph
       const int be = lv.batch_end, L1 = lv.L1, nz = lv.nzero, key = 2 * be + (lv.fr ? 1 : 0);
@@ -628,6 +628,14 @@ struct Evaluator {
     lv.launched = std::max(lv.launched, lv.batch_end);
   }
 
+  // end of a group's first batch of levels: the last evaluation's level count
+  // (8 without one), at most MAX_BATCH_LEVELS so that its counter-ring slots
+  // are distinct (the ring holds RING >= MAX_BATCH_LEVELS levels)
+  int first_batch_end() const
+  {
+    return std::min(n, std::max(2, std::min(MAX_BATCH_LEVELS, guess_levels > 0 ? guess_levels : 8)));
+  }
+
   void lv_begin()
   {
     LevelRun& r = lr;
@@ -635,7 +643,7 @@ struct Evaluator {
     lv.K = r.direct;                      // levels 0..K-1 come from k_direct (0: k_level0 does level 0)
     lv.L1 = lv.K ? lv.K : 1;              // first level the gather kernel runs
     lv.nzero = 3 * RING * r.ntasks;       // the rings (counts, changed, processed) start at zero
-    lv.be0 = std::min(n, std::max(2, guess_levels > 0 ? guess_levels : 8));   // first batch
+    lv.be0 = first_batch_end();
     // frontier mode pays off over long level sequences; with few levels every
     // level is dense and the plain kernel is faster
     lv.fr = guess_levels == 0 || guess_levels > FRONTIER_MIN_LEVELS;
