@@ -545,6 +545,7 @@ struct Evaluator {
   struct LvState {
     int K = 0, L1 = 1, nzero = 0, be0 = 2;
     bool fr = false;                    // frontier-mode level kernel (graphs with many levels)
+    int spec = -1;                      // level of the current group launched count-only (expected last)
     bool graphed = false;
     int first = 0, cnt_t = 0;
     int launched = 0, examined = 0, running = 0, batch_end = 0;
@@ -586,9 +587,15 @@ struct Evaluator {
   // gather levels [from, to) of the current group, then the counter and error copies
   void lv_levels(int from, int to)
   {
+    for (int L = from; L < to; L++) lv_launch(L, L == lv.spec ? LV_COUNT : LV_STORE | LV_COUNT);
+    lv_copies();
+  }
+
+  void lv_launch(int L, int mode)
+  {
     LevelRun& r = lr;
     const int grid = lv.cnt_t * r.per_task_wg;
-    for (int L = from; L < to; L++) {
+    {
       TimedLaunch tl;
       timed_begin(tl, T_LEVEL, L, st);
       auto kern = lv.fr
@@ -597,10 +604,15 @@ struct Evaluator {
           : (r.urow ? (r.batch == 16 ? k_level<true, 16, false> : r.batch == 8 ? k_level<true, 8, false> : k_level<true, 4, false>)
                     : (r.batch == 16 ? k_level<false, 16, false> : r.batch == 8 ? k_level<false, 8, false> : k_level<false, 4, false>));
       hipLaunchKernelGGL(kern, dim3(grid), dim3(LEVEL_WG), 0, st,
-                         r.tasks + lv.first, r.wg_start, lv.cnt_t, L, RING, r.upw, r.xcd, lv.L1);
+                         r.tasks + lv.first, r.wg_start, lv.cnt_t, L, RING, r.upw, r.xcd, lv.L1, mode);
       HIPC(hipGetLastError());
       timed_end(tl);
     }
+  }
+
+  void lv_copies()
+  {
+    LevelRun& r = lr;
     HIPC(hipMemcpyAsync(r.h_cnt, r.cnt, sizeof(u64) * RING * r.ntasks, hipMemcpyDeviceToHost, st));
     HIPC(hipMemcpyAsync(h_err, d_err, sizeof(int), hipMemcpyDeviceToHost, st));
     if (timed())
@@ -615,9 +627,13 @@ struct Evaluator {
     lv.cnt_t = std::min(r.group, r.ntasks - lv.first);
     lv.launched = lv.L1; lv.examined = 0; lv.running = lv.cnt_t;
     lv.batch_end = lv.first == 0 ? lv.be0 : first_batch_end();
+    // the last evaluation ended at level batch_end - 1: expect the same, and
+    // launch that level count-only (timed passes keep every level whole)
+    lv.spec = guess_levels > 0 && lv.batch_end == guess_levels && lv.batch_end - 1 >= lv.L1 && !timed()
+                  ? lv.batch_end - 1 : -1;
     if (lv.first == 0 && lv.graphed) {
       // prepare + the first levels + the first batch + the counter copies as one graph
-      const int be = lv.batch_end, L1 = lv.L1, nz = lv.nzero, key = 2 * be + (lv.fr ? 1 : 0);
+      const int be = lv.batch_end, L1 = lv.L1, nz = lv.nzero, key = lv_key(be);
       if (!graph_ready(key))
         capture([&]() { prepare(r.cnt, nz, r.done, r.ntasks); lv_level0(); lv_levels(L1, be); }, key);
       HIPC(hipGraphLaunch(gexec, st));
@@ -636,6 +652,14 @@ struct Evaluator {
     return std::min(n, std::max(2, std::min(MAX_BATCH_LEVELS, guess_levels > 0 ? guess_levels : 8)));
   }
 
+  // graph key of a first batch ending at be: its length, the kernel variant and
+  // whether its last level is count-only
+  int lv_key(int be) const
+  {
+    const bool spec = guess_levels > 0 && be == guess_levels && be - 1 >= lv.L1 && !timed();
+    return 4 * be + (lv.fr ? 1 : 0) + (spec ? 2 : 0);
+  }
+
   void lv_begin()
   {
     LevelRun& r = lr;
@@ -647,7 +671,7 @@ struct Evaluator {
     // frontier mode pays off over long level sequences; with few levels every
     // level is dense and the plain kernel is faster
     lv.fr = guess_levels == 0 || guess_levels > FRONTIER_MIN_LEVELS;
-    lv.graphed = r.group == r.ntasks && use_graph(2 * lv.be0 + (lv.fr ? 1 : 0));
+    lv.graphed = r.group == r.ntasks && use_graph(lv_key(lv.be0));
     if (!lv.graphed) prepare(r.cnt, lv.nzero, r.done, r.ntasks);   // the first ring slots and the done flags
     lv.sts.assign(r.ntasks, LvTask{1, 0, 0, 0});
     lv.any_disc = false;
@@ -673,6 +697,11 @@ struct Evaluator {
       if (!q.running) lv.running--;
     }
     lv.examined = lv.launched;
+    if (lv.running > 0 && lv.spec >= 0) {
+      // the count-only level was not the last: write its rows now (its counts stand)
+      lv_launch(lv.spec, LV_STORE);
+      lv.spec = -1;
+    }
     if (lv.running > 0 && lv.launched < n) {
       const int step = std::min(MAX_BATCH_LEVELS, std::max(lv.launched, 8));
       lv.batch_end = std::min(n, lv.launched + step);

@@ -36,6 +36,8 @@ struct __attribute__((aligned(16))) u64x2 { u64 x, y; };
 
 #define ODP_NXCD 8          // XCDs of an MI355X; workgroups are dealt round-robin
 #define LEVEL_WG 256        // threads per level-kernel workgroup (4 waves)
+#define LV_STORE 1          // k_level mode bits: write the next rows
+#define LV_COUNT 2          //   ... and add the level's counts (both: a normal level)
 
 // ---------------------------------------------------------------------------
 // shared helpers
@@ -263,10 +265,16 @@ __global__ __launch_bounds__(LEVEL_WG) void k_level0(const LevelTask* __restrict
 // level L-2 value, equal to L-1's), so it is skipped -- no reads, no write.
 // The count of level L is the count of L-1 plus each processed segment's
 // gain, which equals the popcount of the whole level.
+//
+// mode: LV_STORE | LV_COUNT for a normal level; LV_COUNT alone for a level
+// expected to be the last one (its rows are never read again, so they are not
+// written); LV_STORE alone to write the rows of such a level after all when
+// it turned out not to be the last (its counts are already in place).
 template <bool UROW, int G, bool FR>
 __global__ __launch_bounds__(LEVEL_WG) void k_level(const LevelTask* __restrict__ tasks,
                                                     const int* __restrict__ wg_start, int ntasks,
-                                                    int level, int ring, int upw, int xcd, int first_level)
+                                                    int level, int ring, int upw, int xcd, int first_level,
+                                                    int mode)
 {
   int g = blockIdx.x;
   if (xcd) g = remap_xcd(g, gridDim.x);
@@ -283,7 +291,8 @@ __global__ __launch_bounds__(LEVEL_WG) void k_level(const LevelTask* __restrict_
     }
   }
   const int wg = g - wg_start[t];
-  if (wg == 0 && threadIdx.x == 0) {
+  const bool count = (mode & LV_COUNT) != 0, store = (mode & LV_STORE) != 0;
+  if (count && wg == 0 && threadIdx.x == 0) {
     T.cnt[(int64_t)((level + 1) % ring) * T.cstride] = 0;
     if (FR) {
       T.ccnt[(int64_t)((level + 1) % ring) * T.cstride] = 0;
@@ -352,7 +361,7 @@ __global__ __launch_bounds__(LEVEL_WG) void k_level(const LevelTask* __restrict_
             if (!__any(need)) break;
           }
         }
-        *(u64x2*)(nxt + q.r * kwp + q.w) = x;
+        if (store) *(u64x2*)(nxt + q.r * kwp + q.w) = x;
         if (FR) {
           changed = x.x != x0.x || x.y != x0.y;
           pop += (u64)(__popcll(x.x) + __popcll(x.y) - __popcll(x0.x) - __popcll(x0.y));
@@ -360,7 +369,7 @@ __global__ __launch_bounds__(LEVEL_WG) void k_level(const LevelTask* __restrict_
           pop += (u64)(__popcll(x.x) + __popcll(x.y));
         }
       }
-      if (FR) {
+      if (FR && count) {
         // the segment changed if any of its lanes did
         const u64 m = __ballot(changed);
         const bool any = ((m >> (lane & ~(lpr - 1))) & gmask) != 0;
@@ -372,6 +381,7 @@ __global__ __launch_bounds__(LEVEL_WG) void k_level(const LevelTask* __restrict_
       }
     }
   }
+  if (!count) return;
   pop = wave_sum(pop);
   if (lane == 0 && pop) atomicAdd(&T.cnt[(int64_t)(level % ring) * T.cstride], pop);
   if (FR) {

@@ -309,6 +309,36 @@ def test_frontier_and_dense_levels_agree(kw):
             odp.finalize_aspl()
 
 
+def prism_graph(m):
+    """Two m-cycles joined by rungs: 3-regular, diameter m // 2 + 1."""
+    adj = np.zeros((2 * m, 3), np.int32)
+    for i in range(m):
+        adj[i] = [(i - 1) % m, (i + 1) % m, m + i]
+        adj[m + i] = [m + (i - 1) % m, m + (i + 1) % m, i]
+    return adj
+
+
+@pytest.mark.parametrize("plan", [None, dict(plan=LEV, kw=4, upw=4, seg_major=0, xcd=1, direct=0),
+                                  dict(plan=LEV, kw=2, upw=2, seg_major=1, xcd=1, direct=-1)])
+def test_level_count_guess_mismatch(plan):
+    """An evaluation launches the level the previous one ended on without writing
+    its rows (counts only).  Alternating a short-diameter and a long-diameter
+    graph on one Init makes that guess wrong both ways (the level turns out not
+    to be the last, and rows are then written after all; or the graph ends
+    earlier); every result must equal the oracle."""
+    short = odp.generate_regular(200, 3, 41)
+    long_ = prism_graph(100)
+    ws, wl = oracle.evaluate(short, nodes=200), oracle.evaluate(long_, nodes=200)
+    odp.init_aspl_general(200, 3)
+    try:
+        if plan:
+            odp.set_plan(**plan)
+        for adj, want in ((short, ws), (long_, wl), (long_, wl), (short, ws), (short, ws), (long_, wl), (short, ws)):
+            assert same(odp.set_aspl(adj), want)
+    finally:
+        odp.finalize_aspl()
+
+
 def test_batch_policy_autotune_and_graph_replay():
     """Every batch launch policy (block width per size class, workgroups per CU)
     gives the oracle's results; the autotuner keeps one of them; the captured
