@@ -884,19 +884,45 @@ static int rotate_host(int v, int width, int height, int s, int angle)
 }
 
 // orbit-major row permutation of a 2/4-fold symmetric grid (utils.c:2103-2124)
+// Position of each base vertex (local index x * bh + y of a bw x bh region)
+// when the region is walked in GRID_TILE x GRID_TILE tiles: a column block of
+// GRID_TILE^2 sources is then one compact patch of the grid, whose distances
+// to any vertex span few levels (frontier-mode levels skip it sooner).
+static const int GRID_TILE = 32;
+static void tile_rank(int bw, int bh, std::vector<int>& rank)
+{
+  rank.assign((size_t)bw * bh, 0);
+  int pos = 0;
+  for (int tx = 0; tx < bw; tx += GRID_TILE)
+    for (int ty = 0; ty < bh; ty += GRID_TILE)
+      for (int x = tx; x < std::min(bw, tx + GRID_TILE); x++)
+        for (int y = ty; y < std::min(bh, ty + GRID_TILE); y++) rank[(size_t)x * bh + y] = pos++;
+}
+
+// Row of every vertex of a 2/4-fold symmetric grid (utils.c:2103-2124): the base
+// vertices (one per orbit) take rows [0, n/s), their rotations the same row plus
+// k * n/s.  Within the base rows the order is tiled (tile_rank) instead of the
+// reference's x-major order; results do not depend on it (rows and sources are
+// relabelled together).
 static void make_itable(int width, int height, int s, int* itable)
 {
   int based = width * height / s;
+  std::vector<int> rank;
   if (s == 2) {
-    for (int i = 0; i < based; i++) { itable[i] = i; itable[rotate_host(i, width, height, 2, 180)] = i + based; }
+    tile_rank(width / 2, height, rank);
+    for (int i = 0; i < based; i++) {
+      itable[i] = rank[i];
+      itable[rotate_host(i, width, height, 2, 180)] = rank[i] + based;
+    }
   } else {
     int bh = height / 2;
+    tile_rank(width / 2, bh, rank);
     for (int i = 0; i < based; i++) {
       int v = (i / bh) * height + (i % bh);
-      itable[v] = i;
-      itable[rotate_host(v, width, height, 4, 90)] = i + based;
-      itable[rotate_host(v, width, height, 4, 180)] = i + 2 * based;
-      itable[rotate_host(v, width, height, 4, 270)] = i + 3 * based;
+      itable[v] = rank[i];
+      itable[rotate_host(v, width, height, 4, 90)] = rank[i] + based;
+      itable[rotate_host(v, width, height, 4, 180)] = rank[i] + 2 * based;
+      itable[rotate_host(v, width, height, 4, 270)] = rank[i] + 3 * based;
     }
   }
 }
