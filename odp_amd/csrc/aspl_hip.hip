@@ -79,8 +79,10 @@ static void ctx_init()
   C.ncu = prop.multiProcessorCount;
   HIPC(hipStreamCreateWithFlags(&C.stream, hipStreamNonBlocking));
   HIPC(hipFuncSetAttribute((const void*)k_resident, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_DYN_MAX));
-  HIPC(hipFuncSetAttribute((const void*)k_direct<true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_DYN_MAX));
-  HIPC(hipFuncSetAttribute((const void*)k_direct<false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_DYN_MAX));
+  HIPC(hipFuncSetAttribute((const void*)k_direct<true, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_DYN_MAX));
+  HIPC(hipFuncSetAttribute((const void*)k_direct<true, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_DYN_MAX));
+  HIPC(hipFuncSetAttribute((const void*)k_direct<false, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_DYN_MAX));
+  HIPC(hipFuncSetAttribute((const void*)k_direct<false, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_DYN_MAX));
   C.ready = true;
   trace("ctx ready");
 }
@@ -124,7 +126,7 @@ static hipEvent_t ev_get()
   return e;
 }
 // Only evaluations on the library's main stream (the ODP_* lifecycle, the
-// batched and looped paths) are timed; serving workers run untimed.
+// batched and looped paths) are timed; the serving engine's slots run untimed.
 static void timed_begin(TimedLaunch& t, int kind, int level, hipStream_t st = nullptr)
 {
   t.kind = kind; t.level = level;
@@ -202,9 +204,7 @@ struct Evaluator {
     g_seen_adj = nullptr;
   }
   // Graphs are captured and replayed on the library's main stream only; the
-  // serving workers launch eagerly: a run in which two worker threads captured
-  // graphs concurrently ended in a host segmentation fault whose cause was not
-  // found, so that path is off.
+  // serving engine's slots (their own streams) launch eagerly.
   bool graph_on() const { return graphs_env() && !timed() && st == C.stream; }
   const int* cur_adj() const { return ext_adj ? ext_adj : d_adj; }
   bool graph_ready(int key) const { return gexec && g_key == key && g_adj == cur_adj(); }
@@ -263,7 +263,7 @@ struct Evaluator {
     size_t ne = (size_t)n * dpad;
     if (ne > nbr_cap) { dfree(&d_nbr); dalloc(&d_nbr, ne); nbr_cap = ne; }
     dp16 = round_up(deg > 0 ? deg : 1, 8);
-    if (n <= 65535) {
+    if (n <= 65536) {
       size_t ne16 = (size_t)n * dp16;
       if (ne16 > nbr16_cap) { dfree(&d_nbr16); dalloc(&d_nbr16, ne16); nbr16_cap = ne16; }
     }
@@ -357,6 +357,11 @@ struct Evaluator {
   }
 
   // ---- direct enumeration of the first levels (k_direct) ----
+  // k_direct reads the 16-bit table when it exists with the 32-bit table's width
+  bool direct16() const
+  {
+    return plan.plan == ODPX_PLAN_LEVEL && lr.direct > 0 && d_nbr16 && dp16 == dpad;
+  }
   long direct_tree(int K) const
   {
     long t = 0, m = 1;
@@ -508,7 +513,7 @@ struct Evaluator {
     const bool res = plan.plan == ODPX_PLAN_RESIDENT;
     a.adj = ext_adj ? ext_adj : d_adj; a.numdeg = d_numdeg; a.itable = d_itable; a.err = d_err;
     a.nbr = res ? nullptr : d_nbr;
-    a.nbr16 = res ? d_nbr16 : nullptr;
+    a.nbr16 = res || direct16() ? d_nbr16 : nullptr;     // the level plan's k_direct reads 16-bit ids
     a.n = n; a.s = s; a.degree = degree; a.dpad = res ? dp16 : dpad;
     a.width = width; a.height = height; a.grid_s = grid_s;
     a.zero64 = z64; a.nzero64 = nz64; a.zero32 = z32; a.nzero32 = nz32;
@@ -571,11 +576,16 @@ struct Evaluator {
       a.nsrc = std::min(cnt_t * r.kw * 64, nsrc - a.src_lo);
       a.ntask = cnt_t; a.words = cnt_t * r.kw; a.chunks = (a.words + 127) / 128;
       a.K = K; a.list = (int)direct_list(K);
-      a.qmul = dpad > 4 ? (unsigned)((1ULL << 32) / (unsigned)(dpad / 4) + 1) : 0u;
+      const bool id16 = direct16();
+      const unsigned q = (unsigned)(dpad / (id16 ? 8 : 4));
+      a.nbr16 = d_nbr16;
+      a.qmul = q > 1 ? (unsigned)((1ULL << 32) / q + 1) : 0u;
       long m = 1;
       for (int i = 0; i < 8; i++) { a.pw[i] = (int)m; m = std::min<long>(m * dpad, 1L << 30); }
       const bool all = a.src_lo == 0 && a.nsrc == n;
-      hipLaunchKernelGGL(all ? k_direct<true> : k_direct<false>, dim3(r.dgrid), dim3(64 * r.dnw), r.dlds, st, a);
+      auto kern = all ? (id16 ? k_direct<true, true> : k_direct<true, false>)
+                      : (id16 ? k_direct<false, true> : k_direct<false, false>);
+      hipLaunchKernelGGL(kern, dim3(r.dgrid), dim3(64 * r.dnw), r.dlds, st, a);
     } else {
       hipLaunchKernelGGL(k_level0, dim3(cnt_t * r.per_task_wg), dim3(LEVEL_WG), 0, st,
                          r.tasks + first, r.wg_start, cnt_t, RING, r.upw, r.xcd);

@@ -80,7 +80,7 @@ struct PrepArgs {
   const int* numdeg;     // [n] expanded, or null
   const int* itable;     // [n] or null
   int* nbr;              // [n][dpad] 32-bit ids, or null
-  unsigned short* nbr16; // [n][dpad] 16-bit ids (n <= 65536), or null
+  unsigned short* nbr16; // [n][dpad] 16-bit ids (n <= 65536), or null (both may be written)
   int* err;
   int n, s, degree, dpad, width, height, grid_s;
   u64* zero64; int nzero64;   // counters reset to 0 at the start of an evaluation
@@ -150,7 +150,7 @@ __global__ void k_prepare(PrepArgs a)
       else out = a.grid_s ? a.itable[v] : v;
     }
     if (a.nbr16) a.nbr16[(int64_t)row * a.dpad + j] = (unsigned short)out;
-    else a.nbr[(int64_t)row * a.dpad + j] = out;
+    if (a.nbr) a.nbr[(int64_t)row * a.dpad + j] = out;
   }
 }
 
@@ -410,6 +410,7 @@ __global__ __launch_bounds__(LEVEL_WG) void k_level(const LevelTask* __restrict_
 
 struct DirectArgs {
   const int* nbr;        // [n][dpad] row-neighbour table
+  const unsigned short* nbr16;   // the same with 16-bit ids (n <= 65536), read when ID16
   u64* out;              // block t, row v, word w: out[t * tstride + v * kw + w]
   int64_t tstride;       // words between the slabs of consecutive blocks
   u64* cnt;              // count of block t at level L: cnt[L * cstride + t], L < K
@@ -420,8 +421,8 @@ struct DirectArgs {
   int chunks;            // ceil(words / 128): 128-word chunks a wave sweeps per row
   int K;                 // levels computed (2..DIRECT_KMAX)
   int list;              // ints of walk-tree ids kept per wave: 4 + sum of dpad^i, 0 < i < K
-  unsigned qmul;         // floor(2^32 / q) + 1 for q = dpad / 4 > 1: idx / q == umulhi(idx, qmul)
-                         // for idx < 2^32 / q (idx < 2^24 and q <= 256 here)
+  unsigned qmul;         // floor(2^32 / q) + 1 for q = 16-byte loads per table row > 1:
+                         // idx / q == umulhi(idx, qmul) for idx < 2^32 / q (idx < 2^24, q <= 256 here)
   int pw[8];             // dpad^i
 };
 
@@ -436,7 +437,8 @@ __device__ __forceinline__ void direct_set(u32* bm, int id, int src_lo, int nsrc
   if (ALL || (unsigned)c < (unsigned)nsrc) atomicOr(&bm[c >> 5], 1u << (c & 31));
 }
 
-template <bool ALL>
+// ID16: the 16-bit table (eight ids per 16-byte load) instead of the 32-bit one
+template <bool ALL, bool ID16>
 __global__ __launch_bounds__(256) void k_direct(DirectArgs a)
 {
   extern __shared__ __attribute__((aligned(16))) u64 dlds[];
@@ -454,7 +456,8 @@ __global__ __launch_bounds__(256) void k_direct(DirectArgs a)
   const int ncnt = a.K * a.chunks * 64;
   for (int i = threadIdx.x; i < ncnt; i += blockDim.x) lc[i] = 0;
   __syncthreads();
-  const int Q = a.dpad >> 2;                       // 16-byte loads per table row
+  const int IPL = ID16 ? 8 : 4;                    // ids per 16-byte load
+  const int Q = a.dpad / IPL;                      // 16-byte loads per table row
   // Each wave works on its own rows, bitmap and lists: between the phases of a
   // row only the wave's own LDS accesses must be ordered, which LDS guarantees
   // for one wave's instructions, so a wavefront fence (compiler ordering) is
@@ -472,24 +475,42 @@ __global__ __launch_bounds__(256) void k_direct(DirectArgs a)
       const int step = d == 1 ? 4 : m;             // ints the depth d-1 list takes
       int* to = list + off + step;                 // the depth-d list follows
       for (int j0 = 0; j0 < jobs; j0 += 64 * DIRECT_UNROLL) {
-        int4 q[DIRECT_UNROLL];
+        uint4 q[DIRECT_UNROLL];
 #pragma unroll
         for (int u = 0; u < DIRECT_UNROLL; u++) {
           const unsigned idx = j0 + u * 64 + lane;
           if (idx < (unsigned)jobs) {
             const unsigned r = Q == 1 ? idx : __umulhi(idx, a.qmul);
-            q[u] = *(const int4*)(a.nbr + (int64_t)from[r] * a.dpad + 4 * (idx - r * Q));
+            const int64_t e = (int64_t)from[r] * a.dpad + IPL * (idx - r * Q);
+            q[u] = ID16 ? *(const uint4*)(a.nbr16 + e) : *(const uint4*)(a.nbr + e);
           }
         }
 #pragma unroll
         for (int u = 0; u < DIRECT_UNROLL; u++) {
           const int idx = j0 + u * 64 + lane;
           if (idx < jobs) {
-            direct_set<ALL>(bm32, q[u].x, a.src_lo, a.nsrc);
-            direct_set<ALL>(bm32, q[u].y, a.src_lo, a.nsrc);
-            direct_set<ALL>(bm32, q[u].z, a.src_lo, a.nsrc);
-            direct_set<ALL>(bm32, q[u].w, a.src_lo, a.nsrc);
-            if (d < a.K) *(int4*)(to + 4 * idx) = q[u];
+            if (ID16) {
+              int4 lo, hi;
+              lo.x = q[u].x & 0xffff; lo.y = q[u].x >> 16; lo.z = q[u].y & 0xffff; lo.w = q[u].y >> 16;
+              hi.x = q[u].z & 0xffff; hi.y = q[u].z >> 16; hi.z = q[u].w & 0xffff; hi.w = q[u].w >> 16;
+              direct_set<ALL>(bm32, lo.x, a.src_lo, a.nsrc);
+              direct_set<ALL>(bm32, lo.y, a.src_lo, a.nsrc);
+              direct_set<ALL>(bm32, lo.z, a.src_lo, a.nsrc);
+              direct_set<ALL>(bm32, lo.w, a.src_lo, a.nsrc);
+              direct_set<ALL>(bm32, hi.x, a.src_lo, a.nsrc);
+              direct_set<ALL>(bm32, hi.y, a.src_lo, a.nsrc);
+              direct_set<ALL>(bm32, hi.z, a.src_lo, a.nsrc);
+              direct_set<ALL>(bm32, hi.w, a.src_lo, a.nsrc);
+              if (d < a.K) { *(int4*)(to + 8 * idx) = lo; *(int4*)(to + 8 * idx + 4) = hi; }
+            } else {
+              int4 v;
+              v.x = (int)q[u].x; v.y = (int)q[u].y; v.z = (int)q[u].z; v.w = (int)q[u].w;
+              direct_set<ALL>(bm32, v.x, a.src_lo, a.nsrc);
+              direct_set<ALL>(bm32, v.y, a.src_lo, a.nsrc);
+              direct_set<ALL>(bm32, v.z, a.src_lo, a.nsrc);
+              direct_set<ALL>(bm32, v.w, a.src_lo, a.nsrc);
+              if (d < a.K) *(int4*)(to + 4 * idx) = v;
+            }
           }
         }
       }
