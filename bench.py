@@ -56,6 +56,9 @@ WORKLOADS = {
                                   "(16384 sources expanded), MATRIX_SAVING, adjacency resident in HBM"),
     "random1024": ("random", 1024, 6, "single seeded random 6-regular simple graph, n=1024 (CPU-runnable "
                                       "case), adjacency resident in HBM"),
+    "batch1024": ("batch", 1024, 0, "1024 seeded random d-regular graphs, n uniform even in [1024,16384], "
+                                    "d in {4,8,12,16,24}, one 2-opt swap each, resident in HBM; a step is one "
+                                    "batched evaluation of all of them"),
 }
 
 
@@ -262,6 +265,47 @@ def batched_extra(count, loops):
                         "d in {4,8,12,16,24}, one 2-opt swap each"}
 
 
+def batch_workload(args, desc):
+    """The batched path as the measured step (for profiling k_resident): one batch
+    run of the 1024 graphs per step, with the tuned launch policy."""
+    graphs = c4_graphs(1024)
+    ng = odp.batch_load(odp.pack(graphs))
+    odp.batch_autotune(1)
+    ref = odp.batch_run(ng)
+    for _ in range(max(1, args.warmup)):
+        odp.batch_run(ng)
+    t = time.perf_counter()
+    for _ in range(args.steps):
+        r = odp.batch_run(ng)
+    el = time.perf_counter() - t
+    assert all(x.tobytes() == y.tobytes() for x, y in zip(r, ref))
+    odp.stats_enable(True)
+    odp.stats_reset()
+    for _ in range(min(args.steps, 5)):
+        odp.batch_run(ng)
+    st = odp.stats()
+    odp.stats_enable(False)
+    ms = st["resident_ms"] / max(1, st["resident_launches"])
+    alg = st["resident_bytes"] / max(1, st["resident_launches"])
+    achieved = alg / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+    policy = odp.batch_policy()
+    odp.batch_release()
+    out = {"metric": METRIC, "value": round(1024 * args.steps / el, 3), "unit": "evals/s", "n_gpus": 1,
+           "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * el / args.steps, 4),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
+           "data": "synthetic",
+           "config": {"workload": desc, "name": args.workload, "graphs": 1024,
+                      "policy": {"wb_cap_per_class": policy[0], "workgroups_per_cu": policy[1]}},
+           "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_resident",
+                        "launch_ms": round(ms, 4), "alg_bytes_per_launch": alg,
+                        "launches": st["resident_launches"],
+                        "note": "algorithmic bytes are the bit-row operands, which live in LDS: the kernel is "
+                                "LDS-bound, so this is not an HBM fraction"},
+           "cpu_baseline": None}
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -288,6 +332,8 @@ def main():
         sys.exit(2)
 
     gkind, n, d, desc = WORKLOADS[args.workload]
+    if gkind == "batch":
+        return batch_workload(args, desc)
     t0 = time.perf_counter()
     if gkind == "random":
         adj = odp.generate_regular(n, d, args.seed)
