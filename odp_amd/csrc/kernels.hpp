@@ -36,7 +36,6 @@ struct __attribute__((aligned(16))) u64x2 { u64 x, y; };
 
 #define ODP_NXCD 8          // XCDs of an MI355X; workgroups are dealt round-robin
 #define LEVEL_WG 256        // threads per level-kernel workgroup (4 waves)
-#define LV_PF 4             // units whose own rows k_level loads together (plain kernel)
 #define LV_STORE 1          // k_level mode bits: write the next rows
 #define LV_COUNT 2          //   ... and add the level's counts (both: a normal level)
 
@@ -314,24 +313,9 @@ __global__ __launch_bounds__(LEVEL_WG) void k_level(const LevelTask* __restrict_
   const int64_t kwp = T.kwp;
   u64 pop = 0, nch = 0, nact = 0;
   const int base = wg * 4 * upw;
-  // Without frontier mode every unit reads its own rows first: those loads are
-  // issued for LV_PF units at once (a wave's units are otherwise one dependent
-  // load apart, which bounds the levels where most lanes are already saturated).
-  u64x2 xp[LV_PF];
   for (int i = 0; i < upw; i++) {
     const int u = base + i * 4 + wave;
     if (u >= T.units) break;
-    if (!FR && i % LV_PF == 0) {
-#pragma unroll
-      for (int p = 0; p < LV_PF; p++) {
-        const int up = u + 4 * p;
-        if (i + p < upw && up < T.units) {
-          UnitGeom qp = unit_geom(T, up, lane);
-          if (UROW) qp.r = __builtin_amdgcn_readfirstlane(qp.r);
-          if (qp.r < T.n) xp[p] = *(const u64x2*)(cur + qp.r * kwp + qp.w);
-        }
-      }
-    }
     UnitGeom q = unit_geom(T, u, lane);
     if (UROW) q.r = __builtin_amdgcn_readfirstlane(q.r);
     if (q.r < T.n) {
@@ -348,7 +332,7 @@ __global__ __launch_bounds__(LEVEL_WG) void k_level(const LevelTask* __restrict_
       bool changed = false;
       if (act) {
         const u64 f0 = full_word(T.l, q.w), f1 = full_word(T.l, q.w + 1);
-        const u64x2 x0 = FR ? *(const u64x2*)(cur + q.r * kwp + q.w) : xp[i % LV_PF];
+        const u64x2 x0 = *(const u64x2*)(cur + q.r * kwp + q.w);
         u64x2 x = x0;
         // Once a lane's words are saturated the remaining neighbours cannot add
         // a bit (OR is monotone): the lane stops gathering (its loads are masked
