@@ -492,9 +492,6 @@ struct Evaluator {
     HIPC(hipMemcpy(r.tasks, ht.data(), sizeof(ResTask) * ntasks, hipMemcpyHostToDevice));
     r.ntasks = ntasks;
     r.lds = (size_t)n * r.wb * 4;
-    // with room, the launch also holds the 16-bit neighbour table (staged once per block)
-    const size_t staged = ((size_t)res_row_words(n, r.wb) + (size_t)n * (dp16 / 8) * 4) * 4;
-    if (staged <= LDS_DYN_MAX) r.lds = staged;
     int per_cu = (int)std::max<size_t>(1, LDS_LIMIT / (r.lds + LDS_STATIC));
     per_cu = std::min(per_cu, 2048 / r.threads);
     r.grid = std::max(1, std::min(ntasks, per_cu * C.ncu));
@@ -788,8 +785,7 @@ struct Evaluator {
       prepare((u64*)r.out, (int)(sizeof(ResOut) / 8), (u32*)r.qhead, 1);
       TimedLaunch tl;
       timed_begin(tl, T_RESIDENT, 0, st);
-      hipLaunchKernelGGL(k_resident, dim3(r.grid), dim3(r.threads), r.lds, st, r.tasks, r.ntasks, r.qhead, r.out,
-                         (int)r.lds);
+      hipLaunchKernelGGL(k_resident, dim3(r.grid), dim3(r.threads), r.lds, st, r.tasks, r.ntasks, r.qhead, r.out);
       HIPC(hipGetLastError());
       timed_end(tl);
       HIPC(hipMemcpyAsync(r.h_out, r.out, sizeof(ResOut), hipMemcpyDeviceToHost, st));
@@ -1405,8 +1401,7 @@ static void batch_enqueue()
   const int grid = std::max(1, std::min(B.ntasks, per_cu * C.ncu));
   TimedLaunch tr;
   timed_begin(tr, T_RESIDENT, 0);
-  hipLaunchKernelGGL(k_resident, dim3(grid), dim3(1024), B.lds, C.stream, B.d_t, B.ntasks, B.d_qhead, B.d_out,
-                     (int)B.lds);
+  hipLaunchKernelGGL(k_resident, dim3(grid), dim3(1024), B.lds, C.stream, B.d_t, B.ntasks, B.d_qhead, B.d_out);
   HIPC(hipGetLastError());
   timed_end(tr);
   HIPC(hipMemcpyAsync(B.h_out, B.d_out, sizeof(ResOut) * B.small.size(), hipMemcpyDeviceToHost, C.stream));

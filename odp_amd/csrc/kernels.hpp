@@ -653,27 +653,12 @@ __device__ __forceinline__ u64 block_total(u64 v, u64* part, int wave, int nw)
   return uniform_u64(s);
 }
 
-// LDS words (u32) the bit rows of an n-row block of width WB take, rounded so
-// the neighbour ids staged after them start 16-byte aligned
-__host__ __device__ constexpr int res_row_words(int n, int wb) { return (n * wb + 3) & ~3; }
-
-// All levels of one block of T.l <= 32*WB sources.  When the block's 16-bit
-// neighbour table fits in the launch's LDS after the bit rows, it is staged
-// there once and every level reads its ids from LDS instead of global memory.
+// All levels of one block of T.l <= 32*WB sources.
 template <int WB>
 __device__ __forceinline__ void res_run(const ResTask& T, u32* cur, u64 (*s_part)[16], ResOut* out,
-                                        int tid, int nth, int lane, int wave, int nw, int lds_bytes)
+                                        int tid, int nth, int lane, int wave, int nw)
 {
   constexpr int RPT = res_rpt(WB);
-  const int q8 = T.dp >> 3;                              // 16-byte id loads per row
-  const bool staged = (res_row_words(T.n, WB) + T.n * q8 * 4) * 4 <= lds_bytes;   // wave-uniform
-  const uint4* gids = (const uint4*)T.nbr;               // uniform base, 32-bit offsets
-  uint4* lids = (uint4*)(cur + res_row_words(T.n, WB));  // LDS copy when staged
-  if (staged) {
-    for (int i = tid; i < T.n * q8; i += nth) lids[i] = gids[i];
-    __syncthreads();
-  }
-  auto ids_at = [&](u32 i) -> uint4 { return staged ? lids[i] : gids[i]; };
   const u64 target = (u64)T.n * (u64)T.l;
   RowT<WB> full;
 #pragma unroll
@@ -688,9 +673,10 @@ __device__ __forceinline__ void res_run(const ResTask& T, u32* cur, u64 (*s_part
 #pragma unroll
     for (int k = 0; k < WB; k++) row.w[k] = 0;
     row_bit<WB>(row, r - T.src0, T.l);
+    const uint4* ids = (const uint4*)T.nbr;     // uniform base, 32-bit offsets
     const u32 off = (u32)r * (u32)(T.dp >> 3);
     for (int j = 0; j < (T.dp >> 3); j++) {
-      const uint4 v = ids_at(off + j);
+      const uint4 v = ids[off + j];
       row_bit<WB>(row, (int)(v.x & 0xffff) - T.src0, T.l); row_bit<WB>(row, (int)(v.x >> 16) - T.src0, T.l);
       row_bit<WB>(row, (int)(v.y & 0xffff) - T.src0, T.l); row_bit<WB>(row, (int)(v.y >> 16) - T.src0, T.l);
       row_bit<WB>(row, (int)(v.z & 0xffff) - T.src0, T.l); row_bit<WB>(row, (int)(v.z >> 16) - T.src0, T.l);
@@ -722,9 +708,10 @@ __device__ __forceinline__ void res_run(const ResTask& T, u32* cur, u64 (*s_part
 #pragma unroll
         for (int k = 0; k < WB; k++) sat = sat && (row.w[k] == full.w[k]);
         if (!sat) {
+          const uint4* ids = (const uint4*)T.nbr;
           const u32 off = (u32)r * (u32)(T.dp >> 3);
           for (int j = 0; j < (T.dp >> 3); j++) {
-            const uint4 v = ids_at(off + j);
+            const uint4 v = ids[off + j];
             row_or<WB>(row, cur, v.x & 0xffff); row_or<WB>(row, cur, v.x >> 16);
             row_or<WB>(row, cur, v.y & 0xffff); row_or<WB>(row, cur, v.y >> 16);
             row_or<WB>(row, cur, v.z & 0xffff); row_or<WB>(row, cur, v.z >> 16);
@@ -760,8 +747,7 @@ __device__ __forceinline__ void res_run(const ResTask& T, u32* cur, u64 (*s_part
 // hold n * wb * 4 bytes for its largest block, and blockDim.x * res_rpt(wb)
 // must cover n for every block.
 __global__ __launch_bounds__(1024) void k_resident(const ResTask* __restrict__ tasks, int ntasks,
-                                                   int* __restrict__ qhead, ResOut* __restrict__ out,
-                                                   int lds_bytes)
+                                                   int* __restrict__ qhead, ResOut* __restrict__ out)
 {
   extern __shared__ __attribute__((aligned(16))) u32 lds[];
   __shared__ int s_task;
@@ -778,10 +764,10 @@ __global__ __launch_bounds__(1024) void k_resident(const ResTask* __restrict__ t
     if (t >= ntasks) break;
     const ResTask T = tasks[t];
     switch (T.wb) {
-      case 1: res_run<1>(T, lds, s_part, out, tid, nth, lane, wave, nw, lds_bytes); break;
-      case 2: res_run<2>(T, lds, s_part, out, tid, nth, lane, wave, nw, lds_bytes); break;
-      case 4: res_run<4>(T, lds, s_part, out, tid, nth, lane, wave, nw, lds_bytes); break;
-      default: res_run<8>(T, lds, s_part, out, tid, nth, lane, wave, nw, lds_bytes); break;
+      case 1: res_run<1>(T, lds, s_part, out, tid, nth, lane, wave, nw); break;
+      case 2: res_run<2>(T, lds, s_part, out, tid, nth, lane, wave, nw); break;
+      case 4: res_run<4>(T, lds, s_part, out, tid, nth, lane, wave, nw); break;
+      default: res_run<8>(T, lds, s_part, out, tid, nth, lane, wave, nw); break;
     }
   }
 }

@@ -199,8 +199,7 @@ static void sv_small_launch(SvSmallSlot& sl)
   HIPC(hipMemsetAsync(sl.d_out, 0, sizeof(ResOut) * nb, sl.st));
   HIPC(hipMemsetAsync(sl.d_qhead, 0, sizeof(int), sl.st));
   const int grid = std::max(1, std::min((int)sl.tasks.size(), C.ncu));
-  hipLaunchKernelGGL(k_resident, dim3(grid), dim3(1024), lds, sl.st, sl.d_t, (int)sl.tasks.size(), sl.d_qhead, sl.d_out,
-                     (int)lds);
+  hipLaunchKernelGGL(k_resident, dim3(grid), dim3(1024), lds, sl.st, sl.d_t, (int)sl.tasks.size(), sl.d_qhead, sl.d_out);
   HIPC(hipGetLastError());
   HIPC(hipMemcpyAsync(sl.h_out, sl.d_out, sizeof(ResOut) * nb, hipMemcpyDeviceToHost, sl.st));
   HIPC(hipMemcpyAsync(sl.h_status, sl.d_status, sizeof(int) * nb, hipMemcpyDeviceToHost, sl.st));
