@@ -153,7 +153,7 @@ struct LevelRun {
   size_t buf_words = 0;                 // capacity of bufs (64-bit words)
   size_t task_cap = 0;
   u64* bufs = nullptr; u64* cnt = nullptr; u32* done = nullptr;
-  unsigned char* flags = nullptr;       // changed flags: per block of the group 2 x segs x n bytes
+  unsigned char* flags = nullptr;       // per block of the group 4 x segs x n bytes: changed flags, marks
   u64* h_acnt = nullptr;                // processed-unit counts on the host (kernel timing only)
   size_t flag_bytes = 0;
   LevelTask* tasks = nullptr; int* wg_start = nullptr;
@@ -416,11 +416,15 @@ struct Evaluator {
     // bit buffers for `group` column blocks; group g of blocks reuses them
     size_t words = (size_t)group * 2 * n * kwp;
     if (words > r.buf_words) { dfree(&r.bufs); dalloc(&r.bufs, words); r.buf_words = words; }
-    const size_t fbytes = (size_t)group * 2 * segs * n;
-    if (fbytes > r.flag_bytes) { dfree(&r.flags); dalloc(&r.flags, fbytes); r.flag_bytes = fbytes; }
+    const size_t fbytes = (size_t)group * 4 * segs * n;
+    if (fbytes > r.flag_bytes) {
+      dfree(&r.flags); dalloc(&r.flags, fbytes); r.flag_bytes = fbytes;
+      HIPC(hipMemset(r.flags, 0, fbytes));           // marks start clear (later levels keep them so)
+    }
     if ((size_t)ntasks > r.task_cap) {
       dfree(&r.cnt); dfree(&r.done); dfree(&r.tasks); dfree(&r.wg_start); hfree(&r.h_cnt);
-      dalloc(&r.cnt, (size_t)3 * RING * ntasks); dalloc(&r.done, (size_t)ntasks);   // counts, changed, processed
+      dalloc(&r.cnt, (size_t)3 * RING * ntasks);     // counts, changed, processed
+      dalloc(&r.done, (size_t)ntasks + 1);          // done flags, then the table's asymmetry flag
       hfree(&r.h_acnt); halloc(&r.h_acnt, (size_t)RING * ntasks);
       dalloc(&r.tasks, (size_t)ntasks); dalloc(&r.wg_start, (size_t)ntasks + 1);
       halloc(&r.h_cnt, (size_t)RING * ntasks);
@@ -460,8 +464,11 @@ struct Evaluator {
       T.seg_major = r.seg_major; T.cstride = ntasks; T.pad = 0;
       T.ccnt = r.cnt + (size_t)RING * ntasks + t;
       T.acnt = r.cnt + (size_t)2 * RING * ntasks + t;
-      T.chg0 = r.flags + (size_t)(t % group) * 2 * segs * n;
+      T.chg0 = r.flags + (size_t)(t % group) * 4 * segs * n;
       T.chg1 = T.chg0 + (size_t)segs * n;
+      T.act0 = T.chg0 + (size_t)2 * segs * n;
+      T.act1 = T.chg0 + (size_t)3 * segs * n;
+      T.asym = r.done + ntasks;
       r.l[t] = T.l;
       ws[t] = t * r.per_task_wg;     // blocks are equal: the starts of a group rebase to 0
     }
@@ -643,9 +650,9 @@ struct Evaluator {
                   ? lv.batch_end - 1 : -1;
     if (lv.first == 0 && lv.graphed) {
       // prepare + the first levels + the first batch + the counter copies as one graph
-      const int be = lv.batch_end, L1 = lv.L1, nz = lv.nzero, key = lv_key(be);
+      const int be = lv.batch_end, L1 = lv.L1, key = lv_key(be);
       if (!graph_ready(key))
-        capture([&]() { prepare(r.cnt, nz, r.done, r.ntasks); lv_level0(); lv_levels(L1, be); }, key);
+        capture([&]() { lv_prepare(); lv_level0(); lv_levels(L1, be); }, key);
       HIPC(hipGraphLaunch(gexec, st));
     } else {
       lv_level0();
@@ -660,6 +667,20 @@ struct Evaluator {
   int first_batch_end() const
   {
     return std::min(n, std::max(2, std::min(MAX_BATCH_LEVELS, guess_levels > 0 ? guess_levels : 8)));
+  }
+
+  // neighbour table, zeroed rings, done and asymmetry flags; in frontier mode the
+  // symmetry check that lets changed rows mark their neighbours (push mode)
+  void lv_prepare()
+  {
+    LevelRun& r = lr;
+    prepare(r.cnt, lv.nzero, r.done, r.ntasks + 1);
+    if (lv.fr) {
+      const int64_t total = (int64_t)n * dpad;
+      const int grid = (int)std::min<int64_t>(std::max<int64_t>(1, (total + 255) / 256), 4096);
+      hipLaunchKernelGGL(k_symcheck, dim3(grid), dim3(256), 0, st, d_nbr, n, dpad, r.done + r.ntasks);
+      HIPC(hipGetLastError());
+    }
   }
 
   // graph key of a first batch ending at be: its length, the kernel variant and
@@ -682,7 +703,7 @@ struct Evaluator {
     // level is dense and the plain kernel is faster
     lv.fr = guess_levels == 0 || guess_levels > FRONTIER_MIN_LEVELS;
     lv.graphed = r.group == r.ntasks && use_graph(lv_key(lv.be0));
-    if (!lv.graphed) prepare(r.cnt, lv.nzero, r.done, r.ntasks);   // the first ring slots and the done flags
+    if (!lv.graphed) lv_prepare();
     lv.sts.assign(r.ntasks, LvTask{1, 0, 0, 0});
     lv.any_disc = false;
     lv.max_d = 1; lv.lmax = 0;

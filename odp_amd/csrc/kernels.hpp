@@ -154,6 +154,22 @@ __global__ void k_prepare(PrepArgs a)
   }
 }
 
+// Sets *asym when some row u is listed among the neighbours of row r while r
+// is not among u's (entries equal to the row itself are padding).
+__global__ void k_symcheck(const int* __restrict__ nbr, int n, int dpad, u32* asym)
+{
+  const int64_t total = (int64_t)n * dpad;
+  for (int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; idx < total;
+       idx += (int64_t)gridDim.x * blockDim.x) {
+    const int r = (int)(idx / dpad);
+    const int u = nbr[idx];
+    if (u == r) continue;
+    bool found = false;
+    for (int k = 0; k < dpad && !found; k++) found = nbr[(int64_t)u * dpad + k] == r;
+    if (!found) atomicOr(asym, 1u);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // level kernels
 
@@ -176,6 +192,9 @@ struct LevelTask {
   u64* acnt;           // per-level count of processed (row, segment) units, same layout
   unsigned char* chg0; // changed flag of (segment s, row r) at [s * n + r]: levels of even parity
   unsigned char* chg1; //   ... and of odd parity
+  unsigned char* act0; // "may change" marks of (segment, row) for levels of even parity (push mode)
+  unsigned char* act1; //   ... and of odd parity
+  const u32* asym;     // nonzero when the table is not symmetric (push mode needs u in N(r) <=> r in N(u))
 };
 
 __device__ __forceinline__ int find_task(const int* __restrict__ wg_start, int ntasks, int g)
@@ -258,7 +277,10 @@ __global__ __launch_bounds__(LEVEL_WG) void k_level0(const LevelTask* __restrict
 // per batch (in flight together) before the wave checks for saturation.
 //
 // FR (chosen per evaluation, for graphs with many levels): every level records,
-// per (row, segment), whether the segment changed and how many did.  When fewer than half changed at level L-1 (and L-1 was not
+// per (row, segment), whether the segment changed and how many did.  For a
+// symmetric table (push mode) a changed segment also marks itself and its
+// neighbours' segments for level L+1, so the next level tests one byte per
+// row; otherwise it tests the row's and its neighbours' changed flags (pull).  When fewer than half changed at level L-1 (and L-1 was not
 // the first gathered level, which wrote every row), level L runs in frontier
 // mode: a segment whose row and neighbours all stayed unchanged at L-1 cannot
 // change at L, and the buffer level L writes already holds its value (the
@@ -304,6 +326,9 @@ __global__ __launch_bounds__(LEVEL_WG) void k_level(const LevelTask* __restrict_
                   2 * T.ccnt[(int64_t)((level - 1) % ring) * T.cstride] < (u64)T.n * (u64)T.segs;
   const unsigned char* __restrict__ chr = (level & 1) ? T.chg0 : T.chg1;   // written by level L-1
   unsigned char* __restrict__ chw = (level & 1) ? T.chg1 : T.chg0;
+  const bool push = FR && *T.asym == 0;
+  unsigned char* actr = (level & 1) ? T.act1 : T.act0;    // marks for this level
+  unsigned char* actw = (level & 1) ? T.act0 : T.act1;    // marks for level L+1
   const u64* __restrict__ cur = (level & 1) ? T.buf1 : T.buf0;
   u64* __restrict__ nxt = (level & 1) ? T.buf0 : T.buf1;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -322,7 +347,13 @@ __global__ __launch_bounds__(LEVEL_WG) void k_level(const LevelTask* __restrict_
       const int64_t so = (int64_t)q.seg * T.n;
       const int* nb = T.nbr + (int64_t)q.r * T.dpad;
       bool act = true;
-      if (fr) {
+      if (fr && push) {
+        // marked by level L-1 for a change of this row or a neighbour; the pass
+        // that writes the rows clears the mark (a count-only pass leaves it for
+        // its store-only re-run), so the array is clean when level L+1 marks it
+        act = actr[so + q.r] != 0;
+        if (act && store && (lane & (lpr - 1)) == 0) actr[so + q.r] = 0;
+      } else if (fr) {
         act = chr[so + q.r] != 0;
         for (int j = 0; j < T.dpad && !act; j += 4) {
           const int4 v = *(const int4*)(nb + j);
@@ -377,6 +408,13 @@ __global__ __launch_bounds__(LEVEL_WG) void k_level(const LevelTask* __restrict_
           chw[so + q.r] = any ? 1 : 0;
           nch += any ? 1 : 0;
           nact += act ? 1 : 0;
+        }
+        if (push && any) {
+          // mark the row and its neighbours for level L+1 (the table is symmetric:
+          // the rows that gather this one are its neighbours); the row's lanes
+          // share the neighbours
+          if ((lane & (lpr - 1)) == 0) actw[so + q.r] = 1;
+          for (int j = lane & (lpr - 1); j < T.dpad; j += lpr) actw[so + nb[j]] = 1;
         }
       }
     }

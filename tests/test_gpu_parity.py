@@ -292,10 +292,13 @@ def test_frontier_and_dense_levels_agree(kw):
     """The first evaluation after Init runs the frontier-mode level kernel; later
     ones run it only for graphs that needed more than 16 levels, the plain one
     otherwise.  Every evaluation equals the oracle: small and large diameters,
-    a non-regular table, a disconnected graph."""
+    a non-regular table, a directed table, a disconnected graph."""
     cases = [(odp.generate_regular(3000, 5, 21), None), (cycle_graph(600), None)]
     pa, nd = path_graph(257)
     cases.append((pa, nd))
+    # a directed table (steps +1 and +2 around a cycle): not symmetric, so the
+    # frontier levels test neighbours' change flags instead of marks
+    cases.append((np.array([[(i + 1) % 300, (i + 2) % 300] for i in range(300)], np.int32), None))
     cases.append((np.concatenate([cycle_graph(300), cycle_graph(200) + 300]).astype(np.int32), None))
     for adj, nd in cases:
         n = adj.shape[0]
