@@ -154,6 +154,8 @@ struct LevelRun {
   size_t task_cap = 0;
   u64* bufs = nullptr; u64* cnt = nullptr; u32* done = nullptr;
   unsigned char* flags = nullptr;       // per block of the group 4 x segs x n bytes: changed flags, marks
+  unsigned char* wgm = nullptr;         // per block of the group 2 x workgroups bytes: workgroup marks
+  size_t wgm_bytes = 0;
   u64* h_acnt = nullptr;                // processed-unit counts on the host (kernel timing only)
   size_t flag_bytes = 0;
   LevelTask* tasks = nullptr; int* wg_start = nullptr;
@@ -241,7 +243,7 @@ struct Evaluator {
     nbr16_cap = 0;
     hfree(&h_stage); hfree(&h_err);
     dfree(&lr.bufs); dfree(&lr.cnt); dfree(&lr.done); dfree(&lr.tasks); dfree(&lr.wg_start); hfree(&lr.h_cnt);
-    dfree(&lr.flags); hfree(&lr.h_acnt);
+    dfree(&lr.flags); hfree(&lr.h_acnt); dfree(&lr.wgm);
     dfree(&rr.tasks); dfree(&rr.out); dfree(&rr.qhead); hfree(&rr.h_out);
     lr = LevelRun(); rr = ResRun();
     adj_cap = nbr_cap = stage_cap = 0;
@@ -435,6 +437,11 @@ struct Evaluator {
     r.upw = plan.upw; r.seg_major = plan.seg_major; r.xcd = plan.xcd; r.urow = (lpr_log2 == 6);
     r.batch = plan.batch == 8 || plan.batch == 16 ? plan.batch : 4;
     r.per_task_wg = (units + 4 * r.upw - 1) / (4 * r.upw);
+    const size_t wbytes = (size_t)group * 2 * r.per_task_wg;
+    if (wbytes > r.wgm_bytes) {
+      dfree(&r.wgm); dalloc(&r.wgm, wbytes); r.wgm_bytes = wbytes;
+      HIPC(hipMemset(r.wgm, 0, wbytes));
+    }
     int K = plan.direct;
     while (K >= 2 && (direct_tree(K) > DIRECT_TREE_CAP || !direct_waves(K, kw, group))) K--;
     r.direct = plan.direct == -1 ? direct_auto(kw, group) : (K >= 2 ? K : 0);
@@ -469,6 +476,8 @@ struct Evaluator {
       T.act0 = T.chg0 + (size_t)2 * segs * n;
       T.act1 = T.chg0 + (size_t)3 * segs * n;
       T.asym = r.done + ntasks;
+      T.wgm0 = r.wgm + (size_t)(t % group) * 2 * r.per_task_wg;
+      T.wgm1 = T.wgm0 + r.per_task_wg;
       r.l[t] = T.l;
       ws[t] = t * r.per_task_wg;     // blocks are equal: the starts of a group rebase to 0
     }

@@ -195,7 +195,18 @@ struct LevelTask {
   unsigned char* act0; // "may change" marks of (segment, row) for levels of even parity (push mode)
   unsigned char* act1; //   ... and of odd parity
   const u32* asym;     // nonzero when the table is not symmetric (push mode needs u in N(r) <=> r in N(u))
+  unsigned char* wgm0; // per workgroup of the task: some row of it is marked (push mode), even levels
+  unsigned char* wgm1; //   ... odd levels
 };
+
+// workgroup (within its task) that processes segment seg of row r, for upw
+// units per wave and 4 waves per workgroup
+__device__ __forceinline__ int unit_wg(const LevelTask& T, int r, int seg, int upw)
+{
+  const int rg = r >> (6 - T.lpr_log2);
+  const int u = T.seg_major ? seg * T.rgroups + rg : rg * T.segs + seg;
+  return u / (4 * upw);
+}
 
 __device__ __forceinline__ int find_task(const int* __restrict__ wg_start, int ntasks, int g)
 {
@@ -329,6 +340,14 @@ __global__ __launch_bounds__(LEVEL_WG) void k_level(const LevelTask* __restrict_
   const bool push = FR && *T.asym == 0;
   unsigned char* actr = (level & 1) ? T.act1 : T.act0;    // marks for this level
   unsigned char* actw = (level & 1) ? T.act0 : T.act1;    // marks for level L+1
+  unsigned char* wgmw = (level & 1) ? T.wgm0 : T.wgm1;
+  if (fr && push) {
+    // a workgroup none of whose rows is marked has nothing to do at this level
+    unsigned char* wgmr = (level & 1) ? T.wgm1 : T.wgm0;
+    if (wgmr[wg] == 0) return;
+    __syncthreads();                                   // every thread has read the mark
+    if (store && threadIdx.x == 0) wgmr[wg] = 0;       // cleared like the row marks
+  }
   const u64* __restrict__ cur = (level & 1) ? T.buf1 : T.buf0;
   u64* __restrict__ nxt = (level & 1) ? T.buf0 : T.buf1;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -405,16 +424,23 @@ __global__ __launch_bounds__(LEVEL_WG) void k_level(const LevelTask* __restrict_
         const u64 m = __ballot(changed);
         const bool any = ((m >> (lane & ~(lpr - 1))) & gmask) != 0;
         if ((lane & (lpr - 1)) == 0) {
-          chw[so + q.r] = any ? 1 : 0;
+          if (!push) chw[so + q.r] = any ? 1 : 0;     // push mode reads marks, not flags
           nch += any ? 1 : 0;
           nact += act ? 1 : 0;
         }
         if (push && any) {
           // mark the row and its neighbours for level L+1 (the table is symmetric:
-          // the rows that gather this one are its neighbours); the row's lanes
-          // share the neighbours
-          if ((lane & (lpr - 1)) == 0) actw[so + q.r] = 1;
-          for (int j = lane & (lpr - 1); j < T.dpad; j += lpr) actw[so + nb[j]] = 1;
+          // the rows that gather this one are its neighbours), and their
+          // workgroups; the row's lanes share the neighbours
+          if ((lane & (lpr - 1)) == 0) {
+            actw[so + q.r] = 1;
+            wgmw[unit_wg(T, q.r, q.seg, upw)] = 1;
+          }
+          for (int j = lane & (lpr - 1); j < T.dpad; j += lpr) {
+            const int x = nb[j];
+            actw[so + x] = 1;
+            wgmw[unit_wg(T, x, q.seg, upw)] = 1;
+          }
         }
       }
     }
