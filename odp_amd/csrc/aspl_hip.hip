@@ -36,6 +36,8 @@ static const int LEVEL_BLOCK_WORDS = 16;        // default column-block width of
 static const int RES_WB_MAX = 8;                // widest resident block: 256 sources
 static const int BATCH_CLASSES = 5;             // size classes of the batch launch policy
 static const int FRONTIER_MIN_LEVELS = 16;      // frontier-mode levels when the last evaluation needed more
+static const u32 LIST_CAP = 1u << 22;           // unsaturated chunks a level may list (32 MiB)
+static const int SPARSE_GRID = 2048;            // workgroups of a k_sparse launch (8 per CU)
 static const long DIRECT_TREE_MAX = 8192;       // walk-tree nodes per row the automatic direct K allows
 static const long DIRECT_TREE_CAP = 65536;      // ... and any requested K (larger K is lowered)
 static const double MEM_THRESHOLD_MB = 2048.0;  // MATRIX -> MATRIX_SAVING switch (parameter.h:4)
@@ -108,6 +110,16 @@ static bool graphs_env()
   return env != 0;
 }
 
+// capacity of the unsaturated-chunk list (env ODP_HIP_LIST_CAP lowers it; a
+// level that lists more chunks than that is counted by visiting every chunk)
+static u32 list_cap_env()
+{
+  const char* v = getenv("ODP_HIP_LIST_CAP");
+  if (!v) return LIST_CAP;
+  const long c = atol(v);
+  return c < 0 ? 0u : (c > (long)LIST_CAP ? LIST_CAP : (u32)c);
+}
+
 // ---------------------------------------------------------------------------
 // kernel timing: HIP events recorded on the launch stream around launches
 
@@ -141,6 +153,15 @@ static void timed_end(TimedLaunch& t)
   g_pending.push_back(t);
 }
 
+// zero na words and *q (a kernel, so that sequences containing it can live in
+// a hipGraph): the batch's per-graph accumulators and queue head, a list counter
+__global__ void k_clear(u64* a, int na, int* q)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < na) a[i] = 0;
+  if (i == 0) *q = 0;
+}
+
 // ---------------------------------------------------------------------------
 // one graph on the device, the row-neighbour table and its plan
 
@@ -156,6 +177,9 @@ struct LevelRun {
   unsigned char* flags = nullptr;       // per block of the group 4 x segs x n bytes: changed flags, marks
   unsigned char* wgm = nullptr;         // per block of the group 2 x workgroups bytes: workgroup marks
   size_t wgm_bytes = 0;
+  u64* list = nullptr;                  // unsaturated chunks of the level before a count-only one
+  u32* list_n = nullptr;
+  u32 list_cap = 0;
   u64* h_acnt = nullptr;                // processed-unit counts on the host (kernel timing only)
   size_t flag_bytes = 0;
   LevelTask* tasks = nullptr; int* wg_start = nullptr;
@@ -243,7 +267,7 @@ struct Evaluator {
     nbr16_cap = 0;
     hfree(&h_stage); hfree(&h_err);
     dfree(&lr.bufs); dfree(&lr.cnt); dfree(&lr.done); dfree(&lr.tasks); dfree(&lr.wg_start); hfree(&lr.h_cnt);
-    dfree(&lr.flags); hfree(&lr.h_acnt); dfree(&lr.wgm);
+    dfree(&lr.flags); hfree(&lr.h_acnt); dfree(&lr.wgm); dfree(&lr.list); dfree(&lr.list_n);
     dfree(&rr.tasks); dfree(&rr.out); dfree(&rr.qhead); hfree(&rr.h_out);
     lr = LevelRun(); rr = ResRun();
     adj_cap = nbr_cap = stage_cap = 0;
@@ -437,6 +461,8 @@ struct Evaluator {
     r.upw = plan.upw; r.seg_major = plan.seg_major; r.xcd = plan.xcd; r.urow = (lpr_log2 == 6);
     r.batch = plan.batch == 8 || plan.batch == 16 ? plan.batch : 4;
     r.per_task_wg = (units + 4 * r.upw - 1) / (4 * r.upw);
+    if (!r.list) { dalloc(&r.list, (size_t)LIST_CAP); dalloc(&r.list_n, 1); }
+    r.list_cap = list_cap_env();
     const size_t wbytes = (size_t)group * 2 * r.per_task_wg;
     if (wbytes > r.wgm_bytes) {
       dfree(&r.wgm); dalloc(&r.wgm, wbytes); r.wgm_bytes = wbytes;
@@ -610,10 +636,23 @@ struct Evaluator {
     timed_end(tl);
   }
 
-  // gather levels [from, to) of the current group, then the counter and error copies
+  // gather levels [from, to) of the current group, then the counter and error copies.
+  // The count-only level counts just the chunks the level before it listed as
+  // unsaturated (plain kernel variant, when that level is a gather level).
   void lv_levels(int from, int to)
   {
-    for (int L = from; L < to; L++) lv_launch(L, L == lv.spec ? LV_COUNT : LV_STORE | LV_COUNT);
+    LevelRun& r = lr;
+    const bool listed = !lv.fr && lv.spec - 1 >= lv.L1 && r.ntasks < 65536 && n < (1 << 24) && r.kwp < (1 << 24);
+    for (int L = from; L < to; L++) {
+      int mode = LV_STORE | LV_COUNT;
+      if (L == lv.spec) mode = listed ? LV_SPARSE : LV_COUNT;
+      else if (listed && L == lv.spec - 1) {
+        mode |= LV_LIST;
+        hipLaunchKernelGGL(k_clear, dim3(1), dim3(64), 0, st, (u64*)nullptr, 0, (int*)r.list_n);
+        HIPC(hipGetLastError());
+      }
+      lv_launch(L, mode);
+    }
     lv_copies();
   }
 
@@ -621,19 +660,27 @@ struct Evaluator {
   {
     LevelRun& r = lr;
     const int grid = lv.cnt_t * r.per_task_wg;
-    {
-      TimedLaunch tl;
-      timed_begin(tl, T_LEVEL, L, st);
+    const LvList lst{r.list, r.list_n, r.list_cap};
+    TimedLaunch tl;
+    timed_begin(tl, T_LEVEL, L, st);
+    if (mode & LV_SPARSE) {
+      hipLaunchKernelGGL(k_sparse, dim3(std::min(grid, SPARSE_GRID)), dim3(LEVEL_WG), 0, st,
+                         r.tasks + lv.first, lv.cnt_t, L, RING, lst);
+    } else {
+      const bool ls = (mode & LV_LIST) != 0;
       auto kern = lv.fr
-          ? (r.urow ? (r.batch == 16 ? k_level<true, 16, true> : r.batch == 8 ? k_level<true, 8, true> : k_level<true, 4, true>)
-                    : (r.batch == 16 ? k_level<false, 16, true> : r.batch == 8 ? k_level<false, 8, true> : k_level<false, 4, true>))
-          : (r.urow ? (r.batch == 16 ? k_level<true, 16, false> : r.batch == 8 ? k_level<true, 8, false> : k_level<true, 4, false>)
-                    : (r.batch == 16 ? k_level<false, 16, false> : r.batch == 8 ? k_level<false, 8, false> : k_level<false, 4, false>));
+          ? (r.urow ? (r.batch == 16 ? k_level<true, 16, true, false> : r.batch == 8 ? k_level<true, 8, true, false> : k_level<true, 4, true, false>)
+                    : (r.batch == 16 ? k_level<false, 16, true, false> : r.batch == 8 ? k_level<false, 8, true, false> : k_level<false, 4, true, false>))
+          : ls
+          ? (r.urow ? (r.batch == 16 ? k_level<true, 16, false, true> : r.batch == 8 ? k_level<true, 8, false, true> : k_level<true, 4, false, true>)
+                    : (r.batch == 16 ? k_level<false, 16, false, true> : r.batch == 8 ? k_level<false, 8, false, true> : k_level<false, 4, false, true>))
+          : (r.urow ? (r.batch == 16 ? k_level<true, 16, false, false> : r.batch == 8 ? k_level<true, 8, false, false> : k_level<true, 4, false, false>)
+                    : (r.batch == 16 ? k_level<false, 16, false, false> : r.batch == 8 ? k_level<false, 8, false, false> : k_level<false, 4, false, false>));
       hipLaunchKernelGGL(kern, dim3(grid), dim3(LEVEL_WG), 0, st,
-                         r.tasks + lv.first, r.wg_start, lv.cnt_t, L, RING, r.upw, r.xcd, lv.L1, mode);
-      HIPC(hipGetLastError());
-      timed_end(tl);
+                         r.tasks + lv.first, r.wg_start, lv.cnt_t, L, RING, r.upw, r.xcd, lv.L1, mode, lst);
     }
+    HIPC(hipGetLastError());
+    timed_end(tl);
   }
 
   void lv_copies()
@@ -1411,14 +1458,6 @@ extern "C" int ODPX_Batch_load(int ngraphs, const int* nodes, const int* degrees
   return 0;
 }
 
-// zero the per-graph accumulators and the queue head (a kernel, so the
-// sequence can live in a hipGraph)
-__global__ void k_clear(u64* a, int na, int* q)
-{
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < na) a[i] = 0;
-  if (i == 0) *q = 0;
-}
 
 // clear, the persistent resident launch over every small graph, results to the host
 static void batch_enqueue()

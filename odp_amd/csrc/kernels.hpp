@@ -38,6 +38,16 @@ struct __attribute__((aligned(16))) u64x2 { u64 x, y; };
 #define LEVEL_WG 256        // threads per level-kernel workgroup (4 waves)
 #define LV_STORE 1          // k_level mode bits: write the next rows
 #define LV_COUNT 2          //   ... and add the level's counts (both: a normal level)
+#define LV_LIST 4           //   ... and list the chunks left unsaturated (k_level's LS variant)
+#define LV_SPARSE 8         // the count-only level as k_sparse over that list
+
+// chunks (task, row, 16-byte word pair) a level left unsaturated; only those
+// can gain bits at the next level
+struct LvList {
+  u64* e;              // entries: task << 48 | row << 24 | word
+  u32* n;              // entries appended (may exceed cap: then the list is not used)
+  u32 cap;
+};
 
 // ---------------------------------------------------------------------------
 // shared helpers
@@ -291,8 +301,9 @@ __global__ __launch_bounds__(LEVEL_WG) void k_level0(const LevelTask* __restrict
 // per (row, segment), whether the segment changed and how many did.  For a
 // symmetric table (push mode) a changed segment also marks itself and its
 // neighbours' segments for level L+1, so the next level tests one byte per
-// row; otherwise it tests the row's and its neighbours' changed flags (pull).  When fewer than half changed at level L-1 (and L-1 was not
-// the first gathered level, which wrote every row), level L runs in frontier
+// row; otherwise it tests the row's and its neighbours' changed flags (pull).
+// When fewer than half changed at level L-1 (and L-1 was not the first
+// gathered level, which wrote every row), level L runs in frontier
 // mode: a segment whose row and neighbours all stayed unchanged at L-1 cannot
 // change at L, and the buffer level L writes already holds its value (the
 // level L-2 value, equal to L-1's), so it is skipped -- no reads, no write.
@@ -302,12 +313,14 @@ __global__ __launch_bounds__(LEVEL_WG) void k_level0(const LevelTask* __restrict
 // mode: LV_STORE | LV_COUNT for a normal level; LV_COUNT alone for a level
 // expected to be the last one (its rows are never read again, so they are not
 // written); LV_STORE alone to write the rows of such a level after all when
-// it turned out not to be the last (its counts are already in place).
-template <bool UROW, int G, bool FR>
+// it turned out not to be the last (its counts are already in place).  The
+// LS variant (plain mode only) also appends every chunk it leaves unsaturated
+// to lst (LV_LIST), for k_sparse to count the next level from.
+template <bool UROW, int G, bool FR, bool LS>
 __global__ __launch_bounds__(LEVEL_WG) void k_level(const LevelTask* __restrict__ tasks,
                                                     const int* __restrict__ wg_start, int ntasks,
                                                     int level, int ring, int upw, int xcd, int first_level,
-                                                    int mode)
+                                                    int mode, LvList lst)
 {
   int g = blockIdx.x;
   if (xcd) g = remap_xcd(g, gridDim.x);
@@ -362,6 +375,7 @@ __global__ __launch_bounds__(LEVEL_WG) void k_level(const LevelTask* __restrict_
     if (u >= T.units) break;
     UnitGeom q = unit_geom(T, u, lane);
     if (UROW) q.r = __builtin_amdgcn_readfirstlane(q.r);
+    bool unsat = false;
     if (q.r < T.n) {
       const int64_t so = (int64_t)q.seg * T.n;
       const int* nb = T.nbr + (int64_t)q.r * T.dpad;
@@ -412,6 +426,7 @@ __global__ __launch_bounds__(LEVEL_WG) void k_level(const LevelTask* __restrict_
           }
         }
         if (store) *(u64x2*)(nxt + q.r * kwp + q.w) = x;
+        if (LS) unsat = !(x.x == f0 && x.y == f1);
         if (FR) {
           changed = x.x != x0.x || x.y != x0.y;
           pop += (u64)(__popcll(x.x) + __popcll(x.y) - __popcll(x0.x) - __popcll(x0.y));
@@ -444,6 +459,19 @@ __global__ __launch_bounds__(LEVEL_WG) void k_level(const LevelTask* __restrict_
         }
       }
     }
+    if (LS) {
+      // one list slot per unsaturated chunk, one atomic per wave
+      const u64 m = __ballot(unsat);
+      if (m) {
+        u32 b = 0;
+        if (lane == 0) b = atomicAdd(lst.n, (u32)__popcll(m));
+        b = __builtin_amdgcn_readfirstlane(b);
+        if (unsat) {
+          const u32 e = b + (u32)__popcll(m & ((1ULL << lane) - 1));
+          if (e < lst.cap) lst.e[e] = ((u64)t << 48) | ((u64)q.r << 24) | (u64)q.w;
+        }
+      }
+    }
   }
   if (!count) return;
   pop = wave_sum(pop);
@@ -453,6 +481,105 @@ __global__ __launch_bounds__(LEVEL_WG) void k_level(const LevelTask* __restrict_
     nact = wave_sum(nact);
     if (lane == 0 && nch) atomicAdd(&T.ccnt[(int64_t)(level % ring) * T.cstride], nch);
     if (lane == 0 && nact) atomicAdd(&T.acnt[(int64_t)(level % ring) * T.cstride], nact);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// k_sparse: the count of a level expected to be the last one (LV_SPARSE), from
+// the list of chunks the level before it left unsaturated.  A chunk saturated
+// at level L-1 keeps its bits at level L, so the count of level L is that of
+// L-1 plus, over the listed chunks, the bits their neighbours' words add.  As
+// for a count-only k_level launch nothing is written but the counts and the
+// done flags.  When the list overflowed its capacity every chunk of every
+// running task is visited instead.  Any grid size works.
+
+// the bits row r's 16-byte chunk at word w gains at this level (0 if saturated)
+__device__ __forceinline__ int chunk_gain(const LevelTask& T, int level, int r, int w)
+{
+  const u64* __restrict__ cur = (level & 1) ? T.buf1 : T.buf0;
+  const int64_t kwp = T.kwp;
+  const u64 f0 = full_word(T.l, w), f1 = full_word(T.l, w + 1);
+  const u64x2 x0 = *(const u64x2*)(cur + r * kwp + w);
+  if (x0.x == f0 && x0.y == f1) return 0;
+  u64x2 x = x0;
+  const int* nb = T.nbr + (int64_t)r * T.dpad;
+  for (int j = 0; j < T.dpad; j += 4) {            // dpad is a multiple of 4
+    const int4 v = *(const int4*)(nb + j);
+    const u64x2 a = *(const u64x2*)(cur + (int64_t)v.x * kwp + w);
+    const u64x2 b = *(const u64x2*)(cur + (int64_t)v.y * kwp + w);
+    const u64x2 c = *(const u64x2*)(cur + (int64_t)v.z * kwp + w);
+    const u64x2 d = *(const u64x2*)(cur + (int64_t)v.w * kwp + w);
+    x.x |= a.x | b.x | c.x | d.x;
+    x.y |= a.y | b.y | c.y | d.y;
+    if (x.x == f0 && x.y == f1) break;
+  }
+  return __popcll(x.x) + __popcll(x.y) - __popcll(x0.x) - __popcll(x0.y);
+}
+
+// adds each lane's gain to its task's count of the level: one atomic per wave
+// when the lanes with a gain share lane 0's task.  Called by whole waves.
+__device__ __forceinline__ void add_gains(const LevelTask* __restrict__ tasks, int level, int ring,
+                                          int t, int gain)
+{
+  const int t0 = __builtin_amdgcn_readfirstlane(t);
+  if (__all(t == t0 || gain == 0)) {
+    const u64 s = wave_sum((u64)gain);
+    if ((threadIdx.x & 63) == 0 && s) atomicAdd(&tasks[t0].cnt[(int64_t)(level % ring) * tasks[t0].cstride], s);
+  } else if (gain) {
+    atomicAdd(&tasks[t].cnt[(int64_t)(level % ring) * tasks[t].cstride], (u64)gain);
+  }
+}
+
+// the stop test of k_level at this level
+__device__ __forceinline__ bool task_stops(const LevelTask& T, int level, int ring)
+{
+  const u64 c1 = T.cnt[(int64_t)((level - 1) % ring) * T.cstride];
+  return c1 == (u64)T.n * (u64)T.l || (level >= 2 && c1 == T.cnt[(int64_t)((level - 2) % ring) * T.cstride]);
+}
+
+__global__ __launch_bounds__(LEVEL_WG) void k_sparse(const LevelTask* __restrict__ tasks, int ntasks,
+                                                     int level, int ring, LvList lst)
+{
+  const u32 tid = blockIdx.x * blockDim.x + threadIdx.x, nth = gridDim.x * blockDim.x;
+  const int lane = threadIdx.x & 63;
+  // per running task: the count of level L starts from L-1's (slot L was
+  // cleared by level L-1), and slot L+1 is cleared for level L+1
+  for (u32 t = tid; t < (u32)ntasks; t += nth) {
+    const LevelTask& T = tasks[t];
+    if (*T.done) continue;
+    if (task_stops(T, level, ring)) { *T.done = 1; continue; }
+    T.cnt[(int64_t)((level + 1) % ring) * T.cstride] = 0;
+    atomicAdd(&T.cnt[(int64_t)(level % ring) * T.cstride], T.cnt[(int64_t)((level - 1) % ring) * T.cstride]);
+  }
+  const u32 wbase = tid & ~63u;                       // the wave's first thread; loops are wave-uniform
+  const u32 ne = *lst.n;
+  if (ne <= lst.cap) {
+    // A task that stops at this level gains nothing: it stops because level
+    // L-1 reached every pair (nothing listed) or added no bit (no chunk can
+    // gain one now), so its listed chunks need no test.
+    for (u32 b = wbase; b < ne; b += nth) {
+      const u32 e = b + lane;
+      int t = 0, gain = 0;
+      if (e < ne) {
+        const u64 v = lst.e[e];
+        t = (int)(v >> 48);
+        gain = chunk_gain(tasks[t], level, (int)((v >> 24) & 0xffffff), (int)(v & 0xffffff));
+      }
+      add_gains(tasks, level, ring, t, gain);
+    }
+  } else {
+    for (int t = 0; t < ntasks; t++) {
+      const LevelTask& T = tasks[t];
+      if (*T.done || task_stops(T, level, ring)) continue;
+      const int chunks = T.segs << T.lpr_log2;       // 16-byte chunks per row of the block
+      const u64 total = (u64)T.n * (u64)chunks;
+      for (u64 b = wbase; b < total; b += nth) {
+        const u64 i = b + lane;
+        int gain = 0;
+        if (i < total) gain = chunk_gain(T, level, (int)(i / chunks), 2 * (int)(i % chunks));
+        add_gains(tasks, level, ring, t, gain);
+      }
+    }
   }
 }
 

@@ -342,6 +342,60 @@ def test_level_count_guess_mismatch(plan):
         odp.finalize_aspl()
 
 
+def _trimmed(adj, seed, drop):
+    """adj with `drop` random edges removed (both directions): rows compacted,
+    padded with -1, and their degrees."""
+    rng = np.random.default_rng(seed)
+    n, d = adj.shape
+    rows = [list(r) for r in adj]
+    for u in rng.choice(n, drop, replace=False):
+        if len(rows[u]) > 1:
+            v = rows[u][int(rng.integers(len(rows[u])))]
+            if len(rows[v]) > 1 and u != v:
+                rows[u].remove(v)
+                rows[v].remove(u)
+    out = np.full((n, d), -1, np.int32)
+    nd = np.zeros(n, np.int32)
+    for u, r in enumerate(rows):
+        out[u, :len(r)] = r
+        nd[u] = len(r)
+    return out, nd
+
+
+SPARSE_PLANS = [
+    dict(plan=LEV, kw=16, upw=8, seg_major=0, xcd=1, direct=-1),
+    dict(plan=LEV, kw=4, upw=4, seg_major=1, xcd=0, direct=0, batch=8),
+    dict(plan=LEV, kw=64, upw=2, seg_major=0, xcd=1, direct=0, batch=16),
+    dict(plan=LEV, kw=2, upw=2, seg_major=0, xcd=1, group=2, direct=3),
+]
+
+
+@pytest.mark.parametrize("cap", [None, "0", "5"])
+@pytest.mark.parametrize("plan", range(len(SPARSE_PLANS)))
+def test_sparse_last_level(plan, cap, monkeypatch):
+    """An evaluation that expects to end at the level the previous one ended on
+    lists, at the level before, the 16-byte chunks left unsaturated, and counts
+    the last level from those alone (k_sparse); when the list is too small
+    (ODP_HIP_LIST_CAP) it visits every chunk instead.  Repeated evaluations
+    (eager, captured, replayed) of a connected graph, a disconnected one and one
+    with absent edges all give the oracle's results."""
+    if cap is not None:
+        monkeypatch.setenv("ODP_HIP_LIST_CAP", cap)
+    g = odp.generate_regular(1500, 5, 17)
+    half = odp.generate_regular(750, 5, 18)
+    split = np.concatenate([half, half + 750]).astype(np.int32)
+    holes, nd = _trimmed(odp.generate_regular(1500, 5, 19), 7, 300)
+    for adj, deg in ((g, None), (split, None), (holes, nd)):
+        want = oracle.evaluate(adj, nodes=1500, num_degrees=deg)
+        odp.init_aspl_general(1500, 5, deg)
+        try:
+            odp.set_plan(**SPARSE_PLANS[plan])
+            for _ in range(4):
+                assert same(odp.set_aspl(adj), want)
+        finally:
+            odp.finalize_aspl()
+
+
 def test_batch_policy_autotune_and_graph_replay():
     """Every batch launch policy (block width per size class, workgroups per CU)
     gives the oracle's results; the autotuner keeps one of them; the captured
