@@ -315,7 +315,7 @@ def main():
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--plan", default="auto",
                     help="auto (the library's choice) | matrix | matrix-rows | "
-                         "level:<kw>[:upw:seg_major:xcd:group:direct:batch] | resident:<wb>:<threads>")
+                         "level:<kw>[:upw:seg_major:xcd:group:direct:batch:tile] | resident:<wb>:<threads>")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-served", action="store_true", help="skip the serving-engine workload")
@@ -360,7 +360,7 @@ def main():
         odp.set_plan(plan=odp.PLAN_LEVEL, kw=int(f[1]), upw=int(f[2]) if len(f) > 2 else 8,
                      seg_major=int(f[3]) if len(f) > 3 else 0, xcd=int(f[4]) if len(f) > 4 else 1,
                      group=int(f[5]) if len(f) > 5 else 0, direct=int(f[6]) if len(f) > 6 else -1,
-                     batch=int(f[7]) if len(f) > 7 else 0)
+                     batch=int(f[7]) if len(f) > 7 else 0, tile=int(f[8]) if len(f) > 8 else 0)
     elif args.plan.startswith("resident:"):
         f = args.plan.split(":")
         odp.set_plan(plan=odp.PLAN_RESIDENT, wb=int(f[1]), threads=int(f[2]))

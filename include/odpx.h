@@ -52,6 +52,8 @@ typedef struct {
   int direct;     /* level: levels 0..K-1 by direct walk enumeration in one launch:
                      0 = off, -1 = the library picks K, K >= 2 = that K (at most 6)     */
   int batch;      /* level: neighbour rows gathered per batch: 4 (also 0), 8 or 16     */
+  int tile;       /* level, grid graphs (kw = 2): 1 = temporal tiles, several levels per
+                     launch in LDS when the entries are short enough; 0 = off           */
 } ODPX_Plan;
 
 /* Plan of the current evaluator (after Init).  Set: 0 on success, -1 if invalid.

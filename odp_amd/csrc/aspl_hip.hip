@@ -124,7 +124,7 @@ static u32 list_cap_env()
 // kernel timing: HIP events recorded on the launch stream around launches
 
 enum { T_OTHER = 0, T_LEVEL = 1, T_RESIDENT = 2 };
-struct TimedLaunch { hipEvent_t a, b; int kind, level; };
+struct TimedLaunch { hipEvent_t a, b; int kind, level, nlev = 1; };   // nlev: levels the launch runs
 static bool g_stats_on = false;
 static ODPX_Stats g_stats;
 static std::vector<hipEvent_t> g_evpool;
@@ -177,6 +177,8 @@ struct LevelRun {
   unsigned char* flags = nullptr;       // per block of the group 4 x segs x n bytes: changed flags, marks
   unsigned char* wgm = nullptr;         // per block of the group 2 x workgroups bytes: workgroup marks
   size_t wgm_bytes = 0;
+  u32* tq = nullptr;                    // k_tile's unit list (tile mode): lengths, queue head, entries
+  size_t tq_cap = 0;
   u64* list = nullptr;                  // unsaturated chunks of the level before a count-only one
   u32* list_n = nullptr;
   u32 list_cap = 0;
@@ -205,6 +207,11 @@ struct Evaluator {
   const int* ext_adj = nullptr;         // when set, the adjacency is read from this device copy
   int dp16 = 8;
   size_t nbr16_cap = 0;
+  // grid geometry for k_tile (grid graphs whose rows are cells): cell <-> row
+  int *d_xy2row = nullptr, *d_row2xy = nullptr;
+  short* d_toff = nullptr;              // [n][dpad] cell offsets of the table entries
+  size_t toff_cap = 0;
+  int geo_lmax = -1;                    // largest |dx|, |dy| of an adjacency entry; -1: no geometry
   int *h_stage = nullptr, *h_err = nullptr;
   size_t stage_cap = 0;
   ODPX_Plan plan{};
@@ -264,10 +271,11 @@ struct Evaluator {
   {
     drop_graph();
     dfree(&d_adj); dfree(&d_numdeg); dfree(&d_itable); dfree(&d_nbr); dfree(&d_err); dfree(&d_nbr16);
-    nbr16_cap = 0;
+    dfree(&d_xy2row); dfree(&d_row2xy); dfree(&d_toff);
+    nbr16_cap = 0; toff_cap = 0; geo_lmax = -1;
     hfree(&h_stage); hfree(&h_err);
     dfree(&lr.bufs); dfree(&lr.cnt); dfree(&lr.done); dfree(&lr.tasks); dfree(&lr.wg_start); hfree(&lr.h_cnt);
-    dfree(&lr.flags); hfree(&lr.h_acnt); dfree(&lr.wgm); dfree(&lr.list); dfree(&lr.list_n);
+    dfree(&lr.flags); hfree(&lr.h_acnt); dfree(&lr.wgm); dfree(&lr.list); dfree(&lr.list_n); dfree(&lr.tq);
     dfree(&rr.tasks); dfree(&rr.out); dfree(&rr.qhead); hfree(&rr.h_out);
     lr = LevelRun(); rr = ResRun();
     adj_cap = nbr_cap = stage_cap = 0;
@@ -296,6 +304,9 @@ struct Evaluator {
     if (!d_err) { dalloc(&d_err, 1); halloc(&h_err, 1); HIPC(hipMemsetAsync(d_err, 0, sizeof(int), st)); }
     dfree(&d_numdeg);
     dfree(&d_itable);
+    dfree(&d_xy2row); dfree(&d_row2xy);
+    geo_lmax = -1;
+    if (fam == 1 && s == 1) set_geometry(nullptr);    // rows are the cells themselves
     avg_deg = deg;
     if (numdeg_full) {
       dalloc(&d_numdeg, (size_t)n);
@@ -310,6 +321,43 @@ struct Evaluator {
   {
     dalloc(&d_itable, (size_t)n);
     HIPC(hipMemcpy(d_itable, itable, sizeof(int) * n, hipMemcpyHostToDevice));
+    set_geometry(itable);
+  }
+
+  // cell (x, y) = vertex x * height + y sits in row itable[vertex] (or row
+  // vertex without a table); k_tile reads both directions
+  void set_geometry(const int* itable)
+  {
+    std::vector<int> xy(n), rxy(n);
+    for (int v = 0; v < n; v++) {
+      const int r = itable ? itable[v] : v;
+      xy[v] = r;
+      rxy[r] = ((v / height) << 16) | (v % height);
+    }
+    dfree(&d_xy2row); dfree(&d_row2xy);
+    dalloc(&d_xy2row, (size_t)n); dalloc(&d_row2xy, (size_t)n);
+    HIPC(hipMemcpy(d_xy2row, xy.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    HIPC(hipMemcpy(d_row2xy, rxy.data(), sizeof(int) * n, hipMemcpyHostToDevice));
+    const size_t ne = (size_t)n * dpad;
+    if (ne > toff_cap) { dfree(&d_toff); dalloc(&d_toff, ne); toff_cap = ne; }
+  }
+
+  // largest |dx| or |dy| between a base row's cell and an entry's cell (entries
+  // past num_degrees included: a larger value only turns k_tile off)
+  int grid_lmax(const int* adj) const
+  {
+    const int based = n / s, bh = height / 2;
+    int lm = 0;
+    for (int i = 0; i < based; i++) {
+      const int v = s == 4 ? (i / bh) * height + (i % bh) : i;
+      const int vx = v / height, vy = v % height;
+      for (int j = 0; j < degree; j++) {
+        const int u = adj[(size_t)i * degree + j];
+        if (u < 0 || u >= n) continue;
+        lm = std::max(lm, std::max(abs(u / height - vx), abs(u % height - vy)));
+      }
+    }
+    return lm;
   }
 
   void upload(const void* adjacency)
@@ -317,6 +365,7 @@ struct Evaluator {
     trace("upload");
     memcpy(h_stage, adjacency, sizeof(int) * adj_elems());
     HIPC(hipMemcpyAsync(d_adj, h_stage, sizeof(int) * adj_elems(), hipMemcpyHostToDevice, st));
+    if (d_xy2row) geo_lmax = grid_lmax(h_stage);
   }
 
   // ---- plans ----
@@ -328,6 +377,14 @@ struct Evaluator {
     for (int wb = RES_WB_MAX; wb >= 1; wb >>= 1)
       if ((size_t)n * wb * 4 <= LDS_DYN_MAX && n <= res_rpt(wb) * threads) return wb;
     return 0;
+  }
+
+  // k_tile applies to grid graphs whose rows are cells, with table width <= 16
+  // and at least 4 cells per tile (its flags live in the changed-flag array)
+  bool tile_capable() const
+  {
+    const int ntiles = ((width + TILE - 1) / TILE) * ((height + TILE - 1) / TILE);
+    return d_xy2row && dpad <= 16 && 4 * ntiles <= n && !ext_adj;
   }
 
   ODPX_Plan default_plan(int kind) const
@@ -363,6 +420,13 @@ struct Evaluator {
     p.xcd = 1;
     p.group = (kind == KIND_MATRIX) ? 0 : SAVING_GROUP;
     p.direct = -1;
+    if (tile_capable()) {
+      // grid graphs: 2-word blocks and temporal tiles (k_tile); MATRIX_SAVING
+      // keeps the same number of source words resident as with 16-word blocks
+      p.kw = 2;
+      p.tile = 1;
+      if (p.group) p.group = SAVING_GROUP * LEVEL_BLOCK_WORDS / 2;
+    }
     return p;
   }
 
@@ -375,6 +439,7 @@ struct Evaluator {
     }
     if (p.plan == ODPX_PLAN_LEVEL) {
       if (!(p.kw >= 0 && p.upw >= 1 && p.upw <= 64 && p.group >= 0)) return false;
+      if (p.tile != 0 && !(p.tile == 1 && p.kw == 2 && tile_capable())) return false;
       // direct K that does not apply to this graph's blocks falls back to k_level0
       if (p.batch != 0 && p.batch != 4 && p.batch != 8 && p.batch != 16) return false;
       return p.direct == 0 || p.direct == -1 || (p.direct >= 2 && p.direct <= DIRECT_KMAX);
@@ -463,6 +528,10 @@ struct Evaluator {
     r.per_task_wg = (units + 4 * r.upw - 1) / (4 * r.upw);
     if (!r.list) { dalloc(&r.list, (size_t)LIST_CAP); dalloc(&r.list_n, 1); }
     r.list_cap = list_cap_env();
+    if (plan.tile && tile_capable()) {
+      const size_t cap = 3 + (size_t)group * ((width + TILE - 1) / TILE) * ((height + TILE - 1) / TILE);
+      if (cap > r.tq_cap) { dfree(&r.tq); dalloc(&r.tq, cap); r.tq_cap = cap; }
+    }
     const size_t wbytes = (size_t)group * 2 * r.per_task_wg;
     if (wbytes > r.wgm_bytes) {
       dfree(&r.wgm); dalloc(&r.wgm, wbytes); r.wgm_bytes = wbytes;
@@ -575,6 +644,7 @@ struct Evaluator {
       *h_err = 0;
       HIPC(hipMemsetAsync(d_err, 0, sizeof(int), st));
       if (e & 2) ODP_ERROR("num_degrees holds a value outside [0, degree]\n");
+      if (e & 8) ODP_ERROR("libodp_hip: internal error: a grid table entry is longer than measured\n");
       ODP_ERROR("adjacency holds a vertex id outside [0, nodes) within num_degrees\n");
     }
   }
@@ -592,6 +662,7 @@ struct Evaluator {
   struct LvState {
     int K = 0, L1 = 1, nzero = 0, be0 = 2;
     bool fr = false;                    // frontier-mode level kernel (graphs with many levels)
+    int tk = 0;                         // k_tile levels per launch (0: one k_level launch per level)
     int spec = -1;                      // level of the current group launched count-only (expected last)
     bool graphed = false;
     int first = 0, cnt_t = 0;
@@ -642,6 +713,11 @@ struct Evaluator {
   void lv_levels(int from, int to)
   {
     LevelRun& r = lr;
+    if (lv.tk) {
+      for (int L = from; L < to; L += lv.tk) lv_tile(L);
+      lv_copies();
+      return;
+    }
     const bool listed = !lv.fr && lv.spec - 1 >= lv.L1 && r.ntasks < 65536 && n < (1 << 24) && r.kwp < (1 << 24);
     for (int L = from; L < to; L++) {
       int mode = LV_STORE | LV_COUNT;
@@ -654,6 +730,32 @@ struct Evaluator {
       lv_launch(L, mode);
     }
     lv_copies();
+  }
+
+  // levels L .. L+tk-1 of the current group by temporal tiles (k_tile)
+  void lv_tile(int L)
+  {
+    LevelRun& r = lr;
+    const int j = (L - lv.L1) / lv.tk;             // launch number of the group
+    TileArgs g;
+    g.xy2row = d_xy2row; g.toff = d_toff;
+    g.width = width; g.height = height;
+    g.tiles_x = (width + TILE - 1) / TILE; g.tiles_y = (height + TILE - 1) / TILE;
+    g.lmax = std::max(1, geo_lmax); g.k = lv.tk;
+    g.dpar = (lv.L1 + j) & 1;                      // launch 0 reads what level L1 - 1 wrote
+    g.fpar = j & 1; g.first = j == 0; g.xcd = r.xcd;
+    const int units = lv.cnt_t * g.tiles_x * g.tiles_y;
+    auto kern = dpad == 4 ? k_tile<4> : dpad == 8 ? k_tile<8> : dpad == 12 ? k_tile<12> : k_tile<16>;
+    TimedLaunch tl;
+    tl.nlev = lv.tk;
+    timed_begin(tl, T_LEVEL, L, st);
+    if (j == 0) hipLaunchKernelGGL(k_clear, dim3(1), dim3(64), 0, st, (u64*)nullptr, 0, (int*)r.tq);
+    hipLaunchKernelGGL(k_tile_plan, dim3(std::min((units + 255) / 256, 1024)), dim3(256), 0, st,
+                       r.tasks + lv.first, lv.cnt_t, L, RING, g, r.tq);
+    hipLaunchKernelGGL(kern, dim3(std::min(units, C.ncu)), dim3(TILE_THREADS), 0, st, r.tasks + lv.first, L, RING, g,
+                       r.tq);
+    HIPC(hipGetLastError());
+    timed_end(tl);
   }
 
   void lv_launch(int L, int mode)
@@ -702,8 +804,7 @@ struct Evaluator {
     lv.batch_end = lv.first == 0 ? lv.be0 : first_batch_end();
     // the last evaluation ended at level batch_end - 1: expect the same, and
     // launch that level count-only (timed passes keep every level whole)
-    lv.spec = guess_levels > 0 && lv.batch_end == guess_levels && lv.batch_end - 1 >= lv.L1 && !timed()
-                  ? lv.batch_end - 1 : -1;
+    lv.spec = spec_at(lv.batch_end) ? lv.batch_end - 1 : -1;
     if (lv.first == 0 && lv.graphed) {
       // prepare + the first levels + the first batch + the counter copies as one graph
       const int be = lv.batch_end, L1 = lv.L1, key = lv_key(be);
@@ -722,7 +823,25 @@ struct Evaluator {
   // are distinct (the ring holds RING >= MAX_BATCH_LEVELS levels)
   int first_batch_end() const
   {
+    if (lv.tk) return tile_batch_end(lv.L1, guess_levels > 0 ? guess_levels - lv.L1 : 8);
     return std::min(n, std::max(2, std::min(MAX_BATCH_LEVELS, guess_levels > 0 ? guess_levels : 8)));
+  }
+
+  // tile mode: batches are whole launches of lv.tk levels, short enough that
+  // the slots a launch clears ahead (up to tk past the batch) are not slots the
+  // host has yet to read
+  int tile_batch_end(int from, int want) const
+  {
+    const int cap = (MAX_BATCH_LEVELS - 2 * TILE_KMAX) / lv.tk;
+    const int m = std::max(1, std::min(cap, (std::max(want, 1) + lv.tk - 1) / lv.tk));
+    return from + m * lv.tk;
+  }
+
+  // a first batch ending at be ends with a count-only level (the last
+  // evaluation ended there; timed passes and tile mode keep every level whole)
+  bool spec_at(int be) const
+  {
+    return guess_levels > 0 && be == guess_levels && be - 1 >= lv.L1 && !timed() && !lv.tk;
   }
 
   // neighbour table, zeroed rings, done and asymmetry flags; in frontier mode the
@@ -731,6 +850,12 @@ struct Evaluator {
   {
     LevelRun& r = lr;
     prepare(r.cnt, lv.nzero, r.done, r.ntasks + 1);
+    if (lv.tk) {
+      const int64_t total = (int64_t)n * dpad;
+      const int grid = (int)std::min<int64_t>(std::max<int64_t>(1, (total + 255) / 256), 4096);
+      hipLaunchKernelGGL(k_tgeom, dim3(grid), dim3(256), 0, st, d_nbr, n, dpad, d_row2xy, geo_lmax, d_toff, d_err);
+      HIPC(hipGetLastError());
+    }
     if (lv.fr) {
       const int64_t total = (int64_t)n * dpad;
       const int grid = (int)std::min<int64_t>(std::max<int64_t>(1, (total + 255) / 256), 4096);
@@ -743,8 +868,7 @@ struct Evaluator {
   // whether its last level is count-only
   int lv_key(int be) const
   {
-    const bool spec = guess_levels > 0 && be == guess_levels && be - 1 >= lv.L1 && !timed();
-    return 4 * be + (lv.fr ? 1 : 0) + (spec ? 2 : 0);
+    return 16 * (4 * be + (lv.fr ? 1 : 0) + (spec_at(be) ? 2 : 0)) + lv.tk;
   }
 
   void lv_begin()
@@ -754,10 +878,14 @@ struct Evaluator {
     lv.K = r.direct;                      // levels 0..K-1 come from k_direct (0: k_level0 does level 0)
     lv.L1 = lv.K ? lv.K : 1;              // first level the gather kernel runs
     lv.nzero = 3 * RING * r.ntasks;       // the rings (counts, changed, processed) start at zero
+    // temporal tiles when the plan asks and this adjacency's entries are short
+    // enough for at least one level per launch
+    lv.tk = plan.tile && tile_capable() && r.kwp == 2 && geo_lmax >= 0 && geo_lmax <= TILE_HALO
+                ? std::min(TILE_KMAX, TILE_HALO / std::max(1, geo_lmax)) : 0;
     lv.be0 = first_batch_end();
     // frontier mode pays off over long level sequences; with few levels every
     // level is dense and the plain kernel is faster
-    lv.fr = guess_levels == 0 || guess_levels > FRONTIER_MIN_LEVELS;
+    lv.fr = !lv.tk && (guess_levels == 0 || guess_levels > FRONTIER_MIN_LEVELS);
     lv.graphed = r.group == r.ntasks && use_graph(lv_key(lv.be0));
     if (!lv.graphed) lv_prepare();
     lv.sts.assign(r.ntasks, LvTask{1, 0, 0, 0});
@@ -791,7 +919,7 @@ struct Evaluator {
     }
     if (lv.running > 0 && lv.launched < n) {
       const int step = std::min(MAX_BATCH_LEVELS, std::max(lv.launched, 8));
-      lv.batch_end = std::min(n, lv.launched + step);
+      lv.batch_end = lv.tk ? tile_batch_end(lv.launched, step) : std::min(n, lv.launched + step);
       lv_levels(lv.launched, lv.batch_end);
       lv.launched = std::max(lv.launched, lv.batch_end);
       return false;
@@ -841,9 +969,10 @@ struct Evaluator {
         // in frontier mode the active ones; bytes are that share of the full level's
         for (int t = first; t < first + cnt_t; t++)
           if (tl.level <= stv[t].lwork) {
-            const double share = lv.fr ? (double)r.h_acnt[(size_t)(tl.level % RING) * r.ntasks + t] / ((double)n * r.segs)
-                                       : 1.0;
-            bytes += share * (double)n * ((avg_deg + 2.0) * ((r.l[t] + 63) / 64) * 8.0 + 4.0 * avg_deg);
+            const double share = lv.fr || lv.tk
+                                     ? (double)r.h_acnt[(size_t)(tl.level % RING) * r.ntasks + t] / ((double)n * r.segs)
+                                     : 1.0;
+            bytes += share * tl.nlev * (double)n * ((avg_deg + 2.0) * ((r.l[t] + 63) / 64) * 8.0 + 4.0 * avg_deg);
           }
         if (bytes > 0) { g_stats.level_launches++; g_stats.level_ms += ms; g_stats.level_bytes += bytes; }
         else { g_stats.idle_launches++; g_stats.idle_ms += ms; }
@@ -1216,6 +1345,11 @@ extern "C" int ODPX_Swap_device(int a, int b, int c, int d)
     HIPC(hipStreamSynchronize(C.stream));
     return -1;
   }
+  if (v.geo_lmax >= 0) {
+    // a grid's new edges a-c and b-d may be longer than any before (k_tile's halo)
+    auto cheb = [&](int p, int q) { return std::max(abs(p / v.height - q / v.height), abs(p % v.height - q % v.height)); };
+    v.geo_lmax = std::max(v.geo_lmax, std::max(cheb(a, c), cheb(b, d)));
+  }
   return 0;
 }
 
@@ -1247,6 +1381,12 @@ extern "C" int ODPX_Autotune(void* adjacency, int repeats)
         cand.push_back(p);
         if (v.direct_auto(kw == 0 ? W : std::min(kw, W), 1) > 0) { p.direct = -1; cand.push_back(p); }
       }
+  if (v.tile_capable()) {
+    ODPX_Plan p{};
+    p.plan = ODPX_PLAN_LEVEL; p.kw = 2; p.upw = 8; p.xcd = 1; p.direct = -1; p.tile = 1;
+    p.group = E.kind == KIND_MATRIX ? 0 : SAVING_GROUP * LEVEL_BLOCK_WORDS / 2;
+    cand.push_back(p);
+  }
   if (repeats < 1) repeats = 1;
   v.upload(adjacency);
   double best = 1e30;

@@ -584,6 +584,203 @@ __global__ __launch_bounds__(LEVEL_WG) void k_sparse(const LevelTask* __restrict
 }
 
 // ---------------------------------------------------------------------------
+// k_tile: k levels of a grid graph per launch (temporal tiles)
+//
+// The table entries of a grid graph join cells at most lmax apart in x and in
+// y, so a row's value k levels later depends only on the rows within k * lmax
+// cells of it.  A workgroup loads one 16-byte chunk (128 sources, a 2-word
+// column block) of a TILE x TILE tile of cells plus a TILE_HALO-cell ring
+// around it into LDS, runs k <= TILE_HALO / lmax levels there and writes the
+// tile's rows back.  Level i of the launch updates the cells at least
+// (i + 1) * lmax inside the ring: those are the cells later levels read, and
+// every neighbour they read was updated at level i - 1.  Levels alternate
+// between two LDS copies of the region (128 KiB, one barrier per level).
+//
+// A tile is skipped when it cannot change in this launch and its rows in the
+// output buffer are already current: none of its rows changed in the previous
+// launch, and none of its or its 8 neighbour tiles' rows changed at the
+// previous launch's last level (a row changes at level L only if a row it
+// gathers from changed at L-1, so changes spread at most lmax cells per
+// level, and TILE_HALO <= TILE).  Counts are k_level's: level L+i's count is
+// L-1's plus the gains of the tiles' rows up to level L+i.
+#define TILE 32
+#define TILE_HALO 16
+#define TILE_RW 64                                // region side: TILE + 2 * TILE_HALO = one wave
+#define TILE_SLOTS (TILE_RW * TILE_RW)
+#define TILE_THREADS 1024
+#define TILE_PER_THREAD (TILE_SLOTS / TILE_THREADS)
+#define TILE_KMAX 8
+
+struct TileArgs {
+  const int* xy2row;     // cell x * height + y -> row
+  const short* toff;     // [n][dpad]: entry j of row r as the cell offset dx * TILE_RW + dy (0: r itself)
+  int width, height, tiles_x, tiles_y;
+  int lmax;              // largest |dx| or |dy| of an entry (>= 1)
+  int k;                 // levels of this launch, k * lmax <= TILE_HALO
+  int dpar;              // the launch reads buf1 (1) or buf0 (0) and writes the other
+  int fpar;              // parity of the tile flags it writes (it reads the other)
+  int first;             // first launch of the evaluation: every tile runs
+  int xcd;
+};
+
+// cell offsets of the row-neighbour table for k_tile; an entry longer than
+// lmax (which the host measured on the adjacency) is an internal error
+__global__ void k_tgeom(const int* __restrict__ nbr, int n, int dpad, const int* __restrict__ row2xy, int lmax,
+                        short* __restrict__ toff, int* err)
+{
+  const int64_t total = (int64_t)n * dpad;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+    const int a = row2xy[i / dpad], b = row2xy[nbr[i]];
+    int dx = (b >> 16) - (a >> 16), dy = (b & 0xffff) - (a & 0xffff);
+    if (dx > lmax || dx < -lmax || dy > lmax || dy < -lmax) { atomicOr(err, 8); dx = 0; dy = 0; }
+    toff[i] = (short)(dx * TILE_RW + dy);
+  }
+}
+
+// k_tile_plan: per (task, tile) unit of the launch: k_level's stop test, the
+// count slots of levels L..L+k-1 (tile 0 of a task), and the activity test;
+// the active units go to a list that k_tile's workgroups drain.
+// tq: [0..1] list lengths by flag parity, [2] queue head, [3..] the list.
+__global__ void k_tile_plan(const LevelTask* __restrict__ tasks, int ntasks, int level, int ring, TileArgs g,
+                            u32* __restrict__ tq)
+{
+  const int ntiles = g.tiles_x * g.tiles_y;
+  const int gid = blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid == 0) { tq[g.fpar ^ 1] = 0; tq[2] = 0; }    // the next launch's list, this launch's queue
+  for (int u = gid; u < ntasks * ntiles; u += gridDim.x * blockDim.x) {
+    const int t = u / ntiles, tile = u - t * ntiles;
+    const LevelTask& T = tasks[t];
+    const int64_t cs = T.cstride;
+    if (*T.done) continue;
+    const u64 c1 = T.cnt[(int64_t)((level - 1) % ring) * cs];
+    if (c1 == (u64)T.n * (u64)T.l || (level >= 2 && c1 == T.cnt[(int64_t)((level - 2) % ring) * cs])) {
+      if (tile == 0) *T.done = 1;
+      continue;
+    }
+    if (tile == 0) {
+      // levels L..L+k-1 start from L-1's count; the next launch's slots are cleared
+      for (int i = 0; i < g.k; i++) {
+        atomicAdd(&T.cnt[(int64_t)((level + i) % ring) * cs], c1);
+        T.cnt[(int64_t)((level + g.k + i) % ring) * cs] = 0;
+        T.acnt[(int64_t)((level + g.k + i) % ring) * cs] = 0;
+      }
+    }
+    // tile flags of the task: [parity][changed in the launch, changed at its last level][tile]
+    const unsigned char* fr = T.chg0 + (g.fpar ^ 1) * 2 * ntiles;
+    unsigned char* fw = T.chg0 + g.fpar * 2 * ntiles;
+    const int tx = tile / g.tiles_y, ty = tile - tx * g.tiles_y;
+    bool active = g.first != 0 || fr[tile] != 0;
+    for (int ax = max(tx - 1, 0); ax <= min(tx + 1, g.tiles_x - 1) && !active; ax++)
+      for (int ay = max(ty - 1, 0); ay <= min(ty + 1, g.tiles_y - 1) && !active; ay++)
+        active = fr[ntiles + ax * g.tiles_y + ay] != 0;
+    if (active) tq[3 + atomicAdd(&tq[g.fpar], 1u)] = (u32)u;
+    else { fw[tile] = 0; fw[ntiles + tile] = 0; }
+  }
+}
+
+// k_tile: persistent workgroups (one per CU: the two region copies take
+// 128 KiB of LDS) pull units from the list until it is drained
+template <int DP>
+__global__ __launch_bounds__(TILE_THREADS) void k_tile(const LevelTask* __restrict__ tasks, int level, int ring,
+                                                       TileArgs g, u32* __restrict__ tq)
+{
+  __shared__ u64x2 rows[2][TILE_SLOTS];             // the region's chunks, cell (lx, ly) at lx * TILE_RW + ly
+  __shared__ u64 red[TILE_THREADS / 64][TILE_KMAX]; // per wave and level: gains of the tile's rows
+  __shared__ u32 next_unit;
+  const int ntiles = g.tiles_x * g.tiles_y;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const u32 nunits = tq[g.fpar];
+  for (;;) {
+    if (tid == 0) next_unit = atomicAdd(&tq[2], 1u);
+    __syncthreads();
+    const u32 q = __builtin_amdgcn_readfirstlane(next_unit);
+    if (q >= nunits) break;                           // uniform: every thread read the same slot
+    const u32 u = tq[3 + q];
+    const int t = (int)(u / ntiles), tile = (int)(u - (u32)t * ntiles);
+    const LevelTask& T = tasks[t];
+    const int64_t cs = T.cstride;
+    unsigned char* fw = T.chg0 + g.fpar * 2 * ntiles;
+    const int tx = tile / g.tiles_y, ty = tile - tx * g.tiles_y;
+    const u64* __restrict__ in = g.dpar ? T.buf1 : T.buf0;
+    u64* __restrict__ out = g.dpar ? T.buf0 : T.buf1;
+    const int64_t kwp = T.kwp;
+    const int x0 = tx * TILE - TILE_HALO, y0 = ty * TILE - TILE_HALO;
+    // thread tid owns cells (lx, ly) = (wave + 16 m, lane): a wave owns whole region columns
+    int rid[TILE_PER_THREAD];
+    u32 off[TILE_PER_THREAD][DP / 2];
+#pragma unroll
+    for (int m = 0; m < TILE_PER_THREAD; m++) {
+      const int lx = wave + m * (TILE_THREADS / 64), x = x0 + lx, y = y0 + lane;
+      u64x2 v; v.x = 0; v.y = 0;
+      rid[m] = -1;
+#pragma unroll
+      for (int j = 0; j < DP / 2; j++) off[m][j] = 0;
+      if (x >= 0 && x < g.width && y >= 0 && y < g.height) {
+        const int r = g.xy2row[x * g.height + y];
+        rid[m] = r;
+        v = *(const u64x2*)(in + (int64_t)r * kwp);
+        const uint2* o = (const uint2*)(g.toff + (int64_t)r * DP);
+#pragma unroll
+        for (int j = 0; j < DP / 4; j++) { const uint2 w = o[j]; off[m][2 * j] = w.x; off[m][2 * j + 1] = w.y; }
+      }
+      rows[0][lx * TILE_RW + lane] = v;
+    }
+    __syncthreads();
+    const u64 f0 = full_word(T.l, 0), f1 = full_word(T.l, 1);   // a saturated chunk
+    for (int i = 0; i < g.k; i++) {
+      // level i reads copy i & 1 (level i - 1) and writes the other; cells it
+      // leaves out are never read again
+      const u64x2* __restrict__ src = rows[i & 1];
+      u64x2* __restrict__ dst = rows[(i + 1) & 1];
+      const int lo = (i + 1) * g.lmax, hi = TILE_RW - lo;
+      u64 gain = 0;
+#pragma unroll
+      for (int m = 0; m < TILE_PER_THREAD; m++) {
+        const int lx = wave + m * (TILE_THREADS / 64), sl = lx * TILE_RW + lane;
+        if (rid[m] >= 0 && lx >= lo && lx < hi && lane >= lo && lane < hi) {
+          const u64x2 o = src[sl];
+          u64x2 x = o;
+          // a saturated chunk gains nothing: its neighbours are not read
+          if (!(o.x == f0 && o.y == f1))
+#pragma unroll
+          for (int j = 0; j < DP / 2; j++) {
+            const int s0 = sl + (int)(short)(off[m][j] & 0xffff), s1 = sl + (int)(short)(off[m][j] >> 16);
+            if ((unsigned)s0 < TILE_SLOTS) { const u64x2 a = src[s0]; x.x |= a.x; x.y |= a.y; }
+            if ((unsigned)s1 < TILE_SLOTS) { const u64x2 a = src[s1]; x.x |= a.x; x.y |= a.y; }
+          }
+          dst[sl] = x;
+          if (lx >= TILE_HALO && lx < TILE_HALO + TILE && lane >= TILE_HALO && lane < TILE_HALO + TILE)
+            gain += (u64)(__popcll(x.x) + __popcll(x.y) - __popcll(o.x) - __popcll(o.y));
+        }
+      }
+      gain = wave_sum(gain);
+      if (lane == 0) red[wave][i] = gain;
+      __syncthreads();                               // level i is in LDS
+    }
+    // the tile's rows, its counts and flags
+#pragma unroll
+    for (int m = 0; m < TILE_PER_THREAD; m++) {
+      const int lx = wave + m * (TILE_THREADS / 64);
+      if (rid[m] >= 0 && lx >= TILE_HALO && lx < TILE_HALO + TILE && lane >= TILE_HALO && lane < TILE_HALO + TILE)
+        *(u64x2*)(out + (int64_t)rid[m] * kwp) = rows[g.k & 1][lx * TILE_RW + lane];
+    }
+    if (tid < g.k) {
+      u64 cum = 0, last = 0;
+      for (int i = 0; i <= tid; i++)
+        for (int w = 0; w < TILE_THREADS / 64; w++) { cum += red[w][i]; if (i == tid) last += red[w][i]; }
+      if (cum) atomicAdd(&T.cnt[(int64_t)((level + tid) % ring) * cs], cum);
+      if (tid == g.k - 1) {
+        fw[tile] = cum != 0;
+        fw[ntiles + tile] = last != 0;
+        const int cx = min(TILE, g.width - tx * TILE), cy = min(TILE, g.height - ty * TILE);
+        atomicAdd(&T.acnt[(int64_t)(level % ring) * cs], (u64)(cx * cy));   // rows processed (timing)
+      }
+    }
+    __syncthreads();                                 // red and the region are free for the next unit
+  }
+}
+
+// ---------------------------------------------------------------------------
 // direct kernel: the first K levels of a group of column blocks in one launch
 //
 // Row v of level L holds the sources reachable from v by a walk of at most
