@@ -679,54 +679,88 @@ __global__ void k_tile_plan(const LevelTask* __restrict__ tasks, int ntasks, int
 }
 
 // k_tile: persistent workgroups (one per CU: the two region copies take
-// 128 KiB of LDS) pull units from the list until it is drained
+// 128 KiB of LDS) pull units from the list until it is drained.  While a unit
+// runs its levels, the workgroup already claims the next unit and looks up its
+// cells' rows (after levels 0 and 1), so most of the chain of dependent global
+// loads a unit starts with overlaps the previous unit's levels.
+
+// cell (lx, ly) of the region of a tile: thread tid owns (wave + 16 m, lane),
+// a wave whole region columns
+struct TileUnit {
+  int t, tile, tx, ty;
+  __device__ TileUnit(u32 u, int ntiles, int tiles_y)
+  {
+    t = (int)(u / ntiles); tile = (int)(u - (u32)t * ntiles);
+    tx = tile / tiles_y; ty = tile - tx * tiles_y;
+  }
+};
+
 template <int DP>
 __global__ __launch_bounds__(TILE_THREADS) void k_tile(const LevelTask* __restrict__ tasks, int level, int ring,
                                                        TileArgs g, u32* __restrict__ tq)
 {
   __shared__ u64x2 rows[2][TILE_SLOTS];             // the region's chunks, cell (lx, ly) at lx * TILE_RW + ly
   __shared__ u64 red[TILE_THREADS / 64][TILE_KMAX]; // per wave and level: gains of the tile's rows
-  __shared__ u32 next_unit;
+  __shared__ u32 claim_s;
   const int ntiles = g.tiles_x * g.tiles_y;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const u32 nunits = tq[g.fpar];
-  for (;;) {
-    if (tid == 0) next_unit = atomicAdd(&tq[2], 1u);
-    __syncthreads();
-    const u32 q = __builtin_amdgcn_readfirstlane(next_unit);
-    if (q >= nunits) break;                           // uniform: every thread read the same slot
-    const u32 u = tq[3 + q];
-    const int t = (int)(u / ntiles), tile = (int)(u - (u32)t * ntiles);
-    const LevelTask& T = tasks[t];
-    const int64_t cs = T.cstride;
-    unsigned char* fw = T.chg0 + g.fpar * 2 * ntiles;
-    const int tx = tile / g.tiles_y, ty = tile - tx * g.tiles_y;
-    const u64* __restrict__ in = g.dpar ? T.buf1 : T.buf0;
-    u64* __restrict__ out = g.dpar ? T.buf0 : T.buf1;
-    const int64_t kwp = T.kwp;
-    const int x0 = tx * TILE - TILE_HALO, y0 = ty * TILE - TILE_HALO;
-    // thread tid owns cells (lx, ly) = (wave + 16 m, lane): a wave owns whole region columns
-    int rid[TILE_PER_THREAD];
-    u32 off[TILE_PER_THREAD][DP / 2];
+  // rows of the unit's cells (-1: outside the grid) and their chunks
+  auto cell_rows = [&](const TileUnit& U, int* rr) {
 #pragma unroll
     for (int m = 0; m < TILE_PER_THREAD; m++) {
-      const int lx = wave + m * (TILE_THREADS / 64), x = x0 + lx, y = y0 + lane;
-      u64x2 v; v.x = 0; v.y = 0;
-      rid[m] = -1;
+      const int x = U.tx * TILE - TILE_HALO + wave + m * (TILE_THREADS / 64), y = U.ty * TILE - TILE_HALO + lane;
+      rr[m] = (x >= 0 && x < g.width && y >= 0 && y < g.height) ? g.xy2row[x * g.height + y] : -1;
+    }
+  };
+  auto cell_chunks = [&](const TileUnit& U, const int* rr, u64x2* vv) {
+    const LevelTask& T = tasks[U.t];
+    const u64* __restrict__ in = g.dpar ? T.buf1 : T.buf0;
+#pragma unroll
+    for (int m = 0; m < TILE_PER_THREAD; m++) {
+      vv[m].x = 0; vv[m].y = 0;
+      if (rr[m] >= 0) vv[m] = *(const u64x2*)(in + (int64_t)rr[m] * T.kwp);
+    }
+  };
+  if (tid == 0) claim_s = atomicAdd(&tq[2], 1u);
+  __syncthreads();
+  const u32 q0 = __builtin_amdgcn_readfirstlane(claim_s);
+  if (q0 >= nunits) return;                          // uniform
+  u32 u = tq[3 + q0];
+  int rid[TILE_PER_THREAD];
+  cell_rows(TileUnit(u, ntiles, g.tiles_y), rid);
+  for (;;) {
+    u32 claim = 0;
+    if (tid == 0) claim = atomicAdd(&tq[2], 1u);    // the following unit, published after level 0
+    const TileUnit U(u, ntiles, g.tiles_y);
+    const LevelTask& T = tasks[U.t];
+    const int64_t cs = T.cstride;
+    unsigned char* fw = T.chg0 + g.fpar * 2 * ntiles;
+    u64* __restrict__ out = g.dpar ? T.buf0 : T.buf1;
+    const int64_t kwp = T.kwp;
+    u32 off[TILE_PER_THREAD][DP / 2];
+    {
+      u64x2 val[TILE_PER_THREAD];
+      cell_chunks(U, rid, val);
+#pragma unroll
+      for (int m = 0; m < TILE_PER_THREAD; m++) rows[0][(wave + m * (TILE_THREADS / 64)) * TILE_RW + lane] = val[m];
+    }
+#pragma unroll
+    for (int m = 0; m < TILE_PER_THREAD; m++) {
 #pragma unroll
       for (int j = 0; j < DP / 2; j++) off[m][j] = 0;
-      if (x >= 0 && x < g.width && y >= 0 && y < g.height) {
-        const int r = g.xy2row[x * g.height + y];
-        rid[m] = r;
-        v = *(const u64x2*)(in + (int64_t)r * kwp);
-        const uint2* o = (const uint2*)(g.toff + (int64_t)r * DP);
+      if (rid[m] >= 0) {
+        const uint2* o = (const uint2*)(g.toff + (int64_t)rid[m] * DP);
 #pragma unroll
         for (int j = 0; j < DP / 4; j++) { const uint2 w = o[j]; off[m][2 * j] = w.x; off[m][2 * j + 1] = w.y; }
       }
-      rows[0][lx * TILE_RW + lane] = v;
     }
     __syncthreads();
     const u64 f0 = full_word(T.l, 0), f1 = full_word(T.l, 1);   // a saturated chunk
+    u32 qn = nunits, un = 0;
+    int nrid[TILE_PER_THREAD];
+#pragma unroll
+    for (int m = 0; m < TILE_PER_THREAD; m++) nrid[m] = -1;
     for (int i = 0; i < g.k; i++) {
       // level i reads copy i & 1 (level i - 1) and writes the other; cells it
       // leaves out are never read again
@@ -744,9 +778,14 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(const LevelTask* __restri
           if (!(o.x == f0 && o.y == f1))
 #pragma unroll
           for (int j = 0; j < DP / 2; j++) {
-            const int s0 = sl + (int)(short)(off[m][j] & 0xffff), s1 = sl + (int)(short)(off[m][j] >> 16);
-            if ((unsigned)s0 < TILE_SLOTS) { const u64x2 a = src[s0]; x.x |= a.x; x.y |= a.y; }
-            if ((unsigned)s1 < TILE_SLOTS) { const u64x2 a = src[s1]; x.x |= a.x; x.y |= a.y; }
+            // the window keeps every neighbour inside the region; the clamp only
+            // keeps the reads in LDS (branch-free, so the reads issue together)
+            const unsigned s0 = (unsigned)(sl + (int)(short)(off[m][j] & 0xffff));
+            const unsigned s1 = (unsigned)(sl + (int)(short)(off[m][j] >> 16));
+            const u64x2 a = src[min(s0, (unsigned)(TILE_SLOTS - 1))], c = src[min(s1, (unsigned)(TILE_SLOTS - 1))];
+            const u64 ma = s0 < TILE_SLOTS ? ~0ULL : 0ULL, mc = s1 < TILE_SLOTS ? ~0ULL : 0ULL;
+            x.x |= (a.x & ma) | (c.x & mc);
+            x.y |= (a.y & ma) | (c.y & mc);
           }
           dst[sl] = x;
           if (lx >= TILE_HALO && lx < TILE_HALO + TILE && lane >= TILE_HALO && lane < TILE_HALO + TILE)
@@ -755,8 +794,17 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(const LevelTask* __restri
       }
       gain = wave_sum(gain);
       if (lane == 0) red[wave][i] = gain;
+      if (i == 0 && tid == 0) claim_s = claim;
       __syncthreads();                               // level i is in LDS
+      // the next unit's loads, one dependent step per level
+      if (i == 0) {
+        qn = __builtin_amdgcn_readfirstlane(claim_s);
+        if (qn < nunits) un = tq[3 + qn];
+      } else if (i == 1) {
+        if (qn < nunits) cell_rows(TileUnit(un, ntiles, g.tiles_y), nrid);
+      }
     }
+    if (g.k < 2 && qn < nunits) cell_rows(TileUnit(un, ntiles, g.tiles_y), nrid);   // one level: not yet
     // the tile's rows, its counts and flags
 #pragma unroll
     for (int m = 0; m < TILE_PER_THREAD; m++) {
@@ -770,13 +818,17 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(const LevelTask* __restri
         for (int w = 0; w < TILE_THREADS / 64; w++) { cum += red[w][i]; if (i == tid) last += red[w][i]; }
       if (cum) atomicAdd(&T.cnt[(int64_t)((level + tid) % ring) * cs], cum);
       if (tid == g.k - 1) {
-        fw[tile] = cum != 0;
-        fw[ntiles + tile] = last != 0;
-        const int cx = min(TILE, g.width - tx * TILE), cy = min(TILE, g.height - ty * TILE);
+        fw[U.tile] = cum != 0;
+        fw[ntiles + U.tile] = last != 0;
+        const int cx = min(TILE, g.width - U.tx * TILE), cy = min(TILE, g.height - U.ty * TILE);
         atomicAdd(&T.acnt[(int64_t)(level % ring) * cs], (u64)(cx * cy));   // rows processed (timing)
       }
     }
     __syncthreads();                                 // red and the region are free for the next unit
+    if (qn >= nunits) break;                         // uniform
+    u = un;
+#pragma unroll
+    for (int m = 0; m < TILE_PER_THREAD; m++) rid[m] = nrid[m];
   }
 }
 
