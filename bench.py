@@ -9,9 +9,11 @@ evaluations per second.
 
 The timed region runs with kernel timing off (each evaluation replays its
 captured hipGraph).  roofline: the dominant kernel (the gather level, k_level;
-k_resident when the graph's blocks fit LDS) timed in a second pass with HIP events recorded on the library's launch stream
-around every launch; achieved = algorithmic bytes of those launches (rows *
-((d+2) * k * 8 + 4 d) per level, k = 64-bit words per row) / their summed
+k_tile on grid graphs with the tile plan, several levels per launch; k_resident
+when the graph's blocks fit LDS) timed in a second pass with HIP events recorded
+on the library's launch stream around every launch; achieved = algorithmic bytes
+of those launches (rows * ((d+2) * k * 8 + 4 d) per level, k = 64-bit words per
+row; the share of rows a frontier-mode or tile launch processed) / their summed
 duration.
 traffic: measured HBM bytes per launch, from rocprofv3 counter runs (see
 profiles/), or null.
@@ -395,9 +397,10 @@ def main():
     pcie_rate = 3 / (time.perf_counter() - t)
     odp.finalize_aspl()
 
-    # the dominant kernel: the gather level, or the LDS-resident kernel when the plan is resident
-    kern = "k_level" if st["level_launches"] > 0 else "k_resident"
-    pre = "level" if kern == "k_level" else "resident"
+    # the dominant kernel: the gather level (k_tile's launches of several levels
+    # on grid graphs with the tile plan), or the LDS-resident kernel when the plan is resident
+    kern = ("k_tile" if plan.get("tile") else "k_level") if st["level_launches"] > 0 else "k_resident"
+    pre = "resident" if kern == "k_resident" else "level"
     nl = max(1, st[pre + "_launches"])
     level_ms = st[pre + "_ms"] / nl
     achieved = (st[pre + "_bytes"] / nl) / (level_ms * 1e-3) / 1e9 if level_ms > 0 else 0.0
