@@ -598,7 +598,7 @@ struct Evaluator {
     for (int t = 0; t < ntasks; t++) {
       ResTask& T = ht[t];
       T.nbr = d_nbr16; T.n = n; T.dp = dp16; T.src0 = t * b; T.l = std::min(b, nsrc - t * b);
-      T.s = s; T.graph = 0; T.wb = r.wb; T.pad = 0;
+      T.s = s; T.graph = 0; T.wb = r.wb; T.dq = (std::max(degree, 1) + 3) / 4;
     }
     HIPC(hipMemcpy(r.tasks, ht.data(), sizeof(ResTask) * ntasks, hipMemcpyHostToDevice));
     r.ntasks = ntasks;
@@ -1531,7 +1531,7 @@ static void batch_build_tasks()
     for (int t = 0; t < nt; t++) {
       ResTask T;
       T.nbr = B.d_nbr + G.nbr_off; T.n = G.n; T.dp = G.dp; T.src0 = t * b; T.l = std::min(b, G.n - t * b);
-      T.s = 1; T.graph = (int)k; T.wb = wb; T.pad = 0;
+      T.s = 1; T.graph = (int)k; T.wb = wb; T.dq = (std::max(G.degree, 1) + 3) / 4;
       tasks.push_back(T);
       cost.push_back((double)G.n * G.dp * (1.0 + log((double)G.n) / log(std::max(2.0, G.degree - 1.0))));
     }

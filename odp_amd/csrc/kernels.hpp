@@ -1002,7 +1002,7 @@ struct ResTask {
   int s;                       // symmetry factor of the graph
   int graph;                   // index into the per-graph outputs
   int wb;                      // 32-bit words per row of the block: 1, 2, 4 or 8
-  int pad;
+  int dq;                      // groups of 4 ids a row reads: ceil(table width / 4) <= dp / 4
 };
 
 struct ResOut {     // per-graph accumulators
@@ -1113,14 +1113,21 @@ __device__ __forceinline__ void res_run(const ResTask& T, u32* cur, u64 (*s_part
 #pragma unroll
     for (int k = 0; k < WB; k++) row.w[k] = 0;
     row_bit<WB>(row, r - T.src0, T.l);
+    // 8 ids per 16-byte load, then 4 more when the table width leaves them
+    // (the padding past the width is not read)
     const uint4* ids = (const uint4*)T.nbr;     // uniform base, 32-bit offsets
     const u32 off = (u32)r * (u32)(T.dp >> 3);
-    for (int j = 0; j < (T.dp >> 3); j++) {
+    for (int j = 0; j < (T.dq >> 1); j++) {
       const uint4 v = ids[off + j];
       row_bit<WB>(row, (int)(v.x & 0xffff) - T.src0, T.l); row_bit<WB>(row, (int)(v.x >> 16) - T.src0, T.l);
       row_bit<WB>(row, (int)(v.y & 0xffff) - T.src0, T.l); row_bit<WB>(row, (int)(v.y >> 16) - T.src0, T.l);
       row_bit<WB>(row, (int)(v.z & 0xffff) - T.src0, T.l); row_bit<WB>(row, (int)(v.z >> 16) - T.src0, T.l);
       row_bit<WB>(row, (int)(v.w & 0xffff) - T.src0, T.l); row_bit<WB>(row, (int)(v.w >> 16) - T.src0, T.l);
+    }
+    if (T.dq & 1) {
+      const uint2 v = ((const uint2*)T.nbr)[2 * off + (T.dq - 1)];
+      row_bit<WB>(row, (int)(v.x & 0xffff) - T.src0, T.l); row_bit<WB>(row, (int)(v.x >> 16) - T.src0, T.l);
+      row_bit<WB>(row, (int)(v.y & 0xffff) - T.src0, T.l); row_bit<WB>(row, (int)(v.y >> 16) - T.src0, T.l);
     }
     lds_store<WB>(cur + r * WB, row);
 #pragma unroll
@@ -1150,12 +1157,17 @@ __device__ __forceinline__ void res_run(const ResTask& T, u32* cur, u64 (*s_part
         if (!sat) {
           const uint4* ids = (const uint4*)T.nbr;
           const u32 off = (u32)r * (u32)(T.dp >> 3);
-          for (int j = 0; j < (T.dp >> 3); j++) {
+          for (int j = 0; j < (T.dq >> 1); j++) {
             const uint4 v = ids[off + j];
             row_or<WB>(row, cur, v.x & 0xffff); row_or<WB>(row, cur, v.x >> 16);
             row_or<WB>(row, cur, v.y & 0xffff); row_or<WB>(row, cur, v.y >> 16);
             row_or<WB>(row, cur, v.z & 0xffff); row_or<WB>(row, cur, v.z >> 16);
             row_or<WB>(row, cur, v.w & 0xffff); row_or<WB>(row, cur, v.w >> 16);
+          }
+          if (T.dq & 1) {
+            const uint2 v = ((const uint2*)T.nbr)[2 * off + (T.dq - 1)];
+            row_or<WB>(row, cur, v.x & 0xffff); row_or<WB>(row, cur, v.x >> 16);
+            row_or<WB>(row, cur, v.y & 0xffff); row_or<WB>(row, cur, v.y >> 16);
           }
         }
         nx[i] = row;

@@ -182,7 +182,7 @@ static void sv_small_launch(SvSmallSlot& sl)
     for (int t = 0; t * b < c.n; t++) {
       ResTask T;
       T.nbr = c.d_nbr16; T.n = c.n; T.dp = c.dp16; T.src0 = t * b; T.l = std::min(b, c.n - t * b);
-      T.s = 1; T.graph = k; T.wb = c.wb; T.pad = 0;
+      T.s = 1; T.graph = k; T.wb = c.wb; T.dq = (std::max(c.degree, 1) + 3) / 4;
       sl.tasks.push_back(T);
     }
   }
