@@ -58,6 +58,9 @@ WORKLOADS = {
                                   "(16384 sources expanded), MATRIX_SAVING, adjacency resident in HBM"),
     "random1024": ("random", 1024, 6, "single seeded random 6-regular simple graph, n=1024 (CPU-runnable "
                                       "case), adjacency resident in HBM"),
+    "anneal1024": ("anneal", 1024, 6, "simulated annealing of a seeded random 6-regular graph, n=1024, "
+                                      "adjacency on the device; a step is one 2-opt move, its exact evaluation "
+                                      "and the acceptance test (ODPX_Anneal)"),
     "batch1024": ("batch", 1024, 0, "1024 seeded random d-regular graphs, n uniform even in [1024,16384], "
                                     "d in {4,8,12,16,24}, one 2-opt swap each, resident in HBM; a step is one "
                                     "batched evaluation of all of them"),
@@ -308,6 +311,38 @@ def batch_workload(args, desc):
     print(json.dumps(out), flush=True)
 
 
+def anneal_workload(args, desc, n, d):
+    """The annealing loop as the measured step: args.steps moves of one chain
+    (temperature 100 -> 0.2 over the run), after args.warmup moves of another."""
+    adj = odp.generate_regular(n, d, args.seed)
+    os.environ["ODP_ASPL"] = "MATRIX"
+    odp.init_aspl_general(n, d)
+    odp.anneal(adj, max(1, args.warmup), 100.0, 0.2, seed=args.seed + 1)
+    t = time.perf_counter()
+    best, r = odp.anneal(adj, args.steps, 100.0, 0.2, seed=args.seed)
+    el = time.perf_counter() - t
+    check = odp.set_aspl(best)
+    odp.finalize_aspl()
+    assert (check[0], check[1]) == (r["best_diameter"], r["best_sum"])
+    out = {"metric": METRIC, "value": round(r["steps"] / el, 3), "unit": "annealing steps/s", "n_gpus": 1,
+           "steps": r["steps"], "warmup": args.warmup, "ms_per_step": round(1e3 * el / max(1, r["steps"]), 4),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "u64",
+           "data": "synthetic",
+           "config": {"workload": desc, "name": args.workload, "nodes": n, "degree": d, "seed": args.seed,
+                      "accepted": r["accepted"], "best_diameter": r["best_diameter"], "best_sum": r["best_sum"],
+                      "best_aspl": r["best_ASPL"]},
+           "roofline": None}
+    if not args.no_cpu:
+        # one evaluation is most of a step: the reference library's evaluations of the start graph
+        cb, cres = cpu_baseline(adj, dict(nodes=n), "MATRIX")
+        if cb is not None:
+            cb["note"] = "reference evaluations of the start graph, one per annealing step"
+        out["cpu_baseline"] = cb
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -336,6 +371,8 @@ def main():
     gkind, n, d, desc = WORKLOADS[args.workload]
     if gkind == "batch":
         return batch_workload(args, desc)
+    if gkind == "anneal":
+        return anneal_workload(args, desc, n, d)
     t0 = time.perf_counter()
     if gkind == "random":
         adj = odp.generate_regular(n, d, args.seed)

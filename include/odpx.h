@@ -7,6 +7,7 @@
  *  - plan control, autotuning and kernel timing of the ODP_* evaluator;
  *  - device-resident evaluation (the adjacency already in HBM);
  *  - batched evaluation of many graphs of uneven size in one persistent launch;
+ *  - simulated annealing with the adjacency kept on the device;
  *  - a serving engine with continuous batching of evaluation requests (serve.hpp).
  * Every evaluation returns the same integers as ODP_Set_aspl would.
  */
@@ -129,6 +130,29 @@ int  ODPX_Batch_set_policy(const int *wbcap, int per_cu);
 int  ODPX_Batch_autotune(int repeats);
 /* Frees the device buffers the batched and looped paths keep between calls. */
 void ODPX_Batch_release(void);
+
+/* ---- simulated annealing on the device copy ----------------------------- */
+typedef struct {
+  long long steps;            /* moves evaluated                                      */
+  long long accepted;         /* moves the acceptance test kept                       */
+  int    best_diameter;       /* the best graph seen (returned in adjacency)          */
+  long   best_sum;
+  double best_ASPL;
+  int    diameter;            /* the chain's current graph at the end                 */
+  double ASPL;
+  int    optimum;             /* 1: the lower bounds of ODP_Set_lbounds_general reached */
+  double seconds;
+} ODPX_AnnealResult;
+/* The annealing loop of the reference's sample/SA/general.c:149-181 for a general
+ * graph (needs an active ODP_Init_aspl_general(nodes, degree, ...)): ncalcs times,
+ * one random 2-opt move (ODPX_Two_opt, generator seeded by seed) applied to the
+ * device copy, an exact evaluation (as ODP_Set_aspl), the best graph kept, and the
+ * acceptance test of sample/SA/common.c:15-43 (temperature from max_temp to
+ * min_temp geometrically; hill_climbing accepts no worse ASPL) -- a rejected move
+ * is undone on the device.  adjacency int[nodes][degree]: in, the start graph;
+ * out, the best graph.  Stops early at the lower bounds.  0, or -1 if invalid. */
+int ODPX_Anneal(void *adjacency, long long ncalcs, double max_temp, double min_temp, int hill_climbing,
+                int ASPL_priority, uint64_t seed, ODPX_AnnealResult *result);
 
 /* ---- serving: continuous batching of 2-opt evaluation requests --------- */
 /* A chain is one graph (general, s = 1) kept in HBM; a request applies a list

@@ -31,6 +31,17 @@ class Plan(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class AnnealResult(ctypes.Structure):
+    """ODPX_AnnealResult (include/odpx.h)."""
+    _fields_ = [("steps", ctypes.c_longlong), ("accepted", ctypes.c_longlong),
+                ("best_diameter", ctypes.c_int), ("best_sum", ctypes.c_long), ("best_ASPL", ctypes.c_double),
+                ("diameter", ctypes.c_int), ("ASPL", ctypes.c_double), ("optimum", ctypes.c_int),
+                ("seconds", ctypes.c_double)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
 class Stats(ctypes.Structure):
     """ODPX_Stats (include/odpx.h)."""
     _fields_ = [("evals", ctypes.c_longlong), ("eval_ms", ctypes.c_double),
@@ -115,6 +126,7 @@ def load():
     _sig(lib, "ODPX_Upload_adjacency", None, [_P])
     _sig(lib, "ODPX_Set_aspl_resident", None, [ip, ctypes.POINTER(_L), ctypes.POINTER(_D)])
     _sig(lib, "ODPX_Swap_device", _I, [_I, _I, _I, _I])
+    _sig(lib, "ODPX_Anneal", _I, [_P, ctypes.c_longlong, _D, _D, _I, _I, _U64, ctypes.POINTER(AnnealResult)])
     _sig(lib, "ODPX_Stats_enable", None, [_I])
     _sig(lib, "ODPX_Stats_reset", None, [])
     _sig(lib, "ODPX_Stats_get", None, [ctypes.POINTER(Stats)])
@@ -365,6 +377,14 @@ class Rng:
         return int(load().ODPX_Rng_next(_ptr(self.state)))
 
 
+def swap_edges(adjacency, a, b, c, d):
+    """a-b, c-d -> a-c, b-d in place on a full int[n][d] table (ODPX_Swap_edges)."""
+    assert adjacency.dtype == np.int32 and adjacency.flags.c_contiguous
+    n, deg = adjacency.shape
+    if load().ODPX_Swap_edges(n, deg, _ptr(adjacency), a, b, c, d) != 0:
+        raise ValueError("ODPX_Swap_edges: an edge is missing")
+
+
 def two_opt(adjacency, rng):
     """One valid 2-opt move in place on a full int[n][d] table; returns (a, b, c, d)."""
     assert adjacency.dtype == np.int32 and adjacency.flags.c_contiguous
@@ -373,6 +393,16 @@ def two_opt(adjacency, rng):
     if load().ODPX_Two_opt(n, d, _ptr(adjacency), _ptr(rng.state), _ptr(sw)) != 0:
         raise ValueError("ODPX_Two_opt found no valid move")
     return tuple(int(v) for v in sw)
+
+
+def anneal(adjacency, ncalcs, max_temp=100.0, min_temp=0.217147, hill_climbing=False, aspl_priority=False, seed=1):
+    """ODPX_Anneal on the active general Init: returns (best adjacency, result dict)."""
+    a = np.array(adjacency, dtype=np.int32, order="C", copy=True)
+    r = AnnealResult()
+    if load().ODPX_Anneal(_ptr(a), int(ncalcs), float(max_temp), float(min_temp), int(bool(hill_climbing)),
+                          int(bool(aspl_priority)), int(seed), ctypes.byref(r)) != 0:
+        raise ValueError("ODPX_Anneal: invalid arguments or no general s = 1 Init")
+    return a, r.as_dict()
 
 
 def lbounds_general(nodes, degree):

@@ -645,6 +645,7 @@ struct Evaluator {
       HIPC(hipMemsetAsync(d_err, 0, sizeof(int), st));
       if (e & 2) ODP_ERROR("num_degrees holds a value outside [0, degree]\n");
       if (e & 8) ODP_ERROR("libodp_hip: internal error: a grid table entry is longer than measured\n");
+      if (e & 4) ODP_ERROR("libodp_hip: internal error: a device swap found no such edge\n");
       ODP_ERROR("adjacency holds a vertex id outside [0, nodes) within num_degrees\n");
     }
   }
@@ -1349,6 +1350,85 @@ extern "C" int ODPX_Swap_device(int a, int b, int c, int d)
     // a grid's new edges a-c and b-d may be longer than any before (k_tile's halo)
     auto cheb = [&](int p, int q) { return std::max(abs(p / v.height - q / v.height), abs(p % v.height - q % v.height)); };
     v.geo_lmax = std::max(v.geo_lmax, std::max(cheb(a, c), cheb(b, d)));
+  }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// simulated annealing of a general graph, the reference's SA loop
+// (sample/SA/general.c:149-181) with the adjacency kept on the device
+
+// acceptance test of sample/SA/common.c:15-43 with symmetries = 1; the uniform
+// draw comes from the run's generator (the reference calls random())
+static bool sa_accept(int nodes, int cur_d, int d, double cur_a, double a, double temp, bool hill, bool aspl_prio,
+                      uint64_t rng[4])
+{
+  if (hill) return a <= cur_a;
+  if (d < cur_d && !aspl_prio) return true;
+  if (d > cur_d && !aspl_prio) return false;
+  if (a <= cur_a) return true;
+  const double diff = (cur_a - a) * nodes * (nodes - 1);
+  const double u = ((double)(ODPX_Rng_next(rng) >> 11) + 0.5) / 9007199254740992.0;   // (0, 1)
+  return exp(diff / temp) > u;
+}
+
+extern "C" int ODPX_Anneal(void* adjacency, long long ncalcs, double max_temp, double min_temp, int hill_climbing,
+                           int ASPL_priority, uint64_t seed, ODPX_AnnealResult* res)
+{
+  need_active("ODPX_Anneal");
+  Evaluator& v = E.ev;
+  if (v.s != 1 || v.family != 0 || ncalcs < 0 || (!hill_climbing && (max_temp <= 0 || min_temp <= 0))) return -1;
+  const double t0 = wall_time();
+  const int n = v.n, deg = v.degree;
+  int* adj = (int*)adjacency;
+  std::vector<int> cur(adj, adj + (size_t)n * deg);
+  uint64_t rng[4];
+  ODPX_Rng_seed(seed, rng);
+  int diameter; long sum; double ASPL;
+  ODP_Set_aspl(cur.data(), &diameter, &sum, &ASPL);
+  int cur_d = diameter, best_d = diameter;
+  double cur_a = ASPL, best_a = ASPL;
+  long best_sum = sum;
+  int low_d; double low_a;
+  ODP_Set_lbounds_general(n, deg, &low_d, &low_a);
+  long long steps = 0, accepted = 0;
+  bool optimum = diameter == low_d && ASPL == low_a;
+  const double cooling = hill_climbing ? 0.0 : pow(min_temp / max_temp, 1.0 / (double)ncalcs);
+  double temp = max_temp;
+  for (long long i = 0; i < ncalcs && !optimum; i++) {
+    int sw[4];
+    if (ODPX_Two_opt(n, deg, cur.data(), rng, sw) != 0) break;      // no valid move left
+    // the same move on the device copy (valid: the host copy mirrors it)
+    hipLaunchKernelGGL(k_swap, dim3(1), dim3(64), 0, C.stream, v.d_adj, deg, sw[0], sw[1], sw[2], sw[3], v.d_err);
+    HIPC(hipGetLastError());
+    const double t = wall_time();
+    v.evaluate(&diameter, &sum, &ASPL);
+    E.elapsed += wall_time() - t;
+    finish_result(&diameter, &sum, &ASPL);
+    E.times++;
+    steps++;
+    if (diameter < best_d || (diameter == best_d && ASPL < best_a)) {
+      best_d = diameter; best_sum = sum; best_a = ASPL;
+      memcpy(adj, cur.data(), sizeof(int) * (size_t)n * deg);
+      if (diameter == low_d && ASPL == low_a) { optimum = true; break; }
+    }
+    if (sa_accept(n, cur_d, diameter, cur_a, ASPL, temp, hill_climbing != 0, ASPL_priority != 0, rng)) {
+      cur_d = diameter; cur_a = ASPL;
+      accepted++;
+    } else {
+      // undo a-b, c-d -> a-c, b-d on both copies (the next launch on the stream follows it)
+      ODPX_Swap_edges(n, deg, cur.data(), sw[0], sw[2], sw[1], sw[3]);
+      hipLaunchKernelGGL(k_swap, dim3(1), dim3(64), 0, C.stream, v.d_adj, deg, sw[0], sw[2], sw[1], sw[3], v.d_err);
+      HIPC(hipGetLastError());
+    }
+    temp *= cooling;
+  }
+  HIPC(hipStreamSynchronize(C.stream));
+  if (res) {
+    res->steps = steps; res->accepted = accepted;
+    res->best_diameter = best_d; res->best_sum = best_sum; res->best_ASPL = best_a;
+    res->diameter = cur_d; res->ASPL = cur_a; res->optimum = optimum ? 1 : 0;
+    res->seconds = wall_time() - t0;
   }
   return 0;
 }
