@@ -739,11 +739,15 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(const LevelTask* __restri
     u64* __restrict__ out = g.dpar ? T.buf0 : T.buf1;
     const int64_t kwp = T.kwp;
     u32 off[TILE_PER_THREAD][DP / 2];
+    bool nzv = false;                                // some cell of the thread's is nonzero
     {
       u64x2 val[TILE_PER_THREAD];
       cell_chunks(U, rid, val);
 #pragma unroll
-      for (int m = 0; m < TILE_PER_THREAD; m++) rows[0][(wave + m * (TILE_THREADS / 64)) * TILE_RW + lane] = val[m];
+      for (int m = 0; m < TILE_PER_THREAD; m++) {
+        rows[0][(wave + m * (TILE_THREADS / 64)) * TILE_RW + lane] = val[m];
+        nzv = nzv || (val[m].x | val[m].y) != 0;
+      }
     }
 #pragma unroll
     for (int m = 0; m < TILE_PER_THREAD; m++) {
@@ -755,13 +759,21 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(const LevelTask* __restri
         for (int j = 0; j < DP / 4; j++) { const uint2 w = o[j]; off[m][2 * j] = w.x; off[m][2 * j + 1] = w.y; }
       }
     }
-    __syncthreads();
+    // a region whose cells are all zero stays zero for the k levels: its tile
+    // is written as zeros without running them (most units of a first launch)
+    const bool live = __syncthreads_or(nzv) != 0;
     const u64 f0 = full_word(T.l, 0), f1 = full_word(T.l, 1);   // a saturated chunk
     u32 qn = nunits, un = 0;
     int nrid[TILE_PER_THREAD];
 #pragma unroll
     for (int m = 0; m < TILE_PER_THREAD; m++) nrid[m] = -1;
-    for (int i = 0; i < g.k; i++) {
+    if (!live) {
+      if (tid == 0) claim_s = claim;
+      __syncthreads();
+      qn = __builtin_amdgcn_readfirstlane(claim_s);
+      if (qn < nunits) un = tq[3 + qn];
+    }
+    for (int i = 0; i < (live ? g.k : 0); i++) {
       // level i reads copy i & 1 (level i - 1) and writes the other; cells it
       // leaves out are never read again
       const u64x2* __restrict__ src = rows[i & 1];
@@ -804,17 +816,17 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(const LevelTask* __restri
         if (qn < nunits) cell_rows(TileUnit(un, ntiles, g.tiles_y), nrid);
       }
     }
-    if (g.k < 2 && qn < nunits) cell_rows(TileUnit(un, ntiles, g.tiles_y), nrid);   // one level: not yet
+    if ((!live || g.k < 2) && qn < nunits) cell_rows(TileUnit(un, ntiles, g.tiles_y), nrid);   // not yet
     // the tile's rows, its counts and flags
 #pragma unroll
     for (int m = 0; m < TILE_PER_THREAD; m++) {
       const int lx = wave + m * (TILE_THREADS / 64);
       if (rid[m] >= 0 && lx >= TILE_HALO && lx < TILE_HALO + TILE && lane >= TILE_HALO && lane < TILE_HALO + TILE)
-        *(u64x2*)(out + (int64_t)rid[m] * kwp) = rows[g.k & 1][lx * TILE_RW + lane];
+        *(u64x2*)(out + (int64_t)rid[m] * kwp) = rows[live ? (g.k & 1) : 0][lx * TILE_RW + lane];
     }
     if (tid < g.k) {
       u64 cum = 0, last = 0;
-      for (int i = 0; i <= tid; i++)
+      for (int i = 0; live && i <= tid; i++)
         for (int w = 0; w < TILE_THREADS / 64; w++) { cum += red[w][i]; if (i == tid) last += red[w][i]; }
       if (cum) atomicAdd(&T.cnt[(int64_t)((level + tid) % ring) * cs], cum);
       if (tid == g.k - 1) {
