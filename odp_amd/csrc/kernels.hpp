@@ -739,7 +739,8 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(const LevelTask* __restri
     u64* __restrict__ out = g.dpar ? T.buf0 : T.buf1;
     const int64_t kwp = T.kwp;
     u32 off[TILE_PER_THREAD][DP / 2];
-    bool nzv = false;                                // some cell of the thread's is nonzero
+    const u64 f0 = full_word(T.l, 0), f1 = full_word(T.l, 1);   // a saturated chunk
+    bool nzv = false, usv = false;                   // some cell of the thread's is nonzero / unsaturated
     {
       u64x2 val[TILE_PER_THREAD];
       cell_chunks(U, rid, val);
@@ -747,6 +748,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(const LevelTask* __restri
       for (int m = 0; m < TILE_PER_THREAD; m++) {
         rows[0][(wave + m * (TILE_THREADS / 64)) * TILE_RW + lane] = val[m];
         nzv = nzv || (val[m].x | val[m].y) != 0;
+        usv = usv || (rid[m] >= 0 && !(val[m].x == f0 && val[m].y == f1));
       }
     }
 #pragma unroll
@@ -759,10 +761,11 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(const LevelTask* __restri
         for (int j = 0; j < DP / 4; j++) { const uint2 w = o[j]; off[m][2 * j] = w.x; off[m][2 * j + 1] = w.y; }
       }
     }
-    // a region whose cells are all zero stays zero for the k levels: its tile
-    // is written as zeros without running them (most units of a first launch)
-    const bool live = __syncthreads_or(nzv) != 0;
-    const u64 f0 = full_word(T.l, 0), f1 = full_word(T.l, 1);   // a saturated chunk
+    // a region whose cells are all zero, or all saturated, stays so for the k
+    // levels: its tile is written back as loaded without running them (most
+    // units of a first launch, and tiles the BFS front has just left)
+    const bool any_nz = __syncthreads_or(nzv) != 0;
+    const bool live = any_nz && __syncthreads_or(usv) != 0;
     u32 qn = nunits, un = 0;
     int nrid[TILE_PER_THREAD];
 #pragma unroll
