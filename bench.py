@@ -7,14 +7,15 @@ matrix, 512 MiB per buffer, every column block resident).  Also: grid256 (the
 and ASPL) of the adjacency already resident in HBM.  The JSON line's value is
 evaluations per second.
 
-The timed region runs with kernel timing off (each evaluation replays its
-captured hipGraph).  roofline: the dominant kernel (the gather level, k_level;
-k_tile on grid graphs with the tile plan, several levels per launch; k_resident
-when the graph's blocks fit LDS) timed in a second pass with HIP events recorded
-on the library's launch stream around every launch; achieved = algorithmic bytes
-of those launches (rows * ((d+2) * k * 8 + 4 d) per level, k = 64-bit words per
-row; the share of rows a frontier-mode or tile launch processed) / their summed
-duration.
+The timed region replays each evaluation's captured hipGraph.  The same steps
+then run again with HIP events recorded around every launch on the library's
+launch stream (eager launches, the same launch sequence: count-only, listing and
+sparse last levels as they come).  roofline: the dominant kernel over those
+steps (the gather level, k_level, k_sparse excluded; k_tile on grid graphs with
+the tile plan, several levels per launch; k_resident when the graph's blocks fit
+LDS); achieved = algorithmic bytes of its launches (rows * ((d+2) * k * 8 + 4 d)
+per level, k = 64-bit words per row, d+1 for a count-only level; the share of
+rows a frontier-mode or tile launch processed) / their summed duration.
 traffic: measured HBM bytes per launch, from rocprofv3 counter runs (see
 profiles/), or null.
 
@@ -277,16 +278,17 @@ def batch_workload(args, desc):
     ng = odp.batch_load(odp.pack(graphs))
     odp.batch_autotune(1)
     ref = odp.batch_run(ng)
-    for _ in range(max(1, args.warmup)):
+    for _ in range(max(2, args.warmup)):
         odp.batch_run(ng)
     t = time.perf_counter()
     for _ in range(args.steps):
         r = odp.batch_run(ng)
     el = time.perf_counter() - t
     assert all(x.tobytes() == y.tobytes() for x, y in zip(r, ref))
-    odp.stats_enable(True)
+    # the same runs again with HIP events around the launch (eager)
+    odp.stats_enable(True, live=True)
     odp.stats_reset()
-    for _ in range(min(args.steps, 5)):
+    for _ in range(args.steps):
         odp.batch_run(ng)
     st = odp.stats()
     odp.stats_enable(False)
@@ -410,22 +412,26 @@ def main():
         raise SystemExit(f"result {first} differs from the reference's {want}")
     log(f"first evaluation: diameter={first[0]} sum={first[1]} ASPL={first[2]!r}; plan {plan}")
 
-    for _ in range(args.warmup):
+    for _ in range(max(2, args.warmup)):
         odp.set_aspl_resident()
-    # timed region: kernel timing off, so each evaluation replays its captured hipGraph
+    # timed region: each evaluation replays its captured hipGraph
     t = time.perf_counter()
     for i in range(args.steps):
         r = odp.set_aspl_resident()
         assert r == first
     el = time.perf_counter() - t
-    # kernel timing pass: HIP events around every launch (eager launches, same kernels)
-    odp.stats_enable(True)
+    # the same steps again with HIP events around every launch on the library's
+    # stream: the same launch sequence (count-only, listing and sparse last
+    # levels included), launched eagerly
+    odp.stats_enable(True, live=True)
     odp.stats_reset()
-    for i in range(min(args.steps, 10)):
+    t = time.perf_counter()
+    for i in range(args.steps):
         assert odp.set_aspl_resident() == first
+    el_live = time.perf_counter() - t
     st = odp.stats()
     odp.stats_enable(False)
-    log(f"timed {args.steps} evaluations in {el:.3f}s; kernel stats {st}")
+    log(f"timed {args.steps} evaluations in {el:.3f}s ({el_live:.3f}s with kernel timing); kernel stats {st}")
 
     # PCIe-inclusive rate of the drop-in call (host adjacency every call), for DESIGN.md only
     t = time.perf_counter()
@@ -450,7 +456,8 @@ def main():
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": kern, "launch_ms": round(level_ms, 4),
                 "alg_bytes_per_launch": st[pre + "_bytes"] / nl,
-                "launches": st[pre + "_launches"]}
+                "launches": st[pre + "_launches"],
+                "timed_steps_ms_per_step": round(1e3 * el_live / args.steps, 4)}
 
     out = {"metric": METRIC, "value": round(args.steps / el, 3), "unit": "evals/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * el / args.steps, 4),

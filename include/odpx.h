@@ -91,6 +91,10 @@ typedef struct {
   long long other_launches;   /* prepare + level-0 kernels                         */
   double    other_ms;
 } ODPX_Stats;
+/* on: 0 off; 1 every level whole (no count-only or sparse last level); 2 the
+ * launch sequence of untimed evaluations (count-only, listing and sparse last
+ * levels as they come).  Timed evaluations launch eagerly (no hipGraph replay)
+ * with HIP events around every launch on the library's stream. */
 void ODPX_Stats_enable(int on);
 void ODPX_Stats_reset(void);
 void ODPX_Stats_get(ODPX_Stats *out);

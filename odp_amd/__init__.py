@@ -236,8 +236,11 @@ def autotune(adjacency, repeats=2):
     return load().ODPX_Autotune(_ptr(a), repeats)
 
 
-def stats_enable(on=True):
-    load().ODPX_Stats_enable(1 if on else 0)
+def stats_enable(on=True, live=False):
+    """Kernel timing (HIP events around every launch; timed evaluations launch
+    eagerly): live=False runs every level whole, live=True keeps the launch
+    sequence of untimed evaluations (count-only, listing and sparse last levels)."""
+    load().ODPX_Stats_enable((2 if live else 1) if on else 0)
 
 
 def stats_reset():

@@ -124,8 +124,10 @@ static u32 list_cap_env()
 // kernel timing: HIP events recorded on the launch stream around launches
 
 enum { T_OTHER = 0, T_LEVEL = 1, T_RESIDENT = 2 };
-struct TimedLaunch { hipEvent_t a, b; int kind, level, nlev = 1; };   // nlev: levels the launch runs
+// nlev: levels the launch runs; wr: it writes the rows (0: a count-only level)
+struct TimedLaunch { hipEvent_t a, b; int kind, level, nlev = 1, wr = 1; };
 static bool g_stats_on = false;
+static bool g_stats_live = false;       // timed evaluations keep the untimed launch sequence
 static ODPX_Stats g_stats;
 static std::vector<hipEvent_t> g_evpool;
 static std::vector<TimedLaunch> g_pending;
@@ -765,7 +767,9 @@ struct Evaluator {
     const int grid = lv.cnt_t * r.per_task_wg;
     const LvList lst{r.list, r.list_n, r.list_cap};
     TimedLaunch tl;
-    timed_begin(tl, T_LEVEL, L, st);
+    tl.wr = (mode & LV_STORE) ? 1 : 0;
+    // k_sparse is timed apart: its work is the listed chunks, not a level
+    timed_begin(tl, (mode & LV_SPARSE) ? T_OTHER : T_LEVEL, L, st);
     if (mode & LV_SPARSE) {
       hipLaunchKernelGGL(k_sparse, dim3(std::min(grid, SPARSE_GRID)), dim3(LEVEL_WG), 0, st,
                          r.tasks + lv.first, lv.cnt_t, L, RING, lst);
@@ -842,7 +846,7 @@ struct Evaluator {
   // evaluation ended there; timed passes and tile mode keep every level whole)
   bool spec_at(int be) const
   {
-    return guess_levels > 0 && be == guess_levels && be - 1 >= lv.L1 && !timed() && !lv.tk;
+    return guess_levels > 0 && be == guess_levels && be - 1 >= lv.L1 && !(timed() && !g_stats_live) && !lv.tk;
   }
 
   // neighbour table, zeroed rings, done and asymmetry flags; in frontier mode the
@@ -973,7 +977,7 @@ struct Evaluator {
             const double share = lv.fr || lv.tk
                                      ? (double)r.h_acnt[(size_t)(tl.level % RING) * r.ntasks + t] / ((double)n * r.segs)
                                      : 1.0;
-            bytes += share * tl.nlev * (double)n * ((avg_deg + 2.0) * ((r.l[t] + 63) / 64) * 8.0 + 4.0 * avg_deg);
+            bytes += share * tl.nlev * (double)n * ((avg_deg + 1.0 + tl.wr) * ((r.l[t] + 63) / 64) * 8.0 + 4.0 * avg_deg);
           }
         if (bytes > 0) { g_stats.level_launches++; g_stats.level_ms += ms; g_stats.level_bytes += bytes; }
         else { g_stats.idle_launches++; g_stats.idle_ms += ms; }
@@ -1433,7 +1437,11 @@ extern "C" int ODPX_Anneal(void* adjacency, long long ncalcs, double max_temp, d
   return 0;
 }
 
-extern "C" void ODPX_Stats_enable(int on) { g_stats_on = on != 0; }
+extern "C" void ODPX_Stats_enable(int on)
+{
+  g_stats_on = on != 0;
+  g_stats_live = on == 2;
+}
 extern "C" void ODPX_Stats_reset(void) { memset(&g_stats, 0, sizeof g_stats); }
 extern "C" void ODPX_Stats_get(ODPX_Stats* out) { *out = g_stats; }
 
