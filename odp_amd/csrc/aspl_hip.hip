@@ -81,6 +81,10 @@ static void ctx_init()
   C.ncu = prop.multiProcessorCount;
   HIPC(hipStreamCreateWithFlags(&C.stream, hipStreamNonBlocking));
   HIPC(hipFuncSetAttribute((const void*)k_resident, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_DYN_MAX));
+  HIPC(hipFuncSetAttribute((const void*)k_resident_w<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_DYN_MAX));
+  HIPC(hipFuncSetAttribute((const void*)k_resident_w<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_DYN_MAX));
+  HIPC(hipFuncSetAttribute((const void*)k_resident_w<4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_DYN_MAX));
+  HIPC(hipFuncSetAttribute((const void*)k_resident_w<8>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_DYN_MAX));
   HIPC(hipFuncSetAttribute((const void*)k_direct<true, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_DYN_MAX));
   HIPC(hipFuncSetAttribute((const void*)k_direct<true, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_DYN_MAX));
   HIPC(hipFuncSetAttribute((const void*)k_direct<false, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_DYN_MAX));
@@ -157,11 +161,11 @@ static void timed_end(TimedLaunch& t)
 
 // zero na words and *q (a kernel, so that sequences containing it can live in
 // a hipGraph): the batch's per-graph accumulators and queue head, a list counter
-__global__ void k_clear(u64* a, int na, int* q)
+__global__ void k_clear(u64* a, int na, int* q, int nq)
 {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < na) a[i] = 0;
-  if (i == 0) *q = 0;
+  if (i < nq) q[i] = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -727,7 +731,7 @@ struct Evaluator {
       if (L == lv.spec) mode = listed ? LV_SPARSE : LV_COUNT;
       else if (listed && L == lv.spec - 1) {
         mode |= LV_LIST;
-        hipLaunchKernelGGL(k_clear, dim3(1), dim3(64), 0, st, (u64*)nullptr, 0, (int*)r.list_n);
+        hipLaunchKernelGGL(k_clear, dim3(1), dim3(64), 0, st, (u64*)nullptr, 0, (int*)r.list_n, 1);
         HIPC(hipGetLastError());
       }
       lv_launch(L, mode);
@@ -752,7 +756,7 @@ struct Evaluator {
     TimedLaunch tl;
     tl.nlev = lv.tk;
     timed_begin(tl, T_LEVEL, L, st);
-    if (j == 0) hipLaunchKernelGGL(k_clear, dim3(1), dim3(64), 0, st, (u64*)nullptr, 0, (int*)r.tq);
+    if (j == 0) hipLaunchKernelGGL(k_clear, dim3(1), dim3(64), 0, st, (u64*)nullptr, 0, (int*)r.tq, 1);
     hipLaunchKernelGGL(k_tile_plan, dim3(std::min((units + 255) / 256, 1024)), dim3(256), 0, st,
                        r.tasks + lv.first, lv.cnt_t, L, RING, g, r.tq);
     hipLaunchKernelGGL(kern, dim3(std::min(units, C.ncu)), dim3(TILE_THREADS), 0, st, r.tasks + lv.first, L, RING, g,
@@ -996,7 +1000,8 @@ struct Evaluator {
       prepare((u64*)r.out, (int)(sizeof(ResOut) / 8), (u32*)r.qhead, 1);
       TimedLaunch tl;
       timed_begin(tl, T_RESIDENT, 0, st);
-      hipLaunchKernelGGL(k_resident, dim3(r.grid), dim3(r.threads), r.lds, st, r.tasks, r.ntasks, r.qhead, r.out);
+      auto kern = r.wb == 1 ? k_resident_w<1> : r.wb == 2 ? k_resident_w<2> : r.wb == 4 ? k_resident_w<4> : k_resident_w<8>;
+      hipLaunchKernelGGL(kern, dim3(r.grid), dim3(r.threads), r.lds, st, r.tasks, r.ntasks, r.qhead, r.out);
       HIPC(hipGetLastError());
       timed_end(tl);
       HIPC(hipMemcpyAsync(r.h_out, r.out, sizeof(ResOut), hipMemcpyDeviceToHost, st));
@@ -1536,6 +1541,9 @@ static struct {
   std::vector<int> wb;                  // per small graph
   int ntasks = 0;
   size_t lds = 0;
+  // tasks are grouped by block width (8, 4, 2, 1 words per row), one launch each
+  int grp_start[4] = {0, 0, 0, 0}, grp_n[4] = {0, 0, 0, 0};
+  size_t grp_lds[4] = {0, 0, 0, 0};
   // device buffers (grow-only)
   size_t adj_cap = 0, nbr_cap = 0, g_cap = 0, t_cap = 0;
   int *d_adj = nullptr, *d_err = nullptr, *d_qhead = nullptr, *h_err = nullptr;
@@ -1624,11 +1632,20 @@ static void batch_build_tasks()
       cost.push_back((double)G.n * G.dp * (1.0 + log((double)G.n) / log(std::max(2.0, G.degree - 1.0))));
     }
   }
+  // widest blocks first, then each width's blocks longest first (short queue tails)
   std::vector<int> ord(tasks.size());
   for (size_t i = 0; i < ord.size(); i++) ord[i] = (int)i;
-  std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return cost[a] > cost[b]; });
+  std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) {
+    return tasks[a].wb != tasks[b].wb ? tasks[a].wb > tasks[b].wb : cost[a] > cost[b];
+  });
   std::vector<ResTask> sorted(tasks.size());
   for (size_t i = 0; i < ord.size(); i++) sorted[i] = tasks[ord[i]];
+  for (int g = 0; g < 4; g++) { B.grp_start[g] = 0; B.grp_n[g] = 0; B.grp_lds[g] = 0; }
+  for (size_t i = 0; i < sorted.size(); i++) {
+    const int g = sorted[i].wb == 8 ? 0 : sorted[i].wb == 4 ? 1 : sorted[i].wb == 2 ? 2 : 3;
+    if (B.grp_n[g]++ == 0) B.grp_start[g] = (int)i;
+    B.grp_lds[g] = std::max(B.grp_lds[g], (size_t)sorted[i].n * sorted[i].wb * 4);
+  }
   if (sorted.size() > B.t_cap) { dfree(&B.d_t); dalloc(&B.d_t, sorted.size()); B.t_cap = sorted.size(); }
   B.ntasks = (int)sorted.size();
   HIPC(hipMemcpyAsync(B.d_t, sorted.data(), sizeof(ResTask) * sorted.size(), hipMemcpyHostToDevice, C.stream));
@@ -1654,7 +1671,7 @@ extern "C" int ODPX_Batch_load(int ngraphs, const int* nodes, const int* degrees
     (batch_wb(nodes[g]) > 0 ? B.small : B.large).push_back(g);
   }
   if ((size_t)adj_end > B.adj_cap) { dfree(&B.d_adj); dalloc(&B.d_adj, (size_t)adj_end); B.adj_cap = adj_end; }
-  if (!B.d_err) { dalloc(&B.d_err, 1); dalloc(&B.d_qhead, 1); halloc(&B.h_err, 1); HIPC(hipMemsetAsync(B.d_err, 0, 4, C.stream)); }
+  if (!B.d_err) { dalloc(&B.d_err, 1); dalloc(&B.d_qhead, 4); halloc(&B.h_err, 1); HIPC(hipMemsetAsync(B.d_err, 0, 4, C.stream)); }
   HIPC(hipMemcpyAsync(B.d_adj, packed, sizeof(int) * adj_end, hipMemcpyHostToDevice, C.stream));
   if (!B.small.empty()) {
     long long nbr_tot = 0;
@@ -1691,15 +1708,21 @@ extern "C" int ODPX_Batch_load(int ngraphs, const int* nodes, const int* degrees
 static void batch_enqueue()
 {
   const int na = (int)(B.small.size() * sizeof(ResOut) / 8);
-  hipLaunchKernelGGL(k_clear, dim3((na + 255) / 256), dim3(256), 0, C.stream, (u64*)B.d_out, na, B.d_qhead);
+  hipLaunchKernelGGL(k_clear, dim3((na + 255) / 256), dim3(256), 0, C.stream, (u64*)B.d_out, na, B.d_qhead, 4);
   HIPC(hipGetLastError());
-  int per_cu = (int)std::max<size_t>(1, LDS_LIMIT / (B.lds + LDS_STATIC));
-  per_cu = std::min(per_cu, B.per_cu);
-  const int grid = std::max(1, std::min(B.ntasks, per_cu * C.ncu));
   TimedLaunch tr;
   timed_begin(tr, T_RESIDENT, 0);
-  hipLaunchKernelGGL(k_resident, dim3(grid), dim3(1024), B.lds, C.stream, B.d_t, B.ntasks, B.d_qhead, B.d_out);
-  HIPC(hipGetLastError());
+  for (int g = 0; g < 4; g++) {
+    if (!B.grp_n[g]) continue;
+    // each width's launch holds only its own blocks' LDS: more workgroups per CU
+    int per_cu = (int)std::max<size_t>(1, LDS_LIMIT / (B.grp_lds[g] + LDS_STATIC));
+    per_cu = std::min(per_cu, B.per_cu);
+    const int grid = std::max(1, std::min(B.grp_n[g], per_cu * C.ncu));
+    auto kern = g == 0 ? k_resident_w<8> : g == 1 ? k_resident_w<4> : g == 2 ? k_resident_w<2> : k_resident_w<1>;
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(1024), B.grp_lds[g], C.stream, B.d_t + B.grp_start[g], B.grp_n[g],
+                       B.d_qhead + g, B.d_out);
+    HIPC(hipGetLastError());
+  }
   timed_end(tr);
   HIPC(hipMemcpyAsync(B.h_out, B.d_out, sizeof(ResOut) * B.small.size(), hipMemcpyDeviceToHost, C.stream));
 }

@@ -1208,6 +1208,30 @@ __device__ __forceinline__ void res_run(const ResTask& T, u32* cur, u64 (*s_part
   }
 }
 
+// The same loop for blocks of one width only: its register allocation is that
+// width's alone (k_resident holds all four, and the widest pressure makes every
+// width spill), and a launch can size its LDS to its own blocks.
+template <int WB>
+__global__ __launch_bounds__(1024) void k_resident_w(const ResTask* __restrict__ tasks, int ntasks,
+                                                     int* __restrict__ qhead, ResOut* __restrict__ out)
+{
+  extern __shared__ __attribute__((aligned(16))) u32 lds[];
+  __shared__ int s_task;
+  __shared__ u64 s_part[2][16];
+  const int tid = threadIdx.x, nth = blockDim.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), nw = nth >> 6;
+  for (;;) {
+    if (wave == 0) {       // wave-uniform: lane 0 adds 1, the others add 0
+      const int got = atomicAdd(qhead, lane == 0 ? 1 : 0);
+      s_task = __builtin_amdgcn_readfirstlane(got);
+    }
+    __syncthreads();
+    const int t = __builtin_amdgcn_readfirstlane(s_task);
+    if (t >= ntasks) break;
+    res_run<WB>(tasks[t], lds, s_part, out, tid, nth, lane, wave, nw);
+  }
+}
+
 // Persistent: every workgroup pulls blocks from *qhead until none is left, so
 // the grid drains once the queue is exhausted.  Blocks of any width run in the
 // same launch (T.wb picks the instantiation).  The launch's dynamic LDS must
