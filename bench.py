@@ -12,7 +12,7 @@ then run again with HIP events recorded around every launch on the library's
 launch stream (eager launches, the same launch sequence: count-only, listing and
 sparse last levels as they come).  roofline: the dominant kernel over those
 steps (the gather level, k_level, k_sparse excluded; k_tile on grid graphs with
-the tile plan, several levels per launch; k_resident when the graph's blocks fit
+the tile plan, several levels per launch; k_resident_w when the graph's blocks fit
 LDS); achieved = algorithmic bytes of its launches (rows * ((d+2) * k * 8 + 4 d)
 per level, k = 64-bit words per row, d+1 for a count-only level; the share of
 rows a frontier-mode or tile launch processed) / their summed duration.
@@ -272,7 +272,7 @@ def batched_extra(count, loops):
 
 
 def batch_workload(args, desc):
-    """The batched path as the measured step (for profiling k_resident): one batch
+    """The batched path as the measured step (for profiling k_resident_w): one batch
     run of the 1024 graphs per step, with the tuned launch policy."""
     graphs = c4_graphs(1024)
     ng = odp.batch_load(odp.pack(graphs))
@@ -304,7 +304,7 @@ def batch_workload(args, desc):
            "config": {"workload": desc, "name": args.workload, "graphs": 1024,
                       "policy": {"wb_cap_per_class": policy[0], "workgroups_per_cu": policy[1]}},
            "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_resident",
+                        "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None, "kernel": "k_resident_w",
                         "launch_ms": round(ms, 4), "alg_bytes_per_launch": alg,
                         "launches": st["resident_launches"],
                         "note": "algorithmic bytes are the bit-row operands, which live in LDS: the kernel is "
@@ -442,8 +442,8 @@ def main():
 
     # the dominant kernel: the gather level (k_tile's launches of several levels
     # on grid graphs with the tile plan), or the LDS-resident kernel when the plan is resident
-    kern = ("k_tile" if plan.get("tile") else "k_level") if st["level_launches"] > 0 else "k_resident"
-    pre = "resident" if kern == "k_resident" else "level"
+    kern = ("k_tile" if plan.get("tile") else "k_level") if st["level_launches"] > 0 else "k_resident_w"
+    pre = "resident" if kern == "k_resident_w" else "level"
     nl = max(1, st[pre + "_launches"])
     level_ms = st[pre + "_ms"] / nl
     achieved = (st[pre + "_bytes"] / nl) / (level_ms * 1e-3) / 1e9 if level_ms > 0 else 0.0
