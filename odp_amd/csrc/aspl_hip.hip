@@ -37,6 +37,7 @@ static const int RES_WB_MAX = 8;                // widest resident block: 256 so
 static const int BATCH_CLASSES = 5;             // size classes of the batch launch policy
 static const int FRONTIER_MIN_LEVELS = 16;      // frontier-mode levels when the last evaluation needed more
 static const u32 LIST_CAP = 1u << 22;           // unsaturated chunks a level may list (32 MiB)
+static const int TILE_MIN = 32;                 // smallest tile side of k_tile (halo 16)
 static const int SPARSE_GRID = 2048;            // workgroups of a k_sparse launch (8 per CU)
 static const long DIRECT_TREE_MAX = 8192;       // walk-tree nodes per row the automatic direct K allows
 static const long DIRECT_TREE_CAP = 65536;      // ... and any requested K (larger K is lowered)
@@ -385,11 +386,19 @@ struct Evaluator {
     return 0;
   }
 
+  // k_tile's tile side for this adjacency (env ODP_HIP_TILE_SIDE overrides)
+  int tile_side() const
+  {
+    const char* v = getenv("ODP_HIP_TILE_SIDE");
+    if (v) { const int t = atoi(v); if (t >= TILE_MIN && t <= 56 && t % 2 == 0) return t; }
+    return TILE_MIN;
+  }
+
   // k_tile applies to grid graphs whose rows are cells, with table width <= 16
   // and at least 4 cells per tile (its flags live in the changed-flag array)
   bool tile_capable() const
   {
-    const int ntiles = ((width + TILE - 1) / TILE) * ((height + TILE - 1) / TILE);
+    const int ntiles = ((width + TILE_MIN - 1) / TILE_MIN) * ((height + TILE_MIN - 1) / TILE_MIN);
     return d_xy2row && dpad <= 16 && 4 * ntiles <= n && !ext_adj;
   }
 
@@ -535,7 +544,7 @@ struct Evaluator {
     if (!r.list) { dalloc(&r.list, (size_t)LIST_CAP); dalloc(&r.list_n, 1); }
     r.list_cap = list_cap_env();
     if (plan.tile && tile_capable()) {
-      const size_t cap = 3 + (size_t)group * ((width + TILE - 1) / TILE) * ((height + TILE - 1) / TILE);
+      const size_t cap = 3 + (size_t)group * ((width + TILE_MIN - 1) / TILE_MIN) * ((height + TILE_MIN - 1) / TILE_MIN);
       if (cap > r.tq_cap) { dfree(&r.tq); dalloc(&r.tq, cap); r.tq_cap = cap; }
     }
     const size_t wbytes = (size_t)group * 2 * r.per_task_wg;
@@ -670,6 +679,7 @@ struct Evaluator {
     int K = 0, L1 = 1, nzero = 0, be0 = 2;
     bool fr = false;                    // frontier-mode level kernel (graphs with many levels)
     int tk = 0;                         // k_tile levels per launch (0: one k_level launch per level)
+    int tside = TILE_MIN;               // k_tile's tile side (halo = (64 - side) / 2)
     int spec = -1;                      // level of the current group launched count-only (expected last)
     bool graphed = false;
     int first = 0, cnt_t = 0;
@@ -747,7 +757,8 @@ struct Evaluator {
     TileArgs g;
     g.xy2row = d_xy2row; g.toff = d_toff;
     g.width = width; g.height = height;
-    g.tiles_x = (width + TILE - 1) / TILE; g.tiles_y = (height + TILE - 1) / TILE;
+    g.tile = lv.tside; g.halo = (TILE_RW - lv.tside) / 2;
+    g.tiles_x = (width + g.tile - 1) / g.tile; g.tiles_y = (height + g.tile - 1) / g.tile;
     g.lmax = std::max(1, geo_lmax); g.k = lv.tk;
     g.dpar = (lv.L1 + j) & 1;                      // launch 0 reads what level L1 - 1 wrote
     g.fpar = j & 1; g.first = j == 0; g.xcd = r.xcd;
@@ -877,7 +888,7 @@ struct Evaluator {
   // whether its last level is count-only
   int lv_key(int be) const
   {
-    return 16 * (4 * be + (lv.fr ? 1 : 0) + (spec_at(be) ? 2 : 0)) + lv.tk;
+    return 64 * (16 * (4 * be + (lv.fr ? 1 : 0) + (spec_at(be) ? 2 : 0)) + lv.tk) + lv.tside;
   }
 
   void lv_begin()
@@ -889,8 +900,10 @@ struct Evaluator {
     lv.nzero = 3 * RING * r.ntasks;       // the rings (counts, changed, processed) start at zero
     // temporal tiles when the plan asks and this adjacency's entries are short
     // enough for at least one level per launch
-    lv.tk = plan.tile && tile_capable() && r.kwp == 2 && geo_lmax >= 0 && geo_lmax <= TILE_HALO
-                ? std::min(TILE_KMAX, TILE_HALO / std::max(1, geo_lmax)) : 0;
+    lv.tside = tile_side();
+    const int halo = (TILE_RW - lv.tside) / 2;
+    lv.tk = plan.tile && tile_capable() && r.kwp == 2 && geo_lmax >= 0 && geo_lmax <= halo
+                ? std::min(TILE_KMAX, halo / std::max(1, geo_lmax)) : 0;
     lv.be0 = first_batch_end();
     // frontier mode pays off over long level sequences; with few levels every
     // level is dense and the plain kernel is faster

@@ -589,9 +589,9 @@ __global__ __launch_bounds__(LEVEL_WG) void k_sparse(const LevelTask* __restrict
 // The table entries of a grid graph join cells at most lmax apart in x and in
 // y, so a row's value k levels later depends only on the rows within k * lmax
 // cells of it.  A workgroup loads one 16-byte chunk (128 sources, a 2-word
-// column block) of a TILE x TILE tile of cells plus a TILE_HALO-cell ring
-// around it into LDS, runs k <= TILE_HALO / lmax levels there and writes the
-// tile's rows back.  Level i of the launch updates the cells at least
+// column block) of a square tile of cells plus a ring of halo cells around it
+// (tile + 2 * halo = 64 cells a side) into LDS, runs k <= halo / lmax levels
+// there and writes the tile's rows back.  Level i of the launch updates the cells at least
 // (i + 1) * lmax inside the ring: those are the cells later levels read, and
 // every neighbour they read was updated at level i - 1.  Levels alternate
 // between two LDS copies of the region (128 KiB, one barrier per level).
@@ -601,11 +601,9 @@ __global__ __launch_bounds__(LEVEL_WG) void k_sparse(const LevelTask* __restrict
 // launch, and none of its or its 8 neighbour tiles' rows changed at the
 // previous launch's last level (a row changes at level L only if a row it
 // gathers from changed at L-1, so changes spread at most lmax cells per
-// level, and TILE_HALO <= TILE).  Counts are k_level's: level L+i's count is
+// level, and halo <= tile).  Counts are k_level's: level L+i's count is
 // L-1's plus the gains of the tiles' rows up to level L+i.
-#define TILE 32
-#define TILE_HALO 16
-#define TILE_RW 64                                // region side: TILE + 2 * TILE_HALO = one wave
+#define TILE_RW 64                                // region side: tile + 2 * halo = one wave
 #define TILE_SLOTS (TILE_RW * TILE_RW)
 #define TILE_THREADS 1024
 #define TILE_PER_THREAD (TILE_SLOTS / TILE_THREADS)
@@ -615,8 +613,9 @@ struct TileArgs {
   const int* xy2row;     // cell x * height + y -> row
   const short* toff;     // [n][dpad]: entry j of row r as the cell offset dx * TILE_RW + dy (0: r itself)
   int width, height, tiles_x, tiles_y;
+  int tile, halo;        // tile side and halo width in cells, tile + 2 * halo = TILE_RW, halo <= tile
   int lmax;              // largest |dx| or |dy| of an entry (>= 1)
-  int k;                 // levels of this launch, k * lmax <= TILE_HALO
+  int k;                 // levels of this launch, k * lmax <= halo
   int dpar;              // the launch reads buf1 (1) or buf0 (0) and writes the other
   int fpar;              // parity of the tile flags it writes (it reads the other)
   int first;             // first launch of the evaluation: every tile runs
@@ -709,7 +708,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(const LevelTask* __restri
   auto cell_rows = [&](const TileUnit& U, int* rr) {
 #pragma unroll
     for (int m = 0; m < TILE_PER_THREAD; m++) {
-      const int x = U.tx * TILE - TILE_HALO + wave + m * (TILE_THREADS / 64), y = U.ty * TILE - TILE_HALO + lane;
+      const int x = U.tx * g.tile - g.halo + wave + m * (TILE_THREADS / 64), y = U.ty * g.tile - g.halo + lane;
       rr[m] = (x >= 0 && x < g.width && y >= 0 && y < g.height) ? g.xy2row[x * g.height + y] : -1;
     }
   };
@@ -803,7 +802,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(const LevelTask* __restri
             x.y |= (a.y & ma) | (c.y & mc);
           }
           dst[sl] = x;
-          if (lx >= TILE_HALO && lx < TILE_HALO + TILE && lane >= TILE_HALO && lane < TILE_HALO + TILE)
+          if (lx >= g.halo && lx < g.halo + g.tile && lane >= g.halo && lane < g.halo + g.tile)
             gain += (u64)(__popcll(x.x) + __popcll(x.y) - __popcll(o.x) - __popcll(o.y));
         }
       }
@@ -824,7 +823,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(const LevelTask* __restri
 #pragma unroll
     for (int m = 0; m < TILE_PER_THREAD; m++) {
       const int lx = wave + m * (TILE_THREADS / 64);
-      if (rid[m] >= 0 && lx >= TILE_HALO && lx < TILE_HALO + TILE && lane >= TILE_HALO && lane < TILE_HALO + TILE)
+      if (rid[m] >= 0 && lx >= g.halo && lx < g.halo + g.tile && lane >= g.halo && lane < g.halo + g.tile)
         *(u64x2*)(out + (int64_t)rid[m] * kwp) = rows[live ? (g.k & 1) : 0][lx * TILE_RW + lane];
     }
     if (tid < g.k) {
@@ -835,7 +834,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(const LevelTask* __restri
       if (tid == g.k - 1) {
         fw[U.tile] = cum != 0;
         fw[ntiles + U.tile] = last != 0;
-        const int cx = min(TILE, g.width - U.tx * TILE), cy = min(TILE, g.height - U.ty * TILE);
+        const int cx = min(g.tile, g.width - U.tx * g.tile), cy = min(g.tile, g.height - U.ty * g.tile);
         atomicAdd(&T.acnt[(int64_t)(level % ring) * cs], (u64)(cx * cy));   // rows processed (timing)
       }
     }

@@ -5,6 +5,8 @@ LDS (lmax: the longest table entry in x or y).  Every result must equal the
 oracle's bit for bit, for every k, table width, symmetry, grid shape, group
 size and repeated (captured, replayed) evaluation.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -73,14 +75,18 @@ def test_tile_levels(case):
     full = odp.generate_grid(w, h, d, length, s, 5)
     base = full if s == 1 else odp.grid_base_table(w, h, d, s, full)
     want = oracle.evaluate(base, width=w, height=h, symmetries=s)
-    k = min(8, 16 // lmax_of(full, h))
+    side = int(os.environ.get("ODP_HIP_TILE_SIDE", "32"))      # the library's tile side (halo = (64 - side) / 2)
+    k = min(8, ((64 - side) // 2) // lmax_of(full, h))
     for plan in (TILE_PLAN, dict(TILE_PLAN, direct=0), dict(TILE_PLAN, group=3, xcd=0)):
         launches = run(base, w, h, d, s, plan, want)
         # the gather levels ran as launches of k levels each (plus one launch
         # past the end at most) per group of blocks, not one launch per level
         blocks = -(-(-(-(w * h // s) // 64)) // 2)
         groups = -(-blocks // plan["group"]) if plan.get("group") else 1
-        assert launches <= groups * ((want[0] - 1) // k + 2), (plan, launches, k)
+        if k == 0:          # entries longer than the halo: one level per launch
+            assert launches >= want[0] - 6, (plan, launches)
+        else:
+            assert launches <= groups * ((want[0] - 1) // k + 2), (plan, launches, k)
 
 
 def lattice(w, h, cut=None):
