@@ -129,7 +129,7 @@ def c4_graphs(count, seed=2024):
     return graphs
 
 
-def served_extra(nchains=64, saturate_s=4.0, poisson_s=6.0, load=0.6, seed=77):
+def served_extra(nchains=64, saturate_s=4.0, poisson_s=6.0, load=0.6, seed=77, small_slots=2, large_slots=4, rate=None):
     """64 annealing chains, n in {4096, 16384, 65536} x d in {8, 16, 32}, each request one
     random 2-opt swap (accepted: the next request applies to the swapped graph)."""
     ns, ds = (4096, 16384, 65536), (8, 16, 32)
@@ -137,7 +137,7 @@ def served_extra(nchains=64, saturate_s=4.0, poisson_s=6.0, load=0.6, seed=77):
     mirrors = [odp.generate_regular(ns[i % 3], ds[(i // 3) % 3], 5000 + i) for i in range(nchains)]
     log(f"served: generated {nchains} chains in {time.perf_counter() - t0:.1f}s")
     rng = odp.Rng(seed)
-    srv = odp.Server(small_slots=2, large_slots=4, hbm_budget_gb=64)
+    srv = odp.Server(small_slots=small_slots, large_slots=large_slots, hbm_budget_gb=64)
     samples = []            # (graph copy, request id) checked against the reference later
     want_sample = {2: None, 3: None, 5: None}    # chains 2 (n=65536), 3 (n=4096), 5 (n=65536)
     try:
@@ -175,7 +175,7 @@ def served_extra(nchains=64, saturate_s=4.0, poisson_s=6.0, load=0.6, seed=77):
         # open loop: Poisson arrivals, chains uniform; the mix saturates when its slowest
         # class does, so the rate is `load` of min over classes of capacity / class share
         cap_mix = min(x_large * nchains / max(1, n_large), x_small * nchains / max(1, nchains - n_large))
-        rate = max(1.0, load * cap_mix)
+        rate = max(1.0, load * cap_mix) if rate is None else rate
         arr = np.random.default_rng(seed)
         lat, lat_large, starts = [], [], {}
         t = time.perf_counter()
