@@ -106,6 +106,14 @@ template <typename T> static void halloc(T** p, size_t count)
   if (count) HIPC(hipHostMalloc((void**)p, count * sizeof(T), hipHostMallocDefault));
 }
 template <typename T> static void hfree(T** p) { if (*p) HIPC(hipHostFree(*p)); *p = nullptr; }
+// host memory that kernels write directly (coherent: not cached on the device side)
+template <typename T> static void halloc_coherent(T** p, T** dev, size_t count)
+{
+  *p = nullptr; *dev = nullptr;
+  if (!count) return;
+  HIPC(hipHostMalloc((void**)p, count * sizeof(T), hipHostMallocCoherent));
+  HIPC(hipHostGetDevicePointer((void**)dev, *p, 0));
+}
 
 // hipGraph capture and replay of repeated launch sequences (env ODP_HIP_GRAPH=0 turns it off)
 static bool graphs_env()
@@ -200,7 +208,9 @@ struct ResRun {
   int ntasks = 0, wb = 1, threads = 1024, grid = 0;
   size_t lds = 0, task_cap = 0;
   ResTask* tasks = nullptr; ResOut* out = nullptr; int* qhead = nullptr;
-  ResOut* h_out = nullptr;
+  u32* done = nullptr;                  // the launch's arrival counter (left at zero by each launch)
+  ResHost* host = nullptr;              // pinned coherent host memory the launch writes its result to
+  ResHost* host_dev = nullptr;          // its device address
 };
 
 struct Evaluator {
@@ -221,6 +231,19 @@ struct Evaluator {
   int geo_lmax = -1;                    // largest |dx|, |dy| of an adjacency entry; -1: no geometry
   int *h_stage = nullptr, *h_err = nullptr;
   size_t stage_cap = 0;
+  // the resident plan's 16-bit table matches the adjacency at `table_src`: set by
+  // its preparation, kept by device swaps (they patch the table as well), cleared
+  // by an upload, a new shape or a new plan
+  bool table_fresh = false;
+  const int* table_src = nullptr;
+  // the table a device swap patches along with the adjacency; null (and the
+  // table no longer current) when there is none to keep
+  unsigned short* swap_table()
+  {
+    if (table_fresh && table_src == d_adj && plan.plan == ODPX_PLAN_RESIDENT && s == 1 && !grid_s) return d_nbr16;
+    table_fresh = false;
+    return nullptr;
+  }
   ODPX_Plan plan{};
   LevelRun lr;
   ResRun rr;
@@ -277,13 +300,14 @@ struct Evaluator {
   void release()
   {
     drop_graph();
+    table_fresh = false;
     dfree(&d_adj); dfree(&d_numdeg); dfree(&d_itable); dfree(&d_nbr); dfree(&d_err); dfree(&d_nbr16);
     dfree(&d_xy2row); dfree(&d_row2xy); dfree(&d_toff);
     nbr16_cap = 0; toff_cap = 0; geo_lmax = -1;
     hfree(&h_stage); hfree(&h_err);
     dfree(&lr.bufs); dfree(&lr.cnt); dfree(&lr.done); dfree(&lr.tasks); dfree(&lr.wg_start); hfree(&lr.h_cnt);
     dfree(&lr.flags); hfree(&lr.h_acnt); dfree(&lr.wgm); dfree(&lr.list); dfree(&lr.list_n); dfree(&lr.tq);
-    dfree(&rr.tasks); dfree(&rr.out); dfree(&rr.qhead); hfree(&rr.h_out);
+    dfree(&rr.tasks); dfree(&rr.out); dfree(&rr.qhead); dfree(&rr.done); hfree(&rr.host); rr.host_dev = nullptr;
     lr = LevelRun(); rr = ResRun();
     adj_cap = nbr_cap = stage_cap = 0;
     guess_levels = 0;
@@ -295,6 +319,7 @@ struct Evaluator {
     ctx_init();
     if (!st) st = C.stream;
     drop_graph();
+    table_fresh = false;
     family = fam; n = nodes; degree = deg; s = sym; width = w; height = h; grid_s = gs;
     nsrc = n / s;
     dpad = round_up(deg > 0 ? deg : 1, 4);
@@ -370,6 +395,7 @@ struct Evaluator {
   void upload(const void* adjacency)
   {
     trace("upload");
+    table_fresh = false;
     memcpy(h_stage, adjacency, sizeof(int) * adj_elems());
     HIPC(hipMemcpyAsync(d_adj, h_stage, sizeof(int) * adj_elems(), hipMemcpyHostToDevice, st));
     if (d_xy2row) geo_lmax = grid_lmax(h_stage);
@@ -608,7 +634,15 @@ struct Evaluator {
       dalloc(&r.tasks, (size_t)ntasks);
       r.task_cap = ntasks;
     }
-    if (!r.out) { dalloc(&r.out, 1); dalloc(&r.qhead, 1); halloc(&r.h_out, 1); }
+    if (!r.out) {
+      // zero once here: every launch leaves them at zero for the next (res_finish)
+      dalloc(&r.out, 1); dalloc(&r.qhead, 1); dalloc(&r.done, 1);
+      halloc_coherent(&r.host, &r.host_dev, 1);
+      HIPC(hipMemsetAsync(r.out, 0, sizeof(ResOut), st));
+      HIPC(hipMemsetAsync(r.qhead, 0, sizeof(int), st));
+      HIPC(hipMemsetAsync(r.done, 0, sizeof(u32), st));
+      HIPC(hipStreamSynchronize(st));
+    }
     std::vector<ResTask> ht(ntasks);
     for (int t = 0; t < ntasks; t++) {
       ResTask& T = ht[t];
@@ -626,6 +660,7 @@ struct Evaluator {
   void apply_plan(const ODPX_Plan& p)
   {
     drop_graph();
+    table_fresh = false;
     plan = p;
     if (plan.plan == ODPX_PLAN_LEVEL) setup_level();
     else setup_resident();
@@ -1009,30 +1044,38 @@ struct Evaluator {
   {
     ResRun& r = rr;
     trace("run_resident");
+    // at most two launches (the table is rebuilt only when the adjacency changed
+    // by other means than device swaps), no clears and no copies: the resident kernel's last
+    // workgroup writes the result and the error flag to host memory and
+    // re-zeroes its accumulators, queue head and arrival counter
     auto enqueue = [&]() {
-      prepare((u64*)r.out, (int)(sizeof(ResOut) / 8), (u32*)r.qhead, 1);
+      const int* src = ext_adj ? ext_adj : d_adj;
+      if (!table_fresh || table_src != src) {
+        prepare(nullptr, 0, nullptr, 0);
+        table_fresh = true;
+        table_src = src;
+      }
       TimedLaunch tl;
       timed_begin(tl, T_RESIDENT, 0, st);
       auto kern = r.wb == 1 ? k_resident_w<1> : r.wb == 2 ? k_resident_w<2> : r.wb == 4 ? k_resident_w<4> : k_resident_w<8>;
-      hipLaunchKernelGGL(kern, dim3(r.grid), dim3(r.threads), r.lds, st, r.tasks, r.ntasks, r.qhead, r.out);
+      ResFin fin;
+      fin.done = r.done; fin.host = r.host_dev; fin.err = d_err;
+      hipLaunchKernelGGL(kern, dim3(r.grid), dim3(r.threads), r.lds, st, r.tasks, r.ntasks, r.qhead, r.out, fin);
       HIPC(hipGetLastError());
       timed_end(tl);
-      HIPC(hipMemcpyAsync(r.h_out, r.out, sizeof(ResOut), hipMemcpyDeviceToHost, st));
-      HIPC(hipMemcpyAsync(h_err, d_err, sizeof(int), hipMemcpyDeviceToHost, st));
     };
-    if (use_graph(0)) {
-      if (!graph_ready(0)) capture(enqueue, 0);
-      HIPC(hipGraphLaunch(gexec, st));
-    } else {
-      enqueue();
-    }
+    // launched directly: replaying these two launches as a captured graph took
+    // longer (43.5 against 39.3 us per evaluation of n=1024, d=6)
+    enqueue();
     trace("resident launched");
     HIPC(hipStreamSynchronize(st));
     trace("resident synchronized");
+    const ResHost res = *r.host;      // written by the launch; the stream is synchronized
+    *h_err = res.err;
     check_err();
-    *excess = r.h_out->sum;
-    *diam = std::max(1, r.h_out->diam);
-    *disc = (r.h_out->flags & 1) != 0;
+    *excess = res.out.sum;
+    *diam = std::max(1, res.out.diam);
+    *disc = (res.out.flags & 1) != 0;
     if (timed()) {
       for (TimedLaunch& t : g_pending) {
         float ms = 0;
@@ -1041,7 +1084,7 @@ struct Evaluator {
         if (t.kind == T_RESIDENT) {
           double row = (avg_deg + 2.0) * (std::min(32 * r.wb, nsrc) / 8.0) + 4.0 * avg_deg;
           g_stats.resident_launches++; g_stats.resident_ms += ms;
-          g_stats.resident_bytes += (double)r.h_out->levels * n * row;
+          g_stats.resident_bytes += (double)res.out.levels * n * row;
         } else {
           g_stats.other_launches++; g_stats.other_ms += ms;
         }
@@ -1338,7 +1381,11 @@ extern "C" void ODPX_Set_aspl_resident(int* diameter, long* sum, double* ASPL)
   E.times++;
 }
 
-__global__ void k_swap(int* adj, int degree, int a, int b, int c, int d, int* err)
+// 2-opt swap a-b, c-d -> a-c, b-d on the device adjacency (s = 1).  With
+// `table` set it also patches the resident plan's 16-bit table, entry for entry:
+// the table holds the adjacency's entry where it is a neighbour and the row
+// itself where it is padding, so an entry equal to `from` is the same edge.
+__global__ void k_swap(int* adj, int degree, int a, int b, int c, int d, int* err, unsigned short* table, int tdp)
 {
   const int t = threadIdx.x;
   if (t >= 4) return;
@@ -1347,7 +1394,11 @@ __global__ void k_swap(int* adj, int degree, int a, int b, int c, int d, int* er
   const int to = t == 0 ? c : t == 1 ? d : t == 2 ? a : b;
   int* r = adj + (int64_t)row * degree;
   for (int j = 0; j < degree; j++)
-    if (r[j] == from) { r[j] = to; return; }
+    if (r[j] == from) {
+      r[j] = to;
+      if (table && table[(int64_t)row * tdp + j] == (unsigned short)from) table[(int64_t)row * tdp + j] = (unsigned short)to;
+      return;
+    }
   atomicOr(err, 4);
 }
 
@@ -1358,7 +1409,8 @@ extern "C" int ODPX_Swap_device(int a, int b, int c, int d)
   if (v.s != 1) return -1;
   int ids[4] = {a, b, c, d};
   for (int x : ids) if (x < 0 || x >= v.n) return -1;
-  hipLaunchKernelGGL(k_swap, dim3(1), dim3(64), 0, C.stream, v.d_adj, v.degree, a, b, c, d, v.d_err);
+  hipLaunchKernelGGL(k_swap, dim3(1), dim3(64), 0, C.stream, v.d_adj, v.degree, a, b, c, d, v.d_err, v.swap_table(),
+                     v.dp16);
   HIPC(hipGetLastError());
   HIPC(hipMemcpyAsync(v.h_err, v.d_err, sizeof(int), hipMemcpyDeviceToHost, C.stream));
   HIPC(hipStreamSynchronize(C.stream));
@@ -1421,7 +1473,8 @@ extern "C" int ODPX_Anneal(void* adjacency, long long ncalcs, double max_temp, d
     int sw[4];
     if (ODPX_Two_opt(n, deg, cur.data(), rng, sw) != 0) break;      // no valid move left
     // the same move on the device copy (valid: the host copy mirrors it)
-    hipLaunchKernelGGL(k_swap, dim3(1), dim3(64), 0, C.stream, v.d_adj, deg, sw[0], sw[1], sw[2], sw[3], v.d_err);
+    hipLaunchKernelGGL(k_swap, dim3(1), dim3(64), 0, C.stream, v.d_adj, deg, sw[0], sw[1], sw[2], sw[3], v.d_err,
+                       v.swap_table(), v.dp16);
     HIPC(hipGetLastError());
     const double t = wall_time();
     v.evaluate(&diameter, &sum, &ASPL);
@@ -1440,7 +1493,8 @@ extern "C" int ODPX_Anneal(void* adjacency, long long ncalcs, double max_temp, d
     } else {
       // undo a-b, c-d -> a-c, b-d on both copies (the next launch on the stream follows it)
       ODPX_Swap_edges(n, deg, cur.data(), sw[0], sw[2], sw[1], sw[3]);
-      hipLaunchKernelGGL(k_swap, dim3(1), dim3(64), 0, C.stream, v.d_adj, deg, sw[0], sw[2], sw[1], sw[3], v.d_err);
+      hipLaunchKernelGGL(k_swap, dim3(1), dim3(64), 0, C.stream, v.d_adj, deg, sw[0], sw[2], sw[1], sw[3], v.d_err,
+                         v.swap_table(), v.dp16);
       HIPC(hipGetLastError());
     }
     temp *= cooling;
@@ -1733,7 +1787,7 @@ static void batch_enqueue()
     const int grid = std::max(1, std::min(B.grp_n[g], per_cu * C.ncu));
     auto kern = g == 0 ? k_resident_w<8> : g == 1 ? k_resident_w<4> : g == 2 ? k_resident_w<2> : k_resident_w<1>;
     hipLaunchKernelGGL(kern, dim3(grid), dim3(1024), B.grp_lds[g], C.stream, B.d_t + B.grp_start[g], B.grp_n[g],
-                       B.d_qhead + g, B.d_out);
+                       B.d_qhead + g, B.d_out, ResFin{});
     HIPC(hipGetLastError());
   }
   timed_end(tr);

@@ -1207,12 +1207,52 @@ __device__ __forceinline__ void res_run(const ResTask& T, u32* cur, u64 (*s_part
   }
 }
 
+// Result of a single-graph launch as the host reads it (pinned, coherent host memory).
+struct ResHost {
+  ResOut out;
+  int err;          // the device error flag as it stood at the end of the launch
+  int pad;
+};
+
+// Finishing step of a single-graph launch (out[0] only).  The last workgroup to
+// finish hands the totals and the error flag to host memory and leaves the
+// accumulators, the queue head and the arrival counter at zero for the next
+// launch, so a launch needs no clear before it and no copy after it.
+struct ResFin {
+  u32* done;           // workgroups finished; null: no finishing step (batched launches)
+  ResHost* host;       // device address of the host result
+  const int* err;      // device error flag
+};
+
+__device__ __forceinline__ void res_finish(const ResFin& fin, int* qhead, ResOut* out)
+{
+  __threadfence();                        // this workgroup's result atomics precede its arrival
+  if (atomicAdd(fin.done, 1u) != gridDim.x - 1) return;
+  __threadfence();
+  ResHost h;
+  h.out.sum = __hip_atomic_load(&out->sum, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  h.out.diam = __hip_atomic_load(&out->diam, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  h.out.flags = __hip_atomic_load(&out->flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  h.out.levels = __hip_atomic_load(&out->levels, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  h.out.pad = 0;
+  h.err = __hip_atomic_load(fin.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  h.pad = 0;
+  *fin.host = h;
+  __hip_atomic_store(&out->sum, 0ULL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(&out->diam, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(&out->flags, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(&out->levels, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(qhead, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __hip_atomic_store(fin.done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  __threadfence_system();
+}
+
 // The same loop for blocks of one width only: its register allocation is that
 // width's alone (k_resident holds all four, and the widest pressure makes every
 // width spill), and a launch can size its LDS to its own blocks.
 template <int WB>
 __global__ __launch_bounds__(1024) void k_resident_w(const ResTask* __restrict__ tasks, int ntasks,
-                                                     int* __restrict__ qhead, ResOut* __restrict__ out)
+                                                     int* __restrict__ qhead, ResOut* __restrict__ out, ResFin fin)
 {
   extern __shared__ __attribute__((aligned(16))) u32 lds[];
   __shared__ int s_task;
@@ -1228,6 +1268,10 @@ __global__ __launch_bounds__(1024) void k_resident_w(const ResTask* __restrict__
     const int t = __builtin_amdgcn_readfirstlane(s_task);
     if (t >= ntasks) break;
     res_run<WB>(tasks[t], lds, s_part, out, tid, nth, lane, wave, nw);
+  }
+  if (fin.done) {        // uniform: a kernel argument
+    __syncthreads();     // every wave's result atomics are issued
+    if (tid == 0) res_finish(fin, qhead, out);
   }
 }
 

@@ -239,6 +239,37 @@ def test_device_swap_matches_host():
         odp.finalize_aspl()
 
 
+def test_resident_table_follows_swaps_and_uploads():
+    """The resident plan rebuilds its 16-bit neighbour table only when the
+    adjacency changed by other means than device swaps (a swap patches the table
+    as well): evaluations after swaps, after an upload of another graph, after
+    ODP_Set_aspl and after a plan change must all equal the oracle."""
+    n, d = 1500, 6
+    g1 = odp.generate_regular(n, d, 41)
+    g2 = odp.generate_regular(n, d, 42)
+    rng = odp.Rng(5)
+    odp.init_aspl_general(n, d)
+    try:
+        for plan in (PLANS[1], PLANS[10]):
+            odp.set_plan(**plan)
+            host = g1.copy()
+            odp.upload_adjacency(host)
+            assert same(odp.set_aspl_resident(), oracle.evaluate(host, nodes=n))
+            for _ in range(4):
+                a, b, c, dd = odp.two_opt(host, rng)
+                assert odp.swap_device(a, b, c, dd) == 0
+                assert same(odp.set_aspl_resident(), oracle.evaluate(host, nodes=n))
+            odp.upload_adjacency(g2)
+            assert same(odp.set_aspl_resident(), oracle.evaluate(g2, nodes=n))
+            assert same(odp.set_aspl(host), oracle.evaluate(host, nodes=n))
+            assert same(odp.set_aspl_resident(), oracle.evaluate(host, nodes=n))
+            a, b, c, dd = odp.two_opt(host, rng)
+            assert odp.swap_device(a, b, c, dd) == 0
+            assert same(odp.set_aspl_resident(), oracle.evaluate(host, nodes=n))
+    finally:
+        odp.finalize_aspl()
+
+
 def test_autotune_consistent():
     n, d = 4096, 8
     adj = odp.generate_regular(n, d, 5)
