@@ -209,6 +209,8 @@ struct ResRun {
   size_t lds = 0, task_cap = 0;
   ResTask* tasks = nullptr; ResOut* out = nullptr; int* qhead = nullptr;
   u32* done = nullptr;                  // the launch's arrival counter (left at zero by each launch)
+  ResOut* part = nullptr;               // per-workgroup totals
+  size_t part_cap = 0;
   ResHost* host = nullptr;              // pinned coherent host memory the launch writes its result to
   ResHost* host_dev = nullptr;          // its device address
 };
@@ -308,6 +310,7 @@ struct Evaluator {
     dfree(&lr.bufs); dfree(&lr.cnt); dfree(&lr.done); dfree(&lr.tasks); dfree(&lr.wg_start); hfree(&lr.h_cnt);
     dfree(&lr.flags); hfree(&lr.h_acnt); dfree(&lr.wgm); dfree(&lr.list); dfree(&lr.list_n); dfree(&lr.tq);
     dfree(&rr.tasks); dfree(&rr.out); dfree(&rr.qhead); dfree(&rr.done); hfree(&rr.host); rr.host_dev = nullptr;
+    dfree(&rr.part); rr.part_cap = 0;
     lr = LevelRun(); rr = ResRun();
     adj_cap = nbr_cap = stage_cap = 0;
     guess_levels = 0;
@@ -655,6 +658,7 @@ struct Evaluator {
     int per_cu = (int)std::max<size_t>(1, LDS_LIMIT / (r.lds + LDS_STATIC));
     per_cu = std::min(per_cu, 2048 / r.threads);
     r.grid = std::max(1, std::min(ntasks, per_cu * C.ncu));
+    if ((size_t)r.grid > r.part_cap) { dfree(&r.part); dalloc(&r.part, (size_t)r.grid); r.part_cap = r.grid; }
   }
 
   void apply_plan(const ODPX_Plan& p)
@@ -1045,9 +1049,10 @@ struct Evaluator {
     ResRun& r = rr;
     trace("run_resident");
     // at most two launches (the table is rebuilt only when the adjacency changed
-    // by other means than device swaps), no clears and no copies: the resident kernel's last
-    // workgroup writes the result and the error flag to host memory and
-    // re-zeroes its accumulators, queue head and arrival counter
+    // by other means than device swaps), no clears and no copies: the resident
+    // kernel's last workgroup adds up the workgroups' totals, writes the result
+    // and the error flag to host memory and re-zeroes its queue head and
+    // arrival counter
     auto enqueue = [&]() {
       const int* src = ext_adj ? ext_adj : d_adj;
       if (!table_fresh || table_src != src) {
@@ -1059,7 +1064,7 @@ struct Evaluator {
       timed_begin(tl, T_RESIDENT, 0, st);
       auto kern = r.wb == 1 ? k_resident_w<1> : r.wb == 2 ? k_resident_w<2> : r.wb == 4 ? k_resident_w<4> : k_resident_w<8>;
       ResFin fin;
-      fin.done = r.done; fin.host = r.host_dev; fin.err = d_err;
+      fin.done = r.done; fin.part = r.part; fin.host = r.host_dev; fin.err = d_err;
       hipLaunchKernelGGL(kern, dim3(r.grid), dim3(r.threads), r.lds, st, r.tasks, r.ntasks, r.qhead, r.out, fin);
       HIPC(hipGetLastError());
       timed_end(tl);

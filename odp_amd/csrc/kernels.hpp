@@ -1107,10 +1107,11 @@ __device__ __forceinline__ u64 block_total(u64 v, u64* part, int wave, int nw)
   return uniform_u64(s);
 }
 
-// All levels of one block of T.l <= 32*WB sources.
+// All levels of one block of T.l <= 32*WB sources; returns the block's totals
+// (the same value in every thread).
 template <int WB>
-__device__ __forceinline__ void res_run(const ResTask& T, u32* cur, u64 (*s_part)[16], ResOut* out,
-                                        int tid, int nth, int lane, int wave, int nw)
+__device__ __forceinline__ ResOut res_run(const ResTask& T, u32* cur, u64 (*s_part)[16],
+                                          int tid, int nth, int lane, int wave, int nw)
 {
   constexpr int RPT = res_rpt(WB);
   const u64 target = (u64)T.n * (u64)T.l;
@@ -1198,12 +1199,20 @@ __device__ __forceinline__ void res_run(const ResTask& T, u32* cur, u64 (*s_part
     }
     __syncthreads();                               // the new level is visible
   }
+  ResOut r;
+  r.sum = sum; r.diam = D; r.flags = disc ? 1u : 0u; r.levels = (u32)(L + 1); r.pad = 0;
+  return r;
+}
+
+// adds a block's totals into its graph's accumulators
+__device__ __forceinline__ void res_commit(ResOut* out, int graph, const ResOut& b, int lane, int wave)
+{
   if (wave == 0) {       // wave-uniform branch; only lane 0 contributes
     const bool first = lane == 0;
-    atomicAdd(&out[T.graph].sum, first ? sum : 0ULL);
-    atomicMax(&out[T.graph].diam, D);
-    atomicOr(&out[T.graph].flags, disc ? 1u : 0u);
-    atomicAdd(&out[T.graph].levels, first ? (u32)(L + 1) : 0u);
+    atomicAdd(&out[graph].sum, first ? b.sum : 0ULL);
+    atomicMax(&out[graph].diam, b.diam);
+    atomicOr(&out[graph].flags, b.flags);
+    atomicAdd(&out[graph].levels, first ? b.levels : 0u);
   }
 }
 
@@ -1214,37 +1223,60 @@ struct ResHost {
   int pad;
 };
 
-// Finishing step of a single-graph launch (out[0] only).  The last workgroup to
-// finish hands the totals and the error flag to host memory and leaves the
-// accumulators, the queue head and the arrival counter at zero for the next
-// launch, so a launch needs no clear before it and no copy after it.
+// Finishing step of a single-graph launch.  Each workgroup stores its totals
+// (over the blocks it ran) in its own slot; the last one to arrive adds the
+// slots up, hands the result and the error flag to host memory and leaves the
+// queue head and the arrival counter at zero for the next launch, so a launch
+// needs no clear before it and no copy after it.
 struct ResFin {
   u32* done;           // workgroups finished; null: no finishing step (batched launches)
+  ResOut* part;        // [gridDim.x] per-workgroup totals
   ResHost* host;       // device address of the host result
   const int* err;      // device error flag
 };
 
-__device__ __forceinline__ void res_finish(const ResFin& fin, int* qhead, ResOut* out)
+__device__ __forceinline__ u32 wave_or(u32 v)
 {
-  __threadfence();                        // this workgroup's result atomics precede its arrival
-  if (atomicAdd(fin.done, 1u) != gridDim.x - 1) return;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v |= __shfl_xor(v, o, 64);
+  return v;
+}
+
+__device__ __forceinline__ int wave_max(int v)
+{
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// run by wave 0 of the last workgroup
+__device__ __forceinline__ void res_finish(const ResFin& fin, int* qhead, int lane)
+{
   __threadfence();
-  ResHost h;
-  h.out.sum = __hip_atomic_load(&out->sum, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  h.out.diam = __hip_atomic_load(&out->diam, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  h.out.flags = __hip_atomic_load(&out->flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  h.out.levels = __hip_atomic_load(&out->levels, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  h.out.pad = 0;
-  h.err = __hip_atomic_load(fin.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  h.pad = 0;
-  *fin.host = h;
-  __hip_atomic_store(&out->sum, 0ULL, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  __hip_atomic_store(&out->diam, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  __hip_atomic_store(&out->flags, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  __hip_atomic_store(&out->levels, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  __hip_atomic_store(qhead, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  __hip_atomic_store(fin.done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-  __threadfence_system();
+  u64 sum = 0, levels = 0;
+  u32 flags = 0;
+  int diam = 0;
+  for (int i = lane; i < (int)gridDim.x; i += 64) {
+    const ResOut* p = fin.part + i;
+    sum += __hip_atomic_load(&p->sum, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    diam = max(diam, __hip_atomic_load(&p->diam, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    flags |= __hip_atomic_load(&p->flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    levels += __hip_atomic_load(&p->levels, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  sum = wave_sum(sum);
+  levels = wave_sum(levels);
+  flags = wave_or(flags);
+  diam = wave_max(diam);
+  if (lane == 0) {
+    ResHost h;
+    h.out.sum = sum; h.out.diam = diam; h.out.flags = flags; h.out.levels = (u32)levels; h.out.pad = 0;
+    h.err = __hip_atomic_load(fin.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    h.pad = 0;
+    *fin.host = h;
+    __hip_atomic_store(qhead, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(fin.done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __threadfence_system();
+  }
 }
 
 // The same loop for blocks of one width only: its register allocation is that
@@ -1255,23 +1287,38 @@ __global__ __launch_bounds__(1024) void k_resident_w(const ResTask* __restrict__
                                                      int* __restrict__ qhead, ResOut* __restrict__ out, ResFin fin)
 {
   extern __shared__ __attribute__((aligned(16))) u32 lds[];
-  __shared__ int s_task;
+  __shared__ int s_task, s_last;
   __shared__ u64 s_part[2][16];
   const int tid = threadIdx.x, nth = blockDim.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), nw = nth >> 6;
-  for (;;) {
+  ResOut mine;           // this workgroup's totals (single-graph launches)
+  mine.sum = 0; mine.diam = 0; mine.flags = 0; mine.levels = 0; mine.pad = 0;
+  // the first gridDim.x blocks are dealt by workgroup index, the rest popped
+  // from *qhead (which then counts past gridDim.x)
+  int t = blockIdx.x;
+  while (t < ntasks) {
+    const ResOut b = res_run<WB>(tasks[t], lds, s_part, tid, nth, lane, wave, nw);
+    if (fin.done) {
+      mine.sum += b.sum; mine.diam = max(mine.diam, b.diam); mine.flags |= b.flags; mine.levels += b.levels;
+    } else {
+      res_commit(out, tasks[t].graph, b, lane, wave);
+    }
+    if ((int)gridDim.x >= ntasks) break;          // nothing left to pop
     if (wave == 0) {       // wave-uniform: lane 0 adds 1, the others add 0
       const int got = atomicAdd(qhead, lane == 0 ? 1 : 0);
       s_task = __builtin_amdgcn_readfirstlane(got);
     }
     __syncthreads();
-    const int t = __builtin_amdgcn_readfirstlane(s_task);
-    if (t >= ntasks) break;
-    res_run<WB>(tasks[t], lds, s_part, out, tid, nth, lane, wave, nw);
+    t = (int)gridDim.x + __builtin_amdgcn_readfirstlane(s_task);
   }
   if (fin.done) {        // uniform: a kernel argument
-    __syncthreads();     // every wave's result atomics are issued
-    if (tid == 0) res_finish(fin, qhead, out);
+    if (tid == 0) {
+      fin.part[blockIdx.x] = mine;
+      __threadfence();   // the slot precedes the arrival
+      s_last = atomicAdd(fin.done, 1u) == gridDim.x - 1;
+    }
+    __syncthreads();
+    if (s_last && wave == 0) res_finish(fin, qhead, lane);
   }
 }
 
@@ -1297,11 +1344,13 @@ __global__ __launch_bounds__(1024) void k_resident(const ResTask* __restrict__ t
     const int t = __builtin_amdgcn_readfirstlane(s_task);
     if (t >= ntasks) break;
     const ResTask T = tasks[t];
+    ResOut b;
     switch (T.wb) {
-      case 1: res_run<1>(T, lds, s_part, out, tid, nth, lane, wave, nw); break;
-      case 2: res_run<2>(T, lds, s_part, out, tid, nth, lane, wave, nw); break;
-      case 4: res_run<4>(T, lds, s_part, out, tid, nth, lane, wave, nw); break;
-      default: res_run<8>(T, lds, s_part, out, tid, nth, lane, wave, nw); break;
+      case 1: b = res_run<1>(T, lds, s_part, tid, nth, lane, wave, nw); break;
+      case 2: b = res_run<2>(T, lds, s_part, tid, nth, lane, wave, nw); break;
+      case 4: b = res_run<4>(T, lds, s_part, tid, nth, lane, wave, nw); break;
+      default: b = res_run<8>(T, lds, s_part, tid, nth, lane, wave, nw); break;
     }
+    res_commit(out, T.graph, b, lane, wave);
   }
 }
