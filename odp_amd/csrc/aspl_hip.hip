@@ -445,9 +445,12 @@ struct Evaluator {
       for (int c = wbmax; c >= 1; c >>= 1)
         if ((nsrc + 32 * c - 1) / (32 * c) >= C.ncu) { wb = c; break; }
       p.wb = wb;
+      // workgroup size: at most 2 rows per thread up to n = 1024 (measured at
+      // n=1024, d=6: 25.3 us with 512 threads, 26.8 with 256, 28.1 with 1024;
+      // n=2000, d=12: 29.1 / 33.9 / 30.0), all 1024 threads from n = 4096
       p.threads = 1024;
       for (int th : {256, 512, 1024})
-        if (n <= res_rpt(wb) * th && (n >= 4096 ? th == 1024 : true)) { p.threads = th; break; }
+        if (n <= res_rpt(wb) * th && (n >= 4096 ? th == 1024 : (n > 512 ? th >= 512 : true))) { p.threads = th; break; }
       return p;
     }
     // Column blocks of 16 words (1024 sources), each stored as its own
