@@ -93,12 +93,45 @@ void ODP_Conv_adjacency2edge_grid(int width, int height, int degree, int *num_de
 void ODP_Conv_adjacency2edge_general_s(int nodes, int degree, int *num_degrees, void *adjacency, int symmetries, int (*edge)[2]);
 /* include/odp.h:45 ; utils.c:152-193 */
 void ODP_Conv_adjacency2edge_grid_s(int width, int height, int degree, int *num_degrees, void *adjacency, int symmetries, int (*edge)[2]);
-/* include/odp.h:55 ; utils.c:25-36 */
+/* include/odp.h:51 ; utils.c:25-36 */
 void ODP_Print_adjacency(int nodes, int degree, int *num_degrees, void *adjacency);
-/* include/odp.h:56 ; utils.c:38-42 */
+/* include/odp.h:52 ; utils.c:38-42 */
 void ODP_Print_edge_general(int lines, int (*edge)[2]);
-/* include/odp.h:57 ; utils.c:44-50 */
+/* include/odp.h:53 ; utils.c:44-50 */
 void ODP_Print_edge_grid(int lines, int height, int (*edge)[2]);
+
+/* ---- random start graphs and 2-opt moves (the step before the evaluator) ----
+ * Every random decision is a libc rand() draw taken as the reference takes it,
+ * so one seed gives the same graphs and the same moves as the reference
+ * library.  The adjacency arguments are the reference's int[rows][degree]
+ * tables (rows = nodes/symmetries), passed as plain pointers. */
+
+/* include/odp.h:46 ; utils.c:1036-1039 */
+void ODP_Srand(unsigned int seed);
+/* include/odp.h:47 ; utils.c:1065-1084 (edge holds nodes*degree/2 lines) */
+void ODP_Generate_random_general(int nodes, int degree, int (*edge)[2]);
+/* include/odp.h:48 ; utils.c:1086-1114 */
+void ODP_Generate_random_general_s(int nodes, int degree, int symmetries, int (*edge)[2]);
+/* include/odp.h:49 ; utils.c:1577-1625 */
+void ODP_Generate_random_grid(int width, int height, int degree, int length, int (*edge)[2]);
+/* include/odp.h:50 ; utils.c:1627-1750 */
+void ODP_Generate_random_grid_s(int width, int height, int degree, int length, int symmetries, int (*edge)[2]);
+/* include/odp.h:54 ; utils.c:784-788 */
+void ODP_Mutate_adjacency_general(int nodes, int degree, int *num_degrees, ODP_Restore *r, void *adjacency);
+/* include/odp.h:55 ; utils.c:752-764 */
+void ODP_Mutate_adjacency_general_s(int nodes, int degree, int *num_degrees, int symmetries, ODP_Restore *r,
+                                    void *adjacency);
+/* include/odp.h:56 ; utils.c:1565-1574 */
+void ODP_Mutate_adjacency_grid(int width, int height, int degree, int *num_degrees, int length, ODP_Restore *r,
+                               void *adjacency);
+/* include/odp.h:57 ; utils.c:973-989 */
+void ODP_Mutate_adjacency_grid_s(int width, int height, int degree, int *num_degrees, int length, int symmetries,
+                                 ODP_Restore *r, void *adjacency);
+/* include/odp.h:58 ; utils.c:500-507 (the table of the last general mutation) */
+void ODP_Restore_adjacency_general(ODP_Restore r, void *adjacency);
+/* include/odp.h:59 ; utils.c:461-467 (the table of the last ODP_Mutate_adjacency_grid_s;
+ * like the reference, nothing is stored when no such mutation came first) */
+void ODP_Restore_adjacency_grid(ODP_Restore r, void *adjacency);
 
 #ifdef __cplusplus
 }

@@ -66,6 +66,31 @@ static double wall_time()
 static inline int round_up(int x, int m) { return (x + m - 1) / m * m; }
 static inline int ilog2(int x) { int r = 0; while ((1 << (r + 1)) <= x) r++; return r; }
 
+// Annealing programs seed libc's generator once (ODP_Srand) and draw moves
+// and acceptance tests from it between evaluations; the reference library
+// draws nothing while it evaluates.  The HIP runtime does draw from it (its
+// initialisation at least), which would shift such a program's trajectory.
+// So every entry point that can reach the runtime runs with libc's
+// random()/rand() switched to a private state and switches the caller's back
+// on return (setstate keeps the caller's position; nested entries are fine).
+static char g_rand_private[256];
+static bool g_rand_private_ready = false;
+struct RandFence {
+  char* prev;
+  RandFence()
+  {
+    if (!g_rand_private_ready) {
+      prev = initstate(1u, g_rand_private, sizeof g_rand_private);    // also makes it current
+      g_rand_private_ready = true;
+    } else {
+      prev = setstate(g_rand_private);
+    }
+  }
+  ~RandFence() { setstate(prev); }
+  RandFence(const RandFence&) = delete;
+  RandFence& operator=(const RandFence&) = delete;
+};
+
 // ---------------------------------------------------------------------------
 // device context: one stream (default priority) for everything
 
@@ -1241,11 +1266,13 @@ static void lifecycle_init(int family, int nodes, int degree, const int* numdeg_
 
 extern "C" void ODP_Init_aspl_general(int nodes, int degree, int* num_degrees)
 {
+  RandFence rand_fence;
   lifecycle_init(0, nodes, degree, num_degrees, 1, 0, -1, false, nullptr);
 }
 
 extern "C" void ODP_Init_aspl_general_s(int nodes, int degree, int* num_degrees, int symmetries)
 {
+  RandFence rand_fence;
   if (symmetries < 1 || nodes % symmetries != 0)
     ODP_ERROR("nodes(%d) must be divisible by symmetries(%d)\n", nodes, symmetries);
   std::vector<int> full;
@@ -1260,11 +1287,13 @@ extern "C" void ODP_Init_aspl_general_s(int nodes, int degree, int* num_degrees,
 
 extern "C" void ODP_Init_aspl_grid(int width, int height, int degree, int* num_degrees)
 {
+  RandFence rand_fence;
   lifecycle_init(1, width * height, degree, num_degrees, 1, width, height, false, nullptr);
 }
 
 extern "C" void ODP_Init_aspl_grid_s(int width, int height, int degree, int* num_degrees, int symmetries)
 {
+  RandFence rand_fence;
   int nodes = width * height;
   bool gs = (symmetries == 2 || symmetries == 4);
   if (symmetries < 1 || nodes % symmetries != 0)
@@ -1318,6 +1347,7 @@ static void finish_result(int* diameter, long* sum, double* ASPL)
 
 extern "C" void ODP_Set_aspl(void* adjacency, int* diameter, long* sum, double* ASPL)
 {
+  RandFence rand_fence;
   need_active("ODP_Set_aspl");
   double t = wall_time();
   E.ev.upload(adjacency);
@@ -1329,6 +1359,7 @@ extern "C" void ODP_Set_aspl(void* adjacency, int* diameter, long* sum, double* 
 
 extern "C" void ODP_Finalize_aspl(void)
 {
+  RandFence rand_fence;
   if (!E.active) return;
   if (E.profile) {             // utils.c:1268-1294, flavour "HIP"
     char host[256] = {0};
@@ -1367,6 +1398,7 @@ extern "C" int ODPX_Get_plan(ODPX_Plan* p)
 
 extern "C" int ODPX_Set_plan(const ODPX_Plan* p)
 {
+  RandFence rand_fence;
   if (!E.active || !E.ev.plan_valid(*p)) return -1;
   E.ev.apply_plan(*p);
   return 0;
@@ -1374,6 +1406,7 @@ extern "C" int ODPX_Set_plan(const ODPX_Plan* p)
 
 extern "C" void ODPX_Upload_adjacency(void* adjacency)
 {
+  RandFence rand_fence;
   need_active("ODPX_Upload_adjacency");
   E.ev.upload(adjacency);
   HIPC(hipStreamSynchronize(C.stream));
@@ -1381,6 +1414,7 @@ extern "C" void ODPX_Upload_adjacency(void* adjacency)
 
 extern "C" void ODPX_Set_aspl_resident(int* diameter, long* sum, double* ASPL)
 {
+  RandFence rand_fence;
   need_active("ODPX_Set_aspl_resident");
   double t = wall_time();
   E.ev.evaluate(diameter, sum, ASPL);
@@ -1412,6 +1446,7 @@ __global__ void k_swap(int* adj, int degree, int a, int b, int c, int d, int* er
 
 extern "C" int ODPX_Swap_device(int a, int b, int c, int d)
 {
+  RandFence rand_fence;
   need_active("ODPX_Swap_device");
   Evaluator& v = E.ev;
   if (v.s != 1) return -1;
@@ -1457,6 +1492,7 @@ static bool sa_accept(int nodes, int cur_d, int d, double cur_a, double a, doubl
 extern "C" int ODPX_Anneal(void* adjacency, long long ncalcs, double max_temp, double min_temp, int hill_climbing,
                            int ASPL_priority, uint64_t seed, ODPX_AnnealResult* res)
 {
+  RandFence rand_fence;
   need_active("ODPX_Anneal");
   Evaluator& v = E.ev;
   if (v.s != 1 || v.family != 0 || ncalcs < 0 || (!hill_climbing && (max_temp <= 0 || min_temp <= 0))) return -1;
@@ -1527,6 +1563,7 @@ extern "C" void ODPX_Stats_get(ODPX_Stats* out) { *out = g_stats; }
 
 extern "C" int ODPX_Autotune(void* adjacency, int repeats)
 {
+  RandFence rand_fence;
   need_active("ODPX_Autotune");
   Evaluator& v = E.ev;
   std::vector<ODPX_Plan> cand;
@@ -1730,6 +1767,7 @@ static void batch_build_tasks()
 extern "C" int ODPX_Batch_load(int ngraphs, const int* nodes, const int* degrees, const long long* offsets,
                                const int* packed)
 {
+  RandFence rand_fence;
   batch_drop_graph();
   ctx_init();
   B.ng = ngraphs;
@@ -1804,6 +1842,7 @@ static void batch_enqueue()
 
 extern "C" int ODPX_Batch_run(int* diameter, long* sum, double* ASPL)
 {
+  RandFence rand_fence;
   ctx_init();
   double t0 = wall_time();
   if (!B.small.empty()) {
@@ -1853,6 +1892,7 @@ extern "C" int ODPX_Batch_run(int* diameter, long* sum, double* ASPL)
 
 extern "C" int ODPX_Loop_run(int plan, int* diameter, long* sum, double* ASPL)
 {
+  RandFence rand_fence;
   ctx_init();
   for (int g = 0; g < B.ng; g++)
     single_eval_dev(B.single, B.nodes[g], B.degrees[g], B.d_adj + B.offsets[g], plan, &diameter[g], &sum[g], &ASPL[g]);
@@ -1862,6 +1902,7 @@ extern "C" int ODPX_Loop_run(int plan, int* diameter, long* sum, double* ASPL)
 extern "C" int ODPX_Batch_eval(int ngraphs, const int* nodes, const int* degrees, const long long* offsets,
                                const int* packed, int* diameter, long* sum, double* ASPL)
 {
+  RandFence rand_fence;
   ODPX_Batch_load(ngraphs, nodes, degrees, offsets, packed);
   return ODPX_Batch_run(diameter, sum, ASPL);
 }
@@ -1869,6 +1910,7 @@ extern "C" int ODPX_Batch_eval(int ngraphs, const int* nodes, const int* degrees
 extern "C" int ODPX_Loop_eval(int ngraphs, const int* nodes, const int* degrees, const long long* offsets,
                               const int* packed, int plan, int* diameter, long* sum, double* ASPL)
 {
+  RandFence rand_fence;
   ODPX_Batch_load(ngraphs, nodes, degrees, offsets, packed);
   return ODPX_Loop_run(plan, diameter, sum, ASPL);
 }
@@ -1881,6 +1923,7 @@ extern "C" void ODPX_Batch_get_policy(int* wbcap, int* per_cu)
 
 extern "C" int ODPX_Batch_set_policy(const int* wbcap, int per_cu)
 {
+  RandFence rand_fence;
   for (int c = 0; c < BATCH_CLASSES; c++)
     if (wbcap[c] != 1 && wbcap[c] != 2 && wbcap[c] != 4 && wbcap[c] != 8) return -1;
   if (per_cu != 1 && per_cu != 2) return -1;
@@ -1896,6 +1939,7 @@ extern "C" int ODPX_Batch_set_policy(const int* wbcap, int per_cu)
 // Every candidate must return identical results (exit(1) otherwise).
 extern "C" int ODPX_Batch_autotune(int repeats)
 {
+  RandFence rand_fence;
   ctx_init();
   if (B.small.empty()) return 0;
   if (repeats < 1) repeats = 1;
@@ -1947,6 +1991,7 @@ extern "C" int ODPX_Batch_autotune(int repeats)
 
 extern "C" void ODPX_Batch_release(void)
 {
+  RandFence rand_fence;
   batch_drop_graph();
   dfree(&B.d_adj); dfree(&B.d_nbr); dfree(&B.d_err); dfree(&B.d_qhead); hfree(&B.h_err);
   dfree(&B.d_g); dfree(&B.d_out); hfree(&B.h_out); dfree(&B.d_t);

@@ -341,6 +341,7 @@ static size_t sv_level_bytes(int n)
 
 extern "C" int ODPX_Serve_start(int small_slots, int large_slots, double hbm_budget_gb)
 {
+  RandFence rand_fence;
   ctx_init();
   std::lock_guard<std::mutex> lk(S.mu);
   if (S.running) return -1;
@@ -376,6 +377,7 @@ static void sv_make_large_slots()
 
 extern "C" int ODPX_Serve_add_chain(int nodes, int degree, const int* adjacency)
 {
+  RandFence rand_fence;
   std::lock_guard<std::mutex> lk(S.mu);
   if (!S.running || nodes < 2 || degree < 1 || !adjacency) return -1;
   for (long i = 0; i < (long)nodes * degree; i++)
@@ -420,6 +422,7 @@ extern "C" int ODPX_Serve_add_chain(int nodes, int degree, const int* adjacency)
 
 extern "C" long long ODPX_Serve_submit(int chain, int nswaps, const int* swaps)
 {
+  RandFence rand_fence;
   std::lock_guard<std::mutex> lk(S.mu);
   if (!S.running || chain < 0 || chain >= (int)S.chains.size() || nswaps < 0) return -1;
   if (!S.chains[chain].small && S.nlarge == 0) return -1;
@@ -444,6 +447,7 @@ static void sv_fill(long long id, const SvReq& r, ODPX_Result* o)
 
 extern "C" int ODPX_Serve_poll(ODPX_Result* out, int max)
 {
+  RandFence rand_fence;
   std::lock_guard<std::mutex> lk(S.mu);
   if (!S.running) return 0;
   sv_progress();
@@ -460,6 +464,7 @@ extern "C" int ODPX_Serve_poll(ODPX_Result* out, int max)
 
 extern "C" int ODPX_Serve_wait(long long id, ODPX_Result* out)
 {
+  RandFence rand_fence;
   std::lock_guard<std::mutex> lk(S.mu);
   if (!S.running || S.reqs.find(id) == S.reqs.end()) return -1;
   for (;;) {
@@ -486,12 +491,14 @@ static void sv_drain_locked()
 
 extern "C" void ODPX_Serve_drain(void)
 {
+  RandFence rand_fence;
   std::lock_guard<std::mutex> lk(S.mu);
   if (S.running) sv_drain_locked();
 }
 
 extern "C" void ODPX_Serve_stop(void)
 {
+  RandFence rand_fence;
   std::lock_guard<std::mutex> lk(S.mu);
   if (!S.running) return;
   sv_drain_locked();

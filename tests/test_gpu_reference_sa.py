@@ -1,0 +1,80 @@
+"""The reference's own annealing programs on libodp_hip.
+
+sample/SA/{general,general_s,grid,grid_s}.c of the reference, compiled unchanged
+against the reference's include/odp.h (oracle/Makefile, `make sa`), run once
+linked with the reference library (CPU) and once linked with libodp_hip (every
+ODP_Set_aspl on the GPU).  Start graph, moves, restores and acceptance draws
+all come from the same libc random() stream, so the two runs must print the
+same trajectory -- every progress line, the best diameter, sum and ASPL -- and
+differ only in the timing line.
+"""
+import os
+import subprocess
+
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+REF = os.path.join(os.path.dirname(os.path.abspath(__file__)), "..", "oracle", "_ref")
+
+CASES = [
+    ("general", "-N 128 -D 4 -s 7 -n 400 -w 50 -c 0.22"),
+    ("general", "-N 256 -D 6 -s 3 -n 300 -w 30 -c 0.2"),
+    ("general", "-N 96 -D 3 -s 2 -n 300 -w 40 -c 0.2 -Y"),            # hill climbing
+    ("general", "-N 128 -D 4 -s 11 -n 200"),                          # temperature found by its own short run
+    ("general_s", "-N 96 -D 4 -S 4 -s 5 -n 300 -w 50 -c 0.2"),
+    ("general_s", "-N 120 -D 3 -S 2 -s 9 -n 300 -w 50 -c 0.2 -A"),   # ASPL priority
+    ("grid", "-W 10 -H 10 -D 4 -L 3 -s 5 -n 300 -w 50 -c 0.2"),
+    ("grid_s", "-W 12 -H 12 -D 4 -L 3 -S 4 -s 5 -n 300 -w 50 -c 0.2"),
+    ("grid_s", "-W 10 -H 8 -D 3 -L 4 -S 2 -s 6 -n 300 -w 50 -c 0.2"),
+]
+
+
+def run(prog, args, tmp):
+    out = subprocess.run([os.path.join(REF, prog), *args.split()], capture_output=True, text=True, timeout=120,
+                         cwd=tmp)
+    assert out.returncode == 0, (prog, out.stderr[-2000:])
+    return [ln for ln in out.stdout.splitlines() if not ln.startswith("Time")]
+
+
+@pytest.mark.parametrize("case", CASES, ids=lambda c: c[0] + ":" + c[1].replace(" ", ""))
+def test_reference_annealing_program(case, tmp_path):
+    prog, args = case
+    if not (os.path.exists(os.path.join(REF, f"sa_{prog}_ref")) and os.path.exists(os.path.join(REF, f"sa_{prog}_hip"))):
+        pytest.skip("reference annealing programs not built (oracle/Makefile `sa` needs the reference sources)")
+    env_profile = os.environ.pop("ODP_PROFILE", None)
+    try:
+        ref = run(f"sa_{prog}_ref", args, tmp_path)
+        hip = run(f"sa_{prog}_hip", args, tmp_path)
+    finally:
+        if env_profile is not None:
+            os.environ["ODP_PROFILE"] = env_profile
+    assert any(ln.startswith("ASPL ") for ln in ref), ref[-5:]
+    diff = [(i, a, b) for i, (a, b) in enumerate(zip(ref, hip)) if a != b]
+    assert len(ref) == len(hip) and not diff, (len(ref), len(hip), diff[:3])
+
+
+def test_libc_random_stream_untouched():
+    """Evaluations leave the caller's libc random() stream where it was: draws
+    after seeding are the same with or without Init / Set_aspl / Finalize (and
+    the other device entry points) in between."""
+    import ctypes
+    import odp_amd as odp
+    libc = ctypes.CDLL(None)
+    libc.srand.argtypes = [ctypes.c_uint]
+    libc.rand.restype = ctypes.c_int
+    libc.srand(7)
+    want = [libc.rand() for _ in range(5)]
+    libc.srand(7)
+    n, d = 300, 5
+    adj = odp.generate_regular(n, d, 3)
+    odp.init_aspl_general(n, d)
+    odp.set_aspl(adj)
+    odp.upload_adjacency(adj)
+    odp.set_aspl_resident()
+    odp.set_plan(plan=odp.PLAN_LEVEL, kw=2, upw=4, seg_major=0, xcd=1)
+    odp.set_aspl(adj)
+    odp.finalize_aspl()
+    odp.batch_eval(odp.pack([adj, adj]))
+    got = [libc.rand() for _ in range(5)]
+    assert got == want
