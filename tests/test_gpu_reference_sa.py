@@ -54,6 +54,44 @@ def test_reference_annealing_program(case, tmp_path):
     assert len(ref) == len(hip) and not diff, (len(ref), len(hip), diff[:3])
 
 
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+# the reference's sample programs (sample/<prog>.c): the edge file they read, or
+# the file they write
+SAMPLES = [
+    ("general", os.path.join(GOLDEN, "general", "n16d4.edges"), None),
+    ("general", os.path.join(GOLDEN, "general", "n256d10.edges"), None),
+    ("grid", os.path.join(GOLDEN, "grid", "w10h10d4r4g4.edges"), None),
+    ("regular_general", None, None),
+    ("non_regular_general", None, None),
+    ("random_general", None, "general.edges"),
+    ("random_grid", None, "grid.edges"),
+]
+
+
+@pytest.mark.parametrize("case", SAMPLES, ids=lambda c: c[0] + ("-" + os.path.basename(c[1]) if c[1] else ""))
+def test_reference_sample_program(case, tmp_path):
+    """sample/<prog>.c linked with the reference library and with libodp_hip:
+    the same printed results and the same written edge file."""
+    prog, infile, outfile = case
+    if not (os.path.exists(os.path.join(REF, f"smp_{prog}_ref")) and os.path.exists(os.path.join(REF, f"smp_{prog}_hip"))):
+        pytest.skip("reference sample programs not built (oracle/Makefile `sa` needs the reference sources)")
+    outs, files = [], []
+    for lib in ("ref", "hip"):
+        d = tmp_path / lib
+        d.mkdir()
+        args = [infile] if infile else []
+        r = subprocess.run([os.path.join(REF, f"smp_{prog}_{lib}"), *args], capture_output=True, text=True,
+                           timeout=120, cwd=d)
+        assert r.returncode == 0, (prog, lib, r.stderr[-2000:])
+        outs.append(r.stdout)
+        if outfile:
+            files.append((d / outfile).read_text())
+    assert "ASPL" in outs[0] and outs[0] == outs[1]
+    if outfile:
+        assert files[0] and files[0] == files[1]
+
+
 def test_libc_random_stream_untouched():
     """Evaluations leave the caller's libc random() stream where it was: draws
     after seeding are the same with or without Init / Set_aspl / Finalize (and
