@@ -37,6 +37,10 @@ static const int RES_WB_MAX = 8;                // widest resident block: 256 so
 static const int BATCH_CLASSES = 5;             // size classes of the batch launch policy
 static const int FRONTIER_MIN_LEVELS = 16;      // frontier-mode levels when the last evaluation needed more
 static const u32 LIST_CAP = 1u << 22;           // unsaturated chunks a level may list (32 MiB)
+// adjacency tables up to this size reach the device through the resident
+// plan's table build reading pinned host memory (one kernel instead of a DMA
+// command plus the kernel); larger ones are copied by DMA
+static const size_t ZC_MAX_BYTES = 256u << 10;
 static const int TILE_MIN = 32;                 // smallest tile side of k_tile (halo 16)
 static const int SPARSE_GRID = 2048;            // workgroups of a k_sparse launch (8 per CU)
 static const long DIRECT_TREE_MAX = 8192;       // walk-tree nodes per row the automatic direct K allows
@@ -257,6 +261,10 @@ struct Evaluator {
   size_t toff_cap = 0;
   int geo_lmax = -1;                    // largest |dx|, |dy| of an adjacency entry; -1: no geometry
   int *h_stage = nullptr, *h_err = nullptr;
+  int* h_stage_dev = nullptr;           // device address of h_stage (coherent pinned memory)
+  // the last upload is still only in h_stage: the resident plan's next table
+  // build reads it there and copies it to d_adj (small tables: no DMA command)
+  bool adj_on_host = false;
   size_t stage_cap = 0;
   // the resident plan's 16-bit table matches the adjacency at `table_src`: set by
   // its preparation, kept by device swaps (they patch the table as well), cleared
@@ -331,7 +339,7 @@ struct Evaluator {
     dfree(&d_adj); dfree(&d_numdeg); dfree(&d_itable); dfree(&d_nbr); dfree(&d_err); dfree(&d_nbr16);
     dfree(&d_xy2row); dfree(&d_row2xy); dfree(&d_toff);
     nbr16_cap = 0; toff_cap = 0; geo_lmax = -1;
-    hfree(&h_stage); hfree(&h_err);
+    hfree(&h_stage); hfree(&h_err); h_stage_dev = nullptr; adj_on_host = false;
     dfree(&lr.bufs); dfree(&lr.cnt); dfree(&lr.done); dfree(&lr.tasks); dfree(&lr.wg_start); hfree(&lr.h_cnt);
     dfree(&lr.flags); hfree(&lr.h_acnt); dfree(&lr.wgm); dfree(&lr.list); dfree(&lr.list_n); dfree(&lr.tq);
     dfree(&rr.tasks); dfree(&rr.out); dfree(&rr.qhead); dfree(&rr.done); hfree(&rr.host); rr.host_dev = nullptr;
@@ -353,7 +361,12 @@ struct Evaluator {
     dpad = round_up(deg > 0 ? deg : 1, 4);
     has_numdeg = numdeg_full != nullptr;
     if (adj_elems() > adj_cap) { dfree(&d_adj); dalloc(&d_adj, adj_elems()); adj_cap = adj_elems(); }
-    if (adj_elems() > stage_cap) { hfree(&h_stage); halloc(&h_stage, adj_elems()); stage_cap = adj_elems(); }
+    adj_on_host = false;
+    if (adj_elems() > stage_cap) {
+      hfree(&h_stage);
+      halloc_coherent(&h_stage, &h_stage_dev, adj_elems());
+      stage_cap = adj_elems();
+    }
     size_t ne = (size_t)n * dpad;
     if (ne > nbr_cap) { dfree(&d_nbr); dalloc(&d_nbr, ne); nbr_cap = ne; }
     dp16 = round_up(deg > 0 ? deg : 1, 8);
@@ -425,8 +438,22 @@ struct Evaluator {
     trace("upload");
     table_fresh = false;
     memcpy(h_stage, adjacency, sizeof(int) * adj_elems());
-    HIPC(hipMemcpyAsync(d_adj, h_stage, sizeof(int) * adj_elems(), hipMemcpyHostToDevice, st));
+    if (plan.plan == ODPX_PLAN_RESIDENT && !ext_adj && sizeof(int) * adj_elems() <= ZC_MAX_BYTES) {
+      adj_on_host = true;          // the next table build reads it from host memory
+    } else {
+      adj_on_host = false;
+      HIPC(hipMemcpyAsync(d_adj, h_stage, sizeof(int) * adj_elems(), hipMemcpyHostToDevice, st));
+    }
     if (d_xy2row) geo_lmax = grid_lmax(h_stage);
+  }
+
+  // d_adj holds the last upload (copies it from host memory if it is still only there);
+  // called outside stream capture before anything but the resident table build reads d_adj
+  void flush_adj()
+  {
+    if (!adj_on_host) return;
+    adj_on_host = false;
+    HIPC(hipMemcpyAsync(d_adj, h_stage, sizeof(int) * adj_elems(), hipMemcpyHostToDevice, st));
   }
 
   // ---- plans ----
@@ -691,6 +718,7 @@ struct Evaluator {
 
   void apply_plan(const ODPX_Plan& p)
   {
+    flush_adj();
     drop_graph();
     table_fresh = false;
     plan = p;
@@ -704,7 +732,12 @@ struct Evaluator {
   {
     PrepArgs a;
     const bool res = plan.plan == ODPX_PLAN_RESIDENT;
-    a.adj = ext_adj ? ext_adj : d_adj; a.numdeg = d_numdeg; a.itable = d_itable; a.err = d_err;
+    const bool from_host = adj_on_host && !ext_adj;     // set only under the resident plan
+    a.adj = from_host ? h_stage_dev : (ext_adj ? ext_adj : d_adj);
+    a.adj_copy = from_host ? d_adj : nullptr;
+    a.adj_elems = from_host ? (int64_t)adj_elems() : 0;
+    if (from_host) adj_on_host = false;
+    a.numdeg = d_numdeg; a.itable = d_itable; a.err = d_err;
     a.nbr = res ? nullptr : d_nbr;
     a.nbr16 = res || direct16() ? d_nbr16 : nullptr;     // the level plan's k_direct reads 16-bit ids
     a.n = n; a.s = s; a.degree = degree; a.dpad = res ? dp16 : dpad;
@@ -960,6 +993,7 @@ struct Evaluator {
 
   void lv_begin()
   {
+    flush_adj();
     LevelRun& r = lr;
     trace("run_level");
     lv.K = r.direct;                      // levels 0..K-1 come from k_direct (0: k_level0 does level 0)
@@ -1452,6 +1486,7 @@ extern "C" int ODPX_Swap_device(int a, int b, int c, int d)
   if (v.s != 1) return -1;
   int ids[4] = {a, b, c, d};
   for (int x : ids) if (x < 0 || x >= v.n) return -1;
+  v.flush_adj();
   hipLaunchKernelGGL(k_swap, dim3(1), dim3(64), 0, C.stream, v.d_adj, v.degree, a, b, c, d, v.d_err, v.swap_table(),
                      v.dp16);
   HIPC(hipGetLastError());
@@ -1513,6 +1548,7 @@ extern "C" int ODPX_Anneal(void* adjacency, long long ncalcs, double max_temp, d
   bool optimum = diameter == low_d && ASPL == low_a;
   const double cooling = hill_climbing ? 0.0 : pow(min_temp / max_temp, 1.0 / (double)ncalcs);
   double temp = max_temp;
+  v.flush_adj();
   for (long long i = 0; i < ncalcs && !optimum; i++) {
     int sw[4];
     if (ODPX_Two_opt(n, deg, cur.data(), rng, sw) != 0) break;      // no valid move left

@@ -95,6 +95,7 @@ struct PrepArgs {
   int n, s, degree, dpad, width, height, grid_s;
   u64* zero64; int nzero64;   // counters reset to 0 at the start of an evaluation
   u32* zero32; int nzero32;
+  int* adj_copy; int64_t adj_elems;   // adj is in host memory: also copy it here (or null)
 };
 
 __device__ __forceinline__ int grid_rotate(int v, int width, int height, int s, int angle)
@@ -135,6 +136,8 @@ __global__ void k_prepare(PrepArgs a)
   const int64_t stride = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = tid; i < a.nzero64; i += stride) a.zero64[i] = 0;
   for (int64_t i = tid; i < a.nzero32; i += stride) a.zero32[i] = 0;
+  if (a.adj_copy)
+    for (int64_t i = tid; i < a.adj_elems; i += stride) a.adj_copy[i] = a.adj[i];
   const int64_t total = (int64_t)a.n * a.dpad;
   const int b = a.n / a.s;
   for (int64_t idx = tid; idx < total; idx += stride) {

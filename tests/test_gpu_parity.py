@@ -270,6 +270,37 @@ def test_resident_table_follows_swaps_and_uploads():
         odp.finalize_aspl()
 
 
+def test_small_upload_read_by_table_build():
+    """Under the resident plan a small table reaches the device through the
+    table build reading pinned host memory (no DMA); two uploads in a row, a
+    plan change, a device swap and the drop-in path must all see the last
+    upload."""
+    n, d = 800, 4
+    g1 = odp.generate_regular(n, d, 51)
+    g2 = odp.generate_regular(n, d, 52)
+    rng = odp.Rng(8)
+    odp.init_aspl_general(n, d)
+    try:
+        odp.set_plan(**PLANS[1])
+        odp.upload_adjacency(g1)
+        odp.upload_adjacency(g2)
+        assert same(odp.set_aspl_resident(), oracle.evaluate(g2, nodes=n))
+        odp.upload_adjacency(g1)
+        odp.set_plan(**PLANS[4])                      # level plan: the upload is copied first
+        assert same(odp.set_aspl_resident(), oracle.evaluate(g1, nodes=n))
+        odp.set_plan(**PLANS[1])
+        host = g2.copy()
+        odp.upload_adjacency(host)
+        a, b, c, dd = odp.two_opt(host, rng)
+        assert odp.swap_device(a, b, c, dd) == 0      # swaps the uploaded table, not a stale one
+        assert same(odp.set_aspl_resident(), oracle.evaluate(host, nodes=n))
+        assert same(odp.set_aspl(g1), oracle.evaluate(g1, nodes=n))
+        assert same(odp.set_aspl_resident(), oracle.evaluate(g1, nodes=n))
+        assert same(odp.set_aspl(g2), oracle.evaluate(g2, nodes=n))
+    finally:
+        odp.finalize_aspl()
+
+
 def test_autotune_consistent():
     n, d = 4096, 8
     adj = odp.generate_regular(n, d, 5)
