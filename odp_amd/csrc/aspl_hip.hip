@@ -693,7 +693,10 @@ struct Evaluator {
       r.task_cap = ntasks;
     }
     if (!r.out) {
-      // zero once here: every launch leaves them at zero for the next (res_finish)
+      // the queue head and the arrival counter are zeroed once here: every
+      // launch leaves them at zero for the next (res_finish).  `out` is the
+      // kernel's per-graph accumulator argument, unused by single-graph
+      // launches (their totals go through the per-workgroup slots)
       dalloc(&r.out, 1); dalloc(&r.qhead, 1); dalloc(&r.done, 1);
       halloc_coherent(&r.host, &r.host_dev, 1);
       HIPC(hipMemsetAsync(r.out, 0, sizeof(ResOut), st));
