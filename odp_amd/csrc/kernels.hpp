@@ -896,7 +896,7 @@ __device__ __forceinline__ void direct_set(u32* bm, int id, int src_lo, int nsrc
 
 // ID16: the 16-bit table (eight ids per 16-byte load) instead of the 32-bit one
 template <bool ALL, bool ID16>
-__global__ __launch_bounds__(256) void k_direct(DirectArgs a)
+__global__ __launch_bounds__(512) void k_direct(DirectArgs a)
 {
   extern __shared__ __attribute__((aligned(16))) u64 dlds[];
   const int nw = blockDim.x >> 6;
