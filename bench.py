@@ -366,8 +366,8 @@ def main():
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the path shards by "
-              f"source-column blocks, see DESIGN.md)", file=sys.stderr)
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported (the path would shard "
+              f"by source-column blocks, one share per GPU, with one 16-byte reduction of the sums)", file=sys.stderr)
         sys.exit(2)
 
     gkind, n, d, desc = WORKLOADS[args.workload]
@@ -433,7 +433,7 @@ def main():
     odp.stats_enable(False)
     log(f"timed {args.steps} evaluations in {el:.3f}s ({el_live:.3f}s with kernel timing); kernel stats {st}")
 
-    # PCIe-inclusive rate of the drop-in call (host adjacency every call), for DESIGN.md only
+    # PCIe-inclusive rate of the drop-in call (host adjacency every call), reported in config only
     t = time.perf_counter()
     for _ in range(3):
         odp.set_aspl(adj)
