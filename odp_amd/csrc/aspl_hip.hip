@@ -271,6 +271,28 @@ struct Evaluator {
   // by an upload, a new shape or a new plan
   bool table_fresh = false;
   const int* table_src = nullptr;
+  // table entries the next resident launch changes first (triples row, position, id)
+  std::vector<int> pend_patch;
+  // the pending table entries as launch arguments (taken: the list is emptied)
+  ResPatch make_patch()
+  {
+    ResPatch p{};
+    p.n = (int)(pend_patch.size() / 3);
+    if (p.n > RES_PATCH_MAX) ODP_ERROR("libodp_hip: internal error: too many pending table entries\n");
+    for (int i = 0; i < p.n; i++)
+      for (int k = 0; k < 3; k++) p.e[i][k] = pend_patch[3 * i + k];
+    p.table = d_nbr16; p.tdp = dp16; p.adj = d_adj; p.degree = degree;
+    pend_patch.clear();
+    return p;
+  }
+  // stores the pending entries now (no evaluation follows)
+  void apply_pending()
+  {
+    if (pend_patch.empty()) return;
+    hipLaunchKernelGGL(k_patch, dim3(1), dim3(64), 0, st, make_patch());
+    HIPC(hipGetLastError());
+  }
+
   // the table a device swap patches along with the adjacency; null (and the
   // table no longer current) when there is none to keep
   unsigned short* swap_table()
@@ -1139,6 +1161,7 @@ struct Evaluator {
       auto kern = r.wb == 1 ? k_resident_w<1> : r.wb == 2 ? k_resident_w<2> : r.wb == 4 ? k_resident_w<4> : k_resident_w<8>;
       ResFin fin;
       fin.done = r.done; fin.part = r.part; fin.host = r.host_dev; fin.err = d_err;
+      fin.patch = make_patch();
       hipLaunchKernelGGL(kern, dim3(r.grid), dim3(r.threads), r.lds, st, r.tasks, r.ntasks, r.qhead, r.out, fin);
       HIPC(hipGetLastError());
       timed_end(tl);
@@ -1561,13 +1584,31 @@ extern "C" int ODPX_Anneal(void* adjacency, long long ncalcs, double max_temp, d
   const double cooling = hill_climbing ? 0.0 : pow(min_temp / max_temp, 1.0 / (double)ncalcs);
   double temp = max_temp;
   v.flush_adj();
+  // Under the resident plan (its table current) a move is not a launch of its
+  // own: its four table entries ride along with the next evaluation's launch
+  // (the host copy gives their positions).  Otherwise k_swap applies it.
+  const bool fold = v.plan.plan == ODPX_PLAN_RESIDENT && v.swap_table() != nullptr;
+  auto pos_of = [&](int row, int id) {
+    for (int j = 0; j < deg; j++) if (cur[(size_t)row * deg + j] == id) return j;
+    ODP_ERROR("libodp_hip: internal error: a move's edge is not in the host copy\n");
+    return -1;
+  };
+  // the move a-b, c-d -> a-c, b-d has been made on the host copy
+  auto queue_move = [&](int a, int b, int c, int d) {
+    const int e[12] = {a, pos_of(a, c), c, b, pos_of(b, d), d, c, pos_of(c, a), a, d, pos_of(d, b), b};
+    v.pend_patch.insert(v.pend_patch.end(), e, e + 12);
+  };
   for (long long i = 0; i < ncalcs && !optimum; i++) {
     int sw[4];
     if (ODPX_Two_opt(n, deg, cur.data(), rng, sw) != 0) break;      // no valid move left
     // the same move on the device copy (valid: the host copy mirrors it)
-    hipLaunchKernelGGL(k_swap, dim3(1), dim3(64), 0, C.stream, v.d_adj, deg, sw[0], sw[1], sw[2], sw[3], v.d_err,
-                       v.swap_table(), v.dp16);
-    HIPC(hipGetLastError());
+    if (fold) {
+      queue_move(sw[0], sw[1], sw[2], sw[3]);
+    } else {
+      hipLaunchKernelGGL(k_swap, dim3(1), dim3(64), 0, C.stream, v.d_adj, deg, sw[0], sw[1], sw[2], sw[3], v.d_err,
+                         v.swap_table(), v.dp16);
+      HIPC(hipGetLastError());
+    }
     const double t = wall_time();
     v.evaluate(&diameter, &sum, &ASPL);
     E.elapsed += wall_time() - t;
@@ -1585,12 +1626,17 @@ extern "C" int ODPX_Anneal(void* adjacency, long long ncalcs, double max_temp, d
     } else {
       // undo a-b, c-d -> a-c, b-d on both copies (the next launch on the stream follows it)
       ODPX_Swap_edges(n, deg, cur.data(), sw[0], sw[2], sw[1], sw[3]);
-      hipLaunchKernelGGL(k_swap, dim3(1), dim3(64), 0, C.stream, v.d_adj, deg, sw[0], sw[2], sw[1], sw[3], v.d_err,
-                         v.swap_table(), v.dp16);
-      HIPC(hipGetLastError());
+      if (fold) {
+        queue_move(sw[0], sw[2], sw[1], sw[3]);
+      } else {
+        hipLaunchKernelGGL(k_swap, dim3(1), dim3(64), 0, C.stream, v.d_adj, deg, sw[0], sw[2], sw[1], sw[3], v.d_err,
+                           v.swap_table(), v.dp16);
+        HIPC(hipGetLastError());
+      }
     }
     temp *= cooling;
   }
+  v.apply_pending();                 // a last undo, if the last move was rejected
   HIPC(hipStreamSynchronize(C.stream));
   if (res) {
     res->steps = steps; res->accepted = accepted;

@@ -1231,11 +1231,40 @@ struct ResHost {
 // slots up, hands the result and the error flag to host memory and leaves the
 // queue head and the arrival counter at zero for the next launch, so a launch
 // needs no clear before it and no copy after it.
+// Entries to change (2-opt moves): row, position, new id, applied in order to
+// the 16-bit table and the adjacency of one graph.
+#define RES_PATCH_MAX 8
+struct ResPatch {
+  int n;
+  int e[RES_PATCH_MAX][3];
+  unsigned short* table; int tdp;
+  int* adj; int degree;
+};
+
+// one thread: in order, since a later entry may overwrite an earlier one
+__device__ __forceinline__ void res_patch(const ResPatch& p, bool adj_too)
+{
+  for (int i = 0; i < p.n; i++) {
+    const int r = p.e[i][0], c = p.e[i][1], val = p.e[i][2];
+    p.table[(int64_t)r * p.tdp + c] = (unsigned short)val;
+    if (adj_too) p.adj[(int64_t)r * p.degree + c] = val;
+  }
+}
+
+__global__ void k_patch(ResPatch p)
+{
+  if (threadIdx.x == 0) res_patch(p, true);
+}
+
 struct ResFin {
   u32* done;           // workgroups finished; null: no finishing step (batched launches)
   ResOut* part;        // [gridDim.x] per-workgroup totals
   ResHost* host;       // device address of the host result
   const int* err;      // device error flag
+  // applied before the levels: every workgroup stores the entries into the
+  // table it reads (the same values: no conflict), workgroup 0 also into the
+  // adjacency -- a move folded into the evaluation's launch
+  ResPatch patch;
 };
 
 __device__ __forceinline__ u32 wave_or(u32 v)
@@ -1296,6 +1325,13 @@ __global__ __launch_bounds__(1024) void k_resident_w(const ResTask* __restrict__
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), nw = nth >> 6;
   ResOut mine;           // this workgroup's totals (single-graph launches)
   mine.sum = 0; mine.diam = 0; mine.flags = 0; mine.levels = 0; mine.pad = 0;
+  if (fin.patch.n) {     // uniform: a kernel argument
+    if (tid == 0) {
+      res_patch(fin.patch, blockIdx.x == 0);
+      __threadfence();   // stored at L2 before any of this workgroup's table loads
+    }
+    __syncthreads();
+  }
   // the first gridDim.x blocks are dealt by workgroup index, the rest popped
   // from *qhead (which then counts past gridDim.x)
   int t = blockIdx.x;
