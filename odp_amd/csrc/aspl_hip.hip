@@ -273,6 +273,9 @@ struct Evaluator {
   const int* table_src = nullptr;
   // table entries the next resident launch changes first (triples row, position, id)
   std::vector<int> pend_patch;
+  // h_stage holds what the device copy holds (or will, once the pending entries are stored):
+  // false after moves made on the device only
+  bool stage_is_device = false;
   // the pending table entries as launch arguments (taken: the list is emptied)
   ResPatch make_patch()
   {
@@ -358,6 +361,8 @@ struct Evaluator {
   {
     drop_graph();
     table_fresh = false;
+    stage_is_device = false;
+    pend_patch.clear();
     dfree(&d_adj); dfree(&d_numdeg); dfree(&d_itable); dfree(&d_nbr); dfree(&d_err); dfree(&d_nbr16);
     dfree(&d_xy2row); dfree(&d_row2xy); dfree(&d_toff);
     nbr16_cap = 0; toff_cap = 0; geo_lmax = -1;
@@ -378,6 +383,8 @@ struct Evaluator {
     if (!st) st = C.stream;
     drop_graph();
     table_fresh = false;
+    stage_is_device = false;
+    pend_patch.clear();
     family = fam; n = nodes; degree = deg; s = sym; width = w; height = h; grid_s = gs;
     nsrc = n / s;
     dpad = round_up(deg > 0 ? deg : 1, 4);
@@ -455,10 +462,44 @@ struct Evaluator {
     return lm;
   }
 
+  // a few changed entries against the last upload (at most RES_PATCH_MAX, of a
+  // regular s = 1 table, every new id valid) under the resident plan with its
+  // table current: queue them for the next launch (which stores them into the
+  // table and the adjacency first) instead of sending and rebuilding the table.
+  // The annealing pattern -- a 2-opt move, then ODP_Set_aspl on the same array
+  // -- changes four entries.
+  bool upload_as_patch(const int* a)
+  {
+    if (plan.plan != ODPX_PLAN_RESIDENT || s != 1 || grid_s || has_numdeg || ext_adj || adj_on_host) return false;
+    if (!stage_is_device || !table_fresh || table_src != d_adj || !pend_patch.empty()) return false;
+    const size_t m = adj_elems(), chunk = 64;
+    int idx[RES_PATCH_MAX];
+    int k = 0;
+    for (size_t c = 0; c < m; c += chunk) {
+      const size_t len = std::min(chunk, m - c);
+      if (memcmp(a + c, h_stage + c, sizeof(int) * len) == 0) continue;
+      for (size_t i = c; i < c + len; i++)
+        if (a[i] != h_stage[i]) {
+          if (k == RES_PATCH_MAX || a[i] < 0 || a[i] >= n) return false;
+          idx[k++] = (int)i;
+        }
+    }
+    for (int q = 0; q < k; q++) {
+      const int i = idx[q];
+      h_stage[i] = a[i];
+      const int e[3] = {i / degree, i % degree, a[i]};
+      pend_patch.insert(pend_patch.end(), e, e + 3);
+    }
+    if (d_xy2row && k) geo_lmax = grid_lmax(h_stage);
+    return true;
+  }
+
   void upload(const void* adjacency)
   {
     trace("upload");
+    if (upload_as_patch((const int*)adjacency)) return;
     table_fresh = false;
+    stage_is_device = true;
     memcpy(h_stage, adjacency, sizeof(int) * adj_elems());
     if (plan.plan == ODPX_PLAN_RESIDENT && !ext_adj && sizeof(int) * adj_elems() <= ZC_MAX_BYTES) {
       adj_on_host = true;          // the next table build reads it from host memory
@@ -473,6 +514,7 @@ struct Evaluator {
   // called outside stream capture before anything but the resident table build reads d_adj
   void flush_adj()
   {
+    apply_pending();
     if (!adj_on_host) return;
     adj_on_host = false;
     HIPC(hipMemcpyAsync(d_adj, h_stage, sizeof(int) * adj_elems(), hipMemcpyHostToDevice, st));
@@ -1522,6 +1564,7 @@ extern "C" int ODPX_Swap_device(int a, int b, int c, int d)
   int ids[4] = {a, b, c, d};
   for (int x : ids) if (x < 0 || x >= v.n) return -1;
   v.flush_adj();
+  v.stage_is_device = false;         // the device copy moves on its own
   hipLaunchKernelGGL(k_swap, dim3(1), dim3(64), 0, C.stream, v.d_adj, v.degree, a, b, c, d, v.d_err, v.swap_table(),
                      v.dp16);
   HIPC(hipGetLastError());
@@ -1584,6 +1627,7 @@ extern "C" int ODPX_Anneal(void* adjacency, long long ncalcs, double max_temp, d
   const double cooling = hill_climbing ? 0.0 : pow(min_temp / max_temp, 1.0 / (double)ncalcs);
   double temp = max_temp;
   v.flush_adj();
+  v.stage_is_device = false;         // the moves change the device copy only
   // Under the resident plan (its table current) a move is not a launch of its
   // own: its four table entries ride along with the next evaluation's launch
   // (the host copy gives their positions).  Otherwise k_swap applies it.

@@ -270,6 +270,49 @@ def test_resident_table_follows_swaps_and_uploads():
         odp.finalize_aspl()
 
 
+def test_drop_in_small_changes_as_patches():
+    """ODP_Set_aspl on an array that differs from the last upload in a few
+    entries (the annealing pattern: a move, then Set_aspl on the same array)
+    queues those entries for the next launch instead of rebuilding the table.
+    Moves, restores, an unchanged array, two moves at once, more changes than
+    the launch takes, a device swap in between (the staging copy no longer
+    mirrors the device) and a plan change with entries pending must all give
+    the oracle's result."""
+    n, d = 600, 4
+    rng = odp.Rng(4)
+    host = odp.generate_regular(n, d, 61)
+    odp.init_aspl_general(n, d)
+    try:
+        odp.set_plan(**PLANS[1])
+        assert same(odp.set_aspl(host), oracle.evaluate(host, nodes=n))
+        for step in range(12):
+            saved = host.copy()
+            odp.two_opt(host, rng)                       # four entries change
+            assert same(odp.set_aspl(host), oracle.evaluate(host, nodes=n)), step
+            if step % 3 == 2:
+                host = saved                             # and back
+                assert same(odp.set_aspl(host), oracle.evaluate(host, nodes=n)), step
+        assert same(odp.set_aspl(host), oracle.evaluate(host, nodes=n))      # unchanged
+        odp.two_opt(host, rng)
+        odp.two_opt(host, rng)
+        assert same(odp.set_aspl(host), oracle.evaluate(host, nodes=n))      # up to eight entries
+        for _ in range(3):
+            odp.two_opt(host, rng)
+        assert same(odp.set_aspl(host), oracle.evaluate(host, nodes=n))      # more: full upload
+        dev = host.copy()
+        a, b, c, dd = odp.two_opt(dev, rng)
+        assert odp.swap_device(a, b, c, dd) == 0
+        assert same(odp.set_aspl_resident(), oracle.evaluate(dev, nodes=n))
+        assert same(odp.set_aspl(host), oracle.evaluate(host, nodes=n))      # device had moved on
+        h2 = host.copy()
+        odp.two_opt(h2, rng)
+        odp.upload_adjacency(h2)                         # queued as entries
+        odp.set_plan(**PLANS[4])                         # stores them before the level plan reads
+        assert same(odp.set_aspl_resident(), oracle.evaluate(h2, nodes=n))
+    finally:
+        odp.finalize_aspl()
+
+
 def test_small_upload_read_by_table_build():
     """Under the resident plan a small table reaches the device through the
     table build reading pinned host memory (no DMA); two uploads in a row, a
