@@ -3,11 +3,14 @@
 Default workload (--workload random65536): one seeded random 16-regular simple
 graph with n=65536, evaluated with the MATRIX method (the full 65536 x 65536 bit
 matrix, 512 MiB per buffer, every column block resident).  Also: grid256 (the
-256x256 4-fold symmetric grid, MATRIX_SAVING) and random1024 (n=1024, d=6).  A step is one full evaluation (every BFS level, exact sum, diameter
-and ASPL) of the adjacency already resident in HBM.  The JSON line's value is
-evaluations per second.
+256x256 4-fold symmetric grid, MATRIX_SAVING), random1024 (n=1024, d=6),
+batch1024 (the batched path alone) and anneal1024 (ODPX_Anneal steps).  A step
+is one full evaluation (every BFS level, exact sum, diameter and ASPL) of the
+adjacency already resident in HBM.  The JSON line's value is evaluations per
+second.
 
-The timed region replays each evaluation's captured hipGraph.  The same steps
+The timed region replays each level-plan evaluation's captured hipGraph (a
+resident-plan evaluation is one direct launch).  The same steps
 then run again with HIP events recorded around every launch on the library's
 launch stream (eager launches, the same launch sequence: count-only, listing and
 sparse last levels as they come).  roofline: the dominant kernel over those
