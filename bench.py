@@ -468,6 +468,12 @@ def main():
                 "alg_bytes_per_launch": st[pre + "_bytes"] / nl,
                 "launches": st[pre + "_launches"],
                 "timed_steps_ms_per_step": round(1e3 * el_live / args.steps, 4)}
+    if kern == "k_resident_w":
+        roofline["note"] = ("the bit rows live in LDS: the kernel is bound by LDS and launch latency, so this is "
+                            "not an HBM fraction")
+    elif kern == "k_tile":
+        roofline["note"] = ("several levels per launch from LDS: algorithmic bytes above the HBM peak are LDS "
+                            "reuse, not HBM traffic")
 
     out = {"metric": METRIC, "value": round(args.steps / el, 3), "unit": "evals/s", "n_gpus": 1,
            "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(1e3 * el / args.steps, 4),
