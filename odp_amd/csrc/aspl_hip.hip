@@ -21,6 +21,7 @@
 #include <sys/time.h>
 #include <vector>
 #include <algorithm>
+#include <mutex>
 #include "odp.h"
 #include "odpx.h"
 #include "kernels.hpp"
@@ -77,20 +78,30 @@ static inline int ilog2(int x) { int r = 0; while ((1 << (r + 1)) <= x) r++; ret
 // So every entry point that can reach the runtime runs with libc's
 // random()/rand() switched to a private state and switches the caller's back
 // on return (setstate keeps the caller's position; nested entries are fine).
+// Entries may overlap (nested calls, or the serving calls from several
+// threads): only the outermost switches, and the last one out switches back.
 static char g_rand_private[256];
 static bool g_rand_private_ready = false;
+static std::mutex g_rand_mu;
+static int g_rand_depth = 0;
+static char* g_rand_caller = nullptr;
 struct RandFence {
-  char* prev;
   RandFence()
   {
+    std::lock_guard<std::mutex> lk(g_rand_mu);
+    if (g_rand_depth++ > 0) return;
     if (!g_rand_private_ready) {
-      prev = initstate(1u, g_rand_private, sizeof g_rand_private);    // also makes it current
+      g_rand_caller = initstate(1u, g_rand_private, sizeof g_rand_private);    // also makes it current
       g_rand_private_ready = true;
     } else {
-      prev = setstate(g_rand_private);
+      g_rand_caller = setstate(g_rand_private);
     }
   }
-  ~RandFence() { setstate(prev); }
+  ~RandFence()
+  {
+    std::lock_guard<std::mutex> lk(g_rand_mu);
+    if (--g_rand_depth == 0) setstate(g_rand_caller);
+  }
   RandFence(const RandFence&) = delete;
   RandFence& operator=(const RandFence&) = delete;
 };
