@@ -639,3 +639,18 @@ def test_direct_levels(K, kw, group):
         ran_any = ran_any or ran >= 2
         assert ran == 0 or 2 <= ran <= K
     assert ran_any, "k_direct never applied"
+
+
+@pytest.mark.parametrize("n,d", [(66000, 6), (70002, 4)])
+def test_past_16bit_ids(n, d):
+    """Graphs just past 65536 vertices: no 16-bit id table, no resident plan;
+    the level plan's 32-bit tables and k_direct's 32-bit path (K = 4 and 6)."""
+    adj = odp.generate_regular(n, d, 99)
+    want = oracle.evaluate(adj, nodes=n)
+    odp.init_aspl_general(n, d)
+    try:
+        assert odp.get_plan()["plan"] == LEV
+        for _ in range(2):
+            assert same(odp.set_aspl(adj), want)
+    finally:
+        odp.finalize_aspl()
