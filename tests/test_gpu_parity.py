@@ -557,6 +557,23 @@ def test_invalid_vertex_id_exits():
     assert r.returncode == 1 and "outside [0, nodes)" in r.stderr
 
 
+def test_invalid_id_in_a_small_change_exits():
+    """A valid evaluation, then the same array with one entry made invalid:
+    the change is too small to resend the table, but an invalid id still takes
+    the full path and exits as the first evaluation would."""
+    code = (
+        "import sys; sys.path.insert(0, %r)\n"
+        "import odp_amd as o\n"
+        "a = o.generate_regular(200, 4, 5)\n"
+        "o.init_aspl_general(200, 4)\n"
+        "o.set_aspl(a)\n"
+        "print('first ok', flush=True)\n"
+        "a[17, 2] = 200\n"
+        "o.set_aspl(a)\n" % os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    r = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, timeout=300)
+    assert "first ok" in r.stdout and r.returncode == 1 and "outside [0, nodes)" in r.stderr
+
+
 @pytest.mark.parametrize("plan", [None, PLANS[7], PLANS[1]])
 def test_graph_replay_matches_eager(plan):
     """Evaluations replay a captured hipGraph unless kernel timing is on (eager
