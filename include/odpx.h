@@ -72,9 +72,20 @@ void ODPX_Upload_adjacency(void *adjacency);
 /* Evaluates the adjacency already on the device (after ODP_Set_aspl or
  * ODPX_Upload_adjacency, and any ODPX_Swap_device since). */
 void ODPX_Set_aspl_resident(int *diameter, long *sum, double *ASPL);
-/* Applies edge swap a-b, c-d -> a-c, b-d to the device copy (s == 1 only);
- * 0 on success, -1 if an edge is missing. */
+/* Applies edge swap a-b, c-d -> a-c, b-d to the device copy (s == 1 only),
+ * all or nothing: 0 on success; -1, with the graph unchanged, if an id is out of
+ * range, the four ids are not distinct, or edge a-b or c-d is missing. */
 int  ODPX_Swap_device(int a, int b, int c, int d);
+
+/* ---- algorithm kind (host only, no GPU) ---------------------------------- */
+/* The kind ODP_Init_aspl_* picks: ODP_ASPL if set (MATRIX, MATRIX_SAVING; BFS
+ * and unknown values exit as at Init), else MATRIX when the GPU memory estimate
+ * of MATRIX is at most 2048 MB, MATRIX_SAVING otherwise (utils.c:1238-1266 with
+ * is_cpu = false, procs = 1).  Returns 1 (MATRIX) or 2 (MATRIX_SAVING). */
+int    ODPX_Get_kind(int nodes, int degree, int has_num_degrees, int symmetries, int grid_s);
+/* The memory estimate in MB of that rule and of the ODP_PROFILE report
+ * (utils.c:1206-1236 with is_cpu = false, procs = 1). */
+double ODPX_Get_mem_usage(int kind, int nodes, int degree, int has_num_degrees, int symmetries, int grid_s);
 
 /* ---- kernel timing ------------------------------------------------------ */
 typedef struct {
@@ -83,6 +94,9 @@ typedef struct {
   long long level_launches;   /* gather-level launches that did work               */
   double    level_ms;         /* their summed duration, HIP events on the launch stream */
   double    level_bytes;      /* their algorithmic bytes: rows*((d+2)*k*8 + 4d)     */
+  double    level_min_bytes;  /* their compulsory bytes: every row of a working block
+                                 read once and (unless count-only) written once,
+                                 rows*k*8*(1 + written), plus the ids once per launch */
   long long idle_launches;    /* gather-level launches after every block finished  */
   double    idle_ms;
   long long resident_launches;

@@ -46,7 +46,7 @@ class Stats(ctypes.Structure):
     """ODPX_Stats (include/odpx.h)."""
     _fields_ = [("evals", ctypes.c_longlong), ("eval_ms", ctypes.c_double),
                 ("level_launches", ctypes.c_longlong), ("level_ms", ctypes.c_double),
-                ("level_bytes", ctypes.c_double),
+                ("level_bytes", ctypes.c_double), ("level_min_bytes", ctypes.c_double),
                 ("idle_launches", ctypes.c_longlong), ("idle_ms", ctypes.c_double),
                 ("resident_launches", ctypes.c_longlong), ("resident_ms", ctypes.c_double),
                 ("resident_bytes", ctypes.c_double),
@@ -126,6 +126,8 @@ def load():
     _sig(lib, "ODPX_Upload_adjacency", None, [_P])
     _sig(lib, "ODPX_Set_aspl_resident", None, [ip, ctypes.POINTER(_L), ctypes.POINTER(_D)])
     _sig(lib, "ODPX_Swap_device", _I, [_I, _I, _I, _I])
+    _sig(lib, "ODPX_Get_kind", _I, [_I, _I, _I, _I, _I])
+    _sig(lib, "ODPX_Get_mem_usage", _D, [_I, _I, _I, _I, _I, _I])
     _sig(lib, "ODPX_Anneal", _I, [_P, ctypes.c_longlong, _D, _D, _I, _I, _U64, ctypes.POINTER(AnnealResult)])
     _sig(lib, "ODPX_Stats_enable", None, [_I])
     _sig(lib, "ODPX_Stats_reset", None, [])
@@ -216,6 +218,16 @@ def set_aspl_resident():
 
 def swap_device(a, b, c, d):
     return load().ODPX_Swap_device(a, b, c, d)
+
+
+def get_kind(nodes, degree, has_num_degrees=False, symmetries=1, grid_s=False):
+    """The algorithm kind ODP_Init_aspl_* picks (1 MATRIX, 2 MATRIX_SAVING); host only."""
+    return load().ODPX_Get_kind(nodes, degree, int(bool(has_num_degrees)), symmetries, int(bool(grid_s)))
+
+
+def get_mem_usage(kind, nodes, degree, has_num_degrees=False, symmetries=1, grid_s=False):
+    """The memory estimate (MB) behind that choice and the ODP_PROFILE report; host only."""
+    return load().ODPX_Get_mem_usage(kind, nodes, degree, int(bool(has_num_degrees)), symmetries, int(bool(grid_s)))
 
 
 def get_plan():

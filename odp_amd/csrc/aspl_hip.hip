@@ -1174,17 +1174,24 @@ struct Evaluator {
       HIPC(hipEventElapsedTime(&ms, tl.a, tl.b));
       g_evpool.push_back(tl.a); g_evpool.push_back(tl.b);
       if (tl.kind == T_LEVEL) {
-        double bytes = 0;
+        double bytes = 0, minb = 0;
         // a level processes a share of the (row, segment) units: all of them, or
-        // in frontier mode the active ones; bytes are that share of the full level's
+        // in frontier mode the active ones; bytes are that share of the full level's.
+        // Compulsory bytes: each processed row of a working block read once and
+        // (unless the launch only counts) written once, the neighbour ids once per launch
         for (int t = first; t < first + cnt_t; t++)
           if (tl.level <= stv[t].lwork) {
             const double share = lv.fr || lv.tk
                                      ? (double)r.h_acnt[(size_t)(tl.level % RING) * r.ntasks + t] / ((double)n * r.segs)
                                      : 1.0;
             bytes += share * tl.nlev * (double)n * ((avg_deg + 1.0 + tl.wr) * ((r.l[t] + 63) / 64) * 8.0 + 4.0 * avg_deg);
+            minb += share * (double)n * ((r.l[t] + 63) / 64) * 8.0 * (1.0 + tl.wr);
           }
-        if (bytes > 0) { g_stats.level_launches++; g_stats.level_ms += ms; g_stats.level_bytes += bytes; }
+        if (minb > 0) minb += (double)n * avg_deg * 4.0;
+        if (bytes > 0) {
+          g_stats.level_launches++; g_stats.level_ms += ms; g_stats.level_bytes += bytes;
+          g_stats.level_min_bytes += minb;
+        }
         else { g_stats.idle_launches++; g_stats.idle_ms += ms; }
       } else {
         g_stats.other_launches++; g_stats.other_ms += ms;
@@ -1297,6 +1304,16 @@ static int ref_kind(int nodes, int degree, int symmetries, bool numdeg, bool gri
   if (strcmp(val, "BFS") == 0) ODP_ERROR("BFS for HIP is not implemented.\n");
   ODP_ERROR("Unknown ASPL value (%s)\n", val);
   return 0;
+}
+
+extern "C" int ODPX_Get_kind(int nodes, int degree, int has_num_degrees, int symmetries, int grid_s)
+{
+  return ref_kind(nodes, degree, symmetries, has_num_degrees != 0, grid_s != 0);
+}
+
+extern "C" double ODPX_Get_mem_usage(int kind, int nodes, int degree, int has_num_degrees, int symmetries, int grid_s)
+{
+  return ref_mem_mb(kind, nodes, degree, symmetries, has_num_degrees != 0, grid_s != 0);
 }
 
 // ---------------------------------------------------------------------------
@@ -1493,7 +1510,7 @@ extern "C" void ODP_Finalize_aspl(void)
     printf("Hostname        = %s\n", host);
     printf("Number of Times = %u\n", E.times);
     printf("Total Time      = %f sec.\n", E.elapsed);
-    printf("Average Time    = %f sec.\n", E.times ? E.elapsed / E.times : 0.0);
+    printf("Average Time    = %f sec.\n", E.elapsed / E.times);   // as utils.c:1284 (nan when never called)
     printf("Algorithm       = %s (HIP)\n", E.kind == KIND_MATRIX ? "MATRIX" : "MATRIX_SAVING");
     printf("Symmetries      = %d\n", E.ev.s);
     printf("Memory Usage    = %.3f MB\n", mem);
@@ -1545,25 +1562,31 @@ extern "C" void ODPX_Set_aspl_resident(int* diameter, long* sum, double* ASPL)
   E.times++;
 }
 
-// 2-opt swap a-b, c-d -> a-c, b-d on the device adjacency (s = 1).  With
-// `table` set it also patches the resident plan's 16-bit table, entry for entry:
-// the table holds the adjacency's entry where it is a neighbour and the row
-// itself where it is padding, so an entry equal to `from` is the same edge.
+// 2-opt swap a-b, c-d -> a-c, b-d on the device adjacency (s = 1), all or
+// nothing: one thread finds the four entries first (rows a, b, c, d holding b,
+// a, d, c) and writes only when all four exist; otherwise it changes nothing
+// and sets bit 4 of *err.  The ids must be distinct (the host checks), so the
+// four entries are four different cells.  With `table` set it also patches the
+// resident plan's 16-bit table, entry for entry: the table holds the
+// adjacency's entry where it is a neighbour and the row itself where it is
+// padding, so an entry equal to `from` is the same edge.
 __global__ void k_swap(int* adj, int degree, int a, int b, int c, int d, int* err, unsigned short* table, int tdp)
 {
-  const int t = threadIdx.x;
-  if (t >= 4) return;
-  const int row = t == 0 ? a : t == 1 ? b : t == 2 ? c : d;
-  const int from = t == 0 ? b : t == 1 ? a : t == 2 ? d : c;
-  const int to = t == 0 ? c : t == 1 ? d : t == 2 ? a : b;
-  int* r = adj + (int64_t)row * degree;
-  for (int j = 0; j < degree; j++)
-    if (r[j] == from) {
-      r[j] = to;
-      if (table && table[(int64_t)row * tdp + j] == (unsigned short)from) table[(int64_t)row * tdp + j] = (unsigned short)to;
-      return;
-    }
-  atomicOr(err, 4);
+  if (threadIdx.x != 0) return;
+  const int rows[4] = {a, b, c, d}, from[4] = {b, a, d, c}, to[4] = {c, d, a, b};
+  int pos[4];
+  for (int e = 0; e < 4; e++) {
+    const int* r = adj + (int64_t)rows[e] * degree;
+    pos[e] = -1;
+    for (int j = 0; j < degree; j++)
+      if (r[j] == from[e]) { pos[e] = j; break; }
+    if (pos[e] < 0) { atomicOr(err, 4); return; }
+  }
+  for (int e = 0; e < 4; e++) {
+    adj[(int64_t)rows[e] * degree + pos[e]] = to[e];
+    unsigned short* t = table ? table + (int64_t)rows[e] * tdp + pos[e] : nullptr;
+    if (t && *t == (unsigned short)from[e]) *t = (unsigned short)to[e];
+  }
 }
 
 extern "C" int ODPX_Swap_device(int a, int b, int c, int d)
@@ -1574,6 +1597,7 @@ extern "C" int ODPX_Swap_device(int a, int b, int c, int d)
   if (v.s != 1) return -1;
   int ids[4] = {a, b, c, d};
   for (int x : ids) if (x < 0 || x >= v.n) return -1;
+  if (a == b || a == c || a == d || b == c || b == d || c == d) return -1;   // not a 2-opt move
   v.flush_adj();
   v.stage_is_device = false;         // the device copy moves on its own
   hipLaunchKernelGGL(k_swap, dim3(1), dim3(64), 0, C.stream, v.d_adj, v.degree, a, b, c, d, v.d_err, v.swap_table(),

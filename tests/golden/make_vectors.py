@@ -14,9 +14,10 @@ ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "oracle"))
 sys.path.insert(0, os.path.join(ROOT, "tests"))
+sys.path.insert(0, ROOT)
 
 import oracle  # noqa: E402
-from goldens import build_case  # noqa: E402
+from goldens import C4_SAMPLE, C5_SAMPLE, build_case  # noqa: E402
 
 CASES = [
     # single general graphs; the first three are the n=1024 d=6 workload
@@ -52,11 +53,27 @@ CASES = [
     dict(graph="regular", nodes=65536, degree=16, seed=1),
     dict(graph="grid", width=256, height=256, degree=8, length=4, symmetries=4, seed=1),
 ]
+# graphs of the batched workload (1024 candidates) and served chains after three
+# requests of one swap each (tests/goldens.py builds them from these parameters)
+CASES += [dict(graph="batch", index=g, batch_seed=2024) for g in C4_SAMPLE]
+CASES += [dict(graph="chain", chain=i, swaps=3, rng=9000 + i) for i in C5_SAMPLE]
+
+
+def _key(c):
+    return json.dumps({k: v for k, v in c.items() if k not in ("diameter", "sum", "aspl")}, sort_keys=True)
 
 
 def main():
+    """Computes the cases not yet in vectors.json (all with --all)."""
+    path = os.path.join(HERE, "vectors.json")
+    have = {}
+    if os.path.exists(path) and "--all" not in sys.argv:
+        have = {_key(c): c for c in json.load(open(path))["cases"]}
     out = []
     for c in CASES:
+        if _key(c) in have:
+            out.append(have[_key(c)])
+            continue
         kw, adj = build_case(c)
         t = time.time()
         (d, s, a), _ = oracle.run_reference(adj, threads=True, timeout=3600, **kw)
@@ -65,7 +82,7 @@ def main():
         out.append(rec)
     json.dump({"_comment": "expected values from the reference library (oracle/_ref, OpenMP build, default kind) "
                            "on graphs regenerated from the listed generator parameters",
-               "cases": out}, open(os.path.join(HERE, "vectors.json"), "w"), indent=1)
+               "cases": out}, open(path, "w"), indent=1)
 
 
 if __name__ == "__main__":

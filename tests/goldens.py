@@ -70,7 +70,35 @@ def build_case(c):
         w, h, s = c["width"], c["height"], c["symmetries"]
         full = odp.generate_grid(w, h, c["degree"], c["length"], s, c["seed"])
         return dict(width=w, height=h, symmetries=s), odp.grid_base_table(w, h, c["degree"], s, full)
+    if k == "batch":
+        # graph `index` of the batched workload (bench.c4_params / c4_graph)
+        import bench
+        n, d = bench.c4_params(c["index"] + 1, c["batch_seed"])[c["index"]]
+        return dict(nodes=n), bench.c4_graph(c["index"], n, d, c["batch_seed"])
+    if k == "chain":
+        # served chain `chain` (bench.c5_chain) after `swaps` 2-opt moves drawn from Rng(rng)
+        import bench
+        adj = bench.c5_chain(c["chain"])
+        rng = odp.Rng(c["rng"])
+        for _ in range(c["swaps"]):
+            odp.two_opt(adj, rng)
+        return dict(nodes=adj.shape[0]), adj
     raise ValueError(k)
+
+
+def case_nodes(c):
+    """Vertex count of a vectors.json case."""
+    if c["graph"] == "batch":
+        import bench
+        return bench.c4_params(c["index"] + 1, c["batch_seed"])[c["index"]][0]
+    if c["graph"] == "chain":
+        import bench
+        return bench.C5_NODES[c["chain"] % 3]
+    return c.get("nodes") or c["width"] * c["height"]
+
+
+C4_SAMPLE = [0, 59, 63, 341, 618, 771, 837, 868]   # n from 1036 to 16332, every d of the workload
+C5_SAMPLE = list(range(9))                          # chains 0..8: every (n, d) class once
 
 
 # ---- closed-form graphs ------------------------------------------------------

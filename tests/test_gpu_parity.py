@@ -13,7 +13,7 @@ import odp_amd as odp
 import oracle
 from goldens import (TEST_GENERAL, TEST_GRID, GOLDEN, build_case, complete_graph, cycle_graph, cycle_sum,
                      general_fixture, grid_fixture, hypercube, parse_aspl_txt, path_graph, petersen,
-                     symmetric_fixtures, two_triangles, vectors)
+                     symmetric_fixtures, two_triangles, vectors, case_nodes)
 
 pytestmark = pytest.mark.gpu
 
@@ -41,7 +41,23 @@ PLANS = [
 ]
 
 
+LDS_DYN_MAX = 163840 - 512          # LDS one resident workgroup may hold for its block
+
+
+def plan_fits(plan, n):
+    """Whether the library must accept `plan` for a graph of n vertices: a
+    resident block of 32*wb sources needs n*wb*4 bytes of LDS, 16-bit ids
+    (n <= 65535) and at most res_rpt(wb) rows per thread; level plans always fit."""
+    if plan is None or plan["plan"] != RES:
+        return True
+    wb, th = plan["wb"], plan["threads"]
+    rpt = 16 if wb == 1 else (2 if wb == 8 else 32 // wb)
+    return n <= 65535 and n * wb * 4 <= LDS_DYN_MAX and n <= rpt * th
+
+
 def hip_eval(adj, nodes=None, width=None, height=None, symmetries=1, num_degrees=None, plan=None):
+    """One Init / Set_aspl / Finalize cycle; None when `plan` does not fit the
+    graph, after checking that the library refuses exactly those plans."""
     if width is None:
         if symmetries == 1:
             odp.init_aspl_general(nodes, adj.shape[1], num_degrees)
@@ -54,10 +70,12 @@ def hip_eval(adj, nodes=None, width=None, height=None, symmetries=1, num_degrees
             odp.init_aspl_grid_s(width, height, adj.shape[1], num_degrees, symmetries)
     try:
         if plan is not None:
-            try:
-                odp.set_plan(**plan)
-            except ValueError:
-                return None      # plan does not fit this graph (LDS)
+            n = nodes if width is None else width * height
+            if not plan_fits(plan, n):
+                with pytest.raises(ValueError):
+                    odp.set_plan(**plan)
+                return None
+            odp.set_plan(**plan)
         return odp.set_aspl(adj)
     finally:
         odp.finalize_aspl()
@@ -139,7 +157,7 @@ def test_disconnected_sentinels(plan):
     assert got == (INT_MAX, LONG_MAX, DBL_MAX)
 
 
-SMALL_VECTORS = [c for c in vectors() if (c.get("nodes") or c["width"] * c["height"]) <= 30000]
+SMALL_VECTORS = [c for c in vectors() if case_nodes(c) <= 30000]
 
 
 @pytest.mark.parametrize("case", SMALL_VECTORS, ids=lambda c: "-".join(str(v) for k, v in c.items()
@@ -236,6 +254,20 @@ def test_device_swap_matches_host():
             assert odp.swap_device(a, b, c, dd) == 0
             assert same(odp.set_aspl_resident(), oracle.evaluate(host, nodes=n))
         assert odp.swap_device(0, 0, 0, 0) == -1      # missing edge is rejected
+        # half valid: a-b is an edge, c-d is not -- nothing may change
+        a, b = 5, int(host[5][0])
+        c = next(v for v in range(n) if v not in (a, b) and v not in host[a] and v not in host[b])
+        dd = next(v for v in range(n) if v not in (a, b, c) and v not in host[c])
+        assert odp.swap_device(a, b, c, dd) == -1
+        # ids that are not distinct (b == c, a == d) are not a 2-opt move
+        c2 = int(host[b][0]) if int(host[b][0]) != a else int(host[b][1])
+        assert odp.swap_device(a, b, b, c2) == -1
+        assert odp.swap_device(a, b, int(host[c][0]), a) == -1
+        assert same(odp.set_aspl_resident(), oracle.evaluate(host, nodes=n))
+        # and the graph still takes valid moves afterwards
+        a, b, c, dd = odp.two_opt(host, rng)
+        assert odp.swap_device(a, b, c, dd) == 0
+        assert same(odp.set_aspl_resident(), oracle.evaluate(host, nodes=n))
         odp.finalize_aspl()
 
 

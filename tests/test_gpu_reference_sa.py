@@ -9,6 +9,7 @@ same trajectory -- every progress line, the best diameter, sum and ASPL -- and
 differ only in the timing line.
 """
 import os
+import re
 import subprocess
 
 import pytest
@@ -116,3 +117,46 @@ def test_libc_random_stream_untouched():
     odp.batch_eval(odp.pack([adj, adj]))
     got = [libc.rand() for _ in range(5)]
     assert got == want
+
+
+def _profile_block(text):
+    lines = text.splitlines()
+    i = lines.index("------ Profile for SET_ASPL ------")
+    j = lines.index("--------- End of Profile ---------")
+    return lines[i:j + 1]
+
+
+def test_profile_report_layout(tmp_path):
+    """ODP_PROFILE=1: the report printed at ODP_Finalize_aspl has the reference's
+    layout (utils.c:1268-1294) line for line.  Labels, the call count, the
+    symmetry factor, the process and thread counts are equal to the reference
+    program's; the algorithm line names this library's flavour; the memory line
+    is the reference's GPU-branch estimate (utils.c:1206-1236 with is_cpu =
+    false, as aspl_cuda.cu:137 computes it) to three decimals."""
+    prog = "general"
+    if not (os.path.exists(os.path.join(REF, f"smp_{prog}_ref")) and os.path.exists(os.path.join(REF, f"smp_{prog}_hip"))):
+        pytest.skip("reference sample programs not built (oracle/Makefile `sa` needs the reference sources)")
+    infile = os.path.join(GOLDEN, "general", "n256d10.edges")
+    env = dict(os.environ, ODP_PROFILE="1")
+    env.pop("ODP_ASPL", None)
+    blocks = []
+    for lib in ("ref", "hip"):
+        r = subprocess.run([os.path.join(REF, f"smp_{prog}_{lib}"), infile], capture_output=True, text=True,
+                           timeout=120, cwd=tmp_path, env=env)
+        assert r.returncode == 0, r.stderr[-2000:]
+        blocks.append(_profile_block(r.stdout))
+    ref, hip = blocks
+    assert len(ref) == len(hip) == 12
+    label = lambda ln: ln.split("=")[0]
+    assert [label(a) for a in ref] == [label(b) for b in hip]
+    value = lambda blk, name: next(ln.split("=", 1)[1].strip() for ln in blk if ln.startswith(name))
+    for name in ("Number of Times", "Symmetries", "Num of Procs", "Num of Threads"):
+        assert value(ref, name) == value(hip, name), name
+    assert value(ref, "Algorithm") == "MATRIX (SERIAL)" and value(hip, "Algorithm") == "MATRIX (HIP)"
+    for name in ("Total Time", "Average Time"):
+        assert re.fullmatch(r"[0-9]+\.[0-9]{6} sec\.", value(hip, name)), name
+    kind_exe = os.path.join(REF, "odp_ref_kind")
+    if os.path.exists(kind_exe):
+        r = subprocess.run([kind_exe], input="256 10 0 1 0\n", capture_output=True, text=True, timeout=60)
+        mem = float.fromhex(r.stdout.split()[1])
+        assert value(hip, "Memory Usage") == f"{mem:.3f} MB"

@@ -186,3 +186,44 @@ def test_error_grid_s_shape():
 def test_set_before_init_exits():
     r = _run_expect_exit("import numpy as np; o.set_aspl(np.zeros((2, 1), np.int32))")
     assert r.returncode == 1 and "before ODP_Init_aspl" in r.stderr
+
+
+REF_KIND = os.path.join(ROOT, "oracle", "_ref", "odp_ref_kind")
+
+
+def _ref_kinds(shapes, env=None):
+    full = dict(os.environ)
+    full.pop("ODP_ASPL", None)
+    full.update(env or {})
+    text = "".join(f"{n} {d} {nd} {s} {g}\n" for (n, d, nd, s, g) in shapes)
+    r = subprocess.run([REF_KIND], input=text, capture_output=True, text=True, env=full, timeout=120)
+    assert r.returncode == 0, r.stderr
+    return [(int(k), float.fromhex(m1), float.fromhex(m2)) for k, m1, m2 in (ln.split() for ln in r.stdout.splitlines())]
+
+
+@pytest.mark.skipif(not os.path.exists(REF_KIND), reason="reference kind checker not built (needs the reference sources)")
+def test_kind_rule_matches_reference(monkeypatch):
+    """The kind Init picks without ODP_ASPL, and the memory estimate behind it,
+    equal the reference's own rule (utils.c:1206-1266, GPU branch) on both sides
+    of the 2048 MB threshold, for every argument the estimate depends on."""
+    monkeypatch.delenv("ODP_ASPL", raising=False)
+    shapes = []
+    for s, grid_s in ((1, 0), (2, 0), (2, 1), (3, 0), (4, 0), (4, 1)):
+        nb = int((8 * 2**30 * s) ** 0.5)             # n^2 / (4 s) bytes = 2048 MB
+        for d in (3, 16, 64):
+            for nd in (0, 1):
+                for n in range(nb - 40 * s, nb + 40 * s, s):
+                    shapes.append((n - n % s, d, nd, s, grid_s))
+    ref = _ref_kinds(shapes)
+    kinds = set()
+    for (n, d, nd, s, g), (k, m1, m2) in zip(shapes, ref):
+        assert odp.get_kind(n, d, nd, s, g) == k, (n, d, nd, s, g)
+        assert odp.get_mem_usage(1, n, d, nd, s, g) == m1 and odp.get_mem_usage(2, n, d, nd, s, g) == m2
+        kinds.add(k)
+    assert kinds == {1, 2}                          # both sides of the threshold were exercised
+    # ODP_ASPL overrides the threshold on both sides
+    for val, want in (("MATRIX", 1), ("MATRIX_SAVING", 2)):
+        monkeypatch.setenv("ODP_ASPL", val)
+        few = [shapes[0], shapes[-1]]
+        assert [k for k, _, _ in _ref_kinds(few, {"ODP_ASPL": val})] == [want, want]
+        assert [odp.get_kind(*sh) for sh in few] == [want, want]

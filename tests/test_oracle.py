@@ -17,7 +17,7 @@ import odp_amd as odp
 import oracle
 from goldens import (TEST_GENERAL, TEST_GRID, GOLDEN, build_case, complete_graph, cycle_graph, cycle_sum,
                      general_fixture, grid_fixture, hypercube, parse_aspl_txt, path_graph, petersen,
-                     symmetric_fixtures, two_triangles, vectors)
+                     symmetric_fixtures, two_triangles, vectors, case_nodes)
 
 INT_MAX, LONG_MAX = 2**31 - 1, 2**63 - 1
 KINDS = [(oracle.MATRIX, 1), (oracle.MATRIX_SAVING, 1), (oracle.MATRIX_SAVING, 3),
@@ -92,7 +92,7 @@ def closed_forms_ok():
 
 def vectors_ok(max_nodes=20000):
     for c in vectors():
-        if (c.get("nodes") or c["width"] * c["height"]) > max_nodes:
+        if case_nodes(c) > max_nodes:
             continue
         kw, adj = build_case(c)
         r = oracle.evaluate(adj, kind=oracle.MATRIX_SAVING, chunk=8, **kw)
