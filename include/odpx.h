@@ -55,6 +55,9 @@ typedef struct {
   int batch;      /* level: neighbour rows gathered per batch: 4 (also 0), 8 or 16     */
   int tile;       /* level, grid graphs (kw = 2): 1 = temporal tiles, several levels per
                      launch in LDS when the entries are short enough; 0 = off           */
+  int parts;      /* level, 16-word blocks: 0 = one gather per row segment (k_level);
+                     2 or 4 = the XCD-phased gather over that many row parts of
+                     the input slab (k_pgather)                                         */
 } ODPX_Plan;
 
 /* Plan of the current evaluator (after Init).  Set: 0 on success, -1 if invalid.

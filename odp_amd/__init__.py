@@ -25,7 +25,7 @@ class Plan(ctypes.Structure):
     _fields_ = [("plan", ctypes.c_int), ("wb", ctypes.c_int), ("threads", ctypes.c_int),
                 ("kw", ctypes.c_int), ("upw", ctypes.c_int), ("seg_major", ctypes.c_int),
                 ("xcd", ctypes.c_int), ("group", ctypes.c_int), ("direct", ctypes.c_int),
-                ("batch", ctypes.c_int), ("tile", ctypes.c_int)]
+                ("batch", ctypes.c_int), ("tile", ctypes.c_int), ("parts", ctypes.c_int)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}

@@ -52,6 +52,8 @@ static const size_t LDS_STATIC = 512;           // k_resident's own __shared__ d
 static const size_t LDS_DYN_MAX = LDS_LIMIT - LDS_STATIC;
 static const int RING = 260;                    // per-level counter slots kept on the device
 static const int MAX_BATCH_LEVELS = 256;        // levels launched between two host reads (< RING - 2)
+static const int PG_PARTS = 4;                  // row parts of the XCD-phased gather (2 MiB of a 65536-row slab)
+static const int PG_MIN_ROWS = 32768;           // ... used by default from this many rows (a 4 MiB slab)
 
 // ODP_HIP_TRACE=1 prints each host step to stderr (diagnostics)
 static int g_trace = -1;
@@ -163,6 +165,14 @@ static bool graphs_env()
   return env != 0;
 }
 
+// cache policy of k_pgather's gathers (env ODP_HIP_PG_AUX: 0 default, 1 sc0, 16 sc1, 2 nt)
+static int pg_aux_env()
+{
+  const char* v = getenv("ODP_HIP_PG_AUX");
+  const int a = v ? atoi(v) : 0;
+  return (a == 1 || a == 16 || a == 2) ? a : 0;
+}
+
 // capacity of the unsaturated-chunk list (env ODP_HIP_LIST_CAP lowers it; a
 // level that lists more chunks than that is counted by visiting every chunk)
 static u32 list_cap_env()
@@ -239,6 +249,10 @@ struct LevelRun {
   u32 list_cap = 0;
   u64* h_acnt = nullptr;                // processed-unit counts on the host (kernel timing only)
   size_t flag_bytes = 0;
+  // part-phased gather (k_pgather): row parts, sorted neighbour runs, persistent grid
+  int parts = 0, psp = 0, pgrid = 0, pupt = 0, paux = 0;
+  u32* plist = nullptr; int* ppoff = nullptr;  // per unit of PG_UNIT rows: pairs grouped by part, offsets
+  size_t plist_cap = 0, ppoff_cap = 0;
   LevelTask* tasks = nullptr; int* wg_start = nullptr;
   u64* h_cnt = nullptr;
   std::vector<int> l;                   // sources per task
@@ -380,6 +394,7 @@ struct Evaluator {
     hfree(&h_stage); hfree(&h_err); h_stage_dev = nullptr; adj_on_host = false;
     dfree(&lr.bufs); dfree(&lr.cnt); dfree(&lr.done); dfree(&lr.tasks); dfree(&lr.wg_start); hfree(&lr.h_cnt);
     dfree(&lr.flags); hfree(&lr.h_acnt); dfree(&lr.wgm); dfree(&lr.list); dfree(&lr.list_n); dfree(&lr.tq);
+    dfree(&lr.plist); dfree(&lr.ppoff);
     dfree(&rr.tasks); dfree(&rr.out); dfree(&rr.qhead); dfree(&rr.done); hfree(&rr.host); rr.host_dev = nullptr;
     dfree(&rr.part); rr.part_cap = 0;
     lr = LevelRun(); rr = ResRun();
@@ -594,11 +609,14 @@ struct Evaluator {
     p.xcd = 1;
     p.group = (kind == KIND_MATRIX) ? 0 : SAVING_GROUP;
     p.direct = -1;
+    // slabs larger than an XCD's L2 (n x 128 B > 4 MB): the XCD-phased gather
+    p.parts = (p.kw == LEVEL_BLOCK_WORDS && n >= PG_MIN_ROWS) ? PG_PARTS : 0;
     if (tile_capable()) {
       // grid graphs: 2-word blocks and temporal tiles (k_tile); MATRIX_SAVING
       // keeps the same number of source words resident as with 16-word blocks
       p.kw = 2;
       p.tile = 1;
+      p.parts = 0;
       if (p.group) p.group = SAVING_GROUP * LEVEL_BLOCK_WORDS / 2;
     }
     return p;
@@ -616,12 +634,15 @@ struct Evaluator {
       if (p.tile != 0 && !(p.tile == 1 && p.kw == 2 && tile_capable())) return false;
       // direct K that does not apply to this graph's blocks falls back to k_level0
       if (p.batch != 0 && p.batch != 4 && p.batch != 8 && p.batch != 16) return false;
+      if (p.parts != 0 && p.parts != 2 && p.parts != 4) return false;
       return p.direct == 0 || p.direct == -1 || (p.direct >= 2 && p.direct <= DIRECT_KMAX);
     }
     return false;
   }
 
   // ---- direct enumeration of the first levels (k_direct) ----
+  // gather levels run k_pgather: a plan with parts, plain (not frontier, not tile) levels
+  bool pg_on() const { return plan.plan == ODPX_PLAN_LEVEL && lr.parts > 0 && !lv.fr && !lv.tk; }
   // k_direct reads the 16-bit table when it exists with the 32-bit table's width
   bool direct16() const
   {
@@ -719,6 +740,25 @@ struct Evaluator {
     if (wbytes > r.wgm_bytes) {
       dfree(&r.wgm); dalloc(&r.wgm, wbytes); r.wgm_bytes = wbytes;
       HIPC(hipMemset(r.wgm, 0, wbytes));
+    }
+    // the XCD-phased gather applies to 16-word rows and runs of at most 63 ids
+    // the XCD-phased gather applies to 16-word rows, runs of at most 63 ids and
+    // ids below 2^22 (the pair lists pack the row in the unit beside the id)
+    r.parts = (plan.parts > 0 && kwp == 16 && dpad + 1 <= 63 && n <= (1 << 22)) ? plan.parts : 0;
+    plan.parts = r.parts;              // the plan reports what runs
+    if (r.parts) {
+      r.psp = round_up(dpad + 1, 4);
+      r.pupt = (n + PG_UNIT - 1) / PG_UNIT;
+      const size_t nl = (size_t)r.pupt * PG_UNIT * r.psp, np = (size_t)r.pupt * (PG_MAXP + 1);
+      if (nl > r.plist_cap) { dfree(&r.plist); dalloc(&r.plist, nl); r.plist_cap = nl; }
+      if (np > r.ppoff_cap) { dfree(&r.ppoff); dalloc(&r.ppoff, np); r.ppoff_cap = np; }
+      r.paux = pg_aux_env();
+      // persistent grid: every workgroup the CUs hold at once, a multiple of the XCD count
+      int occ = 0, occ_ls = 0;
+      HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, k_pgather<false, 0>, PG_WG, 0));
+      HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ_ls, k_pgather<true, 0>, PG_WG, 0));
+      occ = std::max(1, std::min(occ, occ_ls));
+      r.pgrid = ODP_NXCD * std::max(1, C.ncu / ODP_NXCD) * occ;
     }
     int K = plan.direct;
     while (K >= 2 && (direct_tree(K) > DIRECT_TREE_CAP || !direct_waves(K, kw, group))) K--;
@@ -975,6 +1015,17 @@ struct Evaluator {
     if (mode & LV_SPARSE) {
       hipLaunchKernelGGL(k_sparse, dim3(std::min(grid, SPARSE_GRID)), dim3(LEVEL_WG), 0, st,
                          r.tasks + lv.first, lv.cnt_t, L, RING, lst);
+    } else if (pg_on()) {
+      PgArgs a;
+      a.list = r.plist; a.poff = r.ppoff; a.n = n; a.sp = r.psp; a.P = r.parts;
+      a.upt = r.pupt;
+      a.units = (long long)lv.cnt_t * a.upt;
+      const bool ls = (mode & LV_LIST) != 0;
+      auto kern = r.paux == 1 ? (ls ? k_pgather<true, 1> : k_pgather<false, 1>)
+                : r.paux == 16 ? (ls ? k_pgather<true, 16> : k_pgather<false, 16>)
+                : r.paux == 2 ? (ls ? k_pgather<true, 2> : k_pgather<false, 2>)
+                : (ls ? k_pgather<true, 0> : k_pgather<false, 0>);
+      hipLaunchKernelGGL(kern, dim3(r.pgrid), dim3(PG_WG), 0, st, r.tasks + lv.first, L, RING, mode, lst, a);
     } else {
       const bool ls = (mode & LV_LIST) != 0;
       auto kern = lv.fr
@@ -1067,6 +1118,12 @@ struct Evaluator {
       const int64_t total = (int64_t)n * dpad;
       const int grid = (int)std::min<int64_t>(std::max<int64_t>(1, (total + 255) / 256), 4096);
       hipLaunchKernelGGL(k_symcheck, dim3(grid), dim3(256), 0, st, d_nbr, n, dpad, r.done + r.ntasks);
+      HIPC(hipGetLastError());
+    }
+    if (pg_on()) {
+      // the units' (row, neighbour) pairs grouped by part for k_pgather
+      auto kern = dpad + 1 <= 20 ? k_plist<20> : dpad + 1 <= 36 ? k_plist<36> : k_plist<64>;
+      hipLaunchKernelGGL(kern, dim3(r.pupt), dim3(PG_UNIT), 0, st, d_nbr, n, dpad, r.psp, r.parts, r.plist, r.ppoff);
       HIPC(hipGetLastError());
     }
   }

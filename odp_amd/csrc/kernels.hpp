@@ -33,6 +33,7 @@ typedef unsigned long long u64;
 typedef unsigned int u32;
 
 struct __attribute__((aligned(16))) u64x2 { u64 x, y; };
+typedef u64 u64v2 __attribute__((ext_vector_type(2)));
 
 #define ODP_NXCD 8          // XCDs of an MI355X; workgroups are dealt round-robin
 #define LEVEL_WG 256        // threads per level-kernel workgroup (4 waves)
@@ -484,6 +485,221 @@ __global__ __launch_bounds__(LEVEL_WG) void k_level(const LevelTask* __restrict_
     nact = wave_sum(nact);
     if (lane == 0 && nch) atomicAdd(&T.ccnt[(int64_t)(level % ring) * T.cstride], nch);
     if (lane == 0 && nact) atomicAdd(&T.acnt[(int64_t)(level % ring) * T.cstride], nact);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// part-phased gather level (k_pgather): the plain level of k_level for blocks
+// of 16 words (128-byte rows), arranged so that one XCD's L2 holds what its
+// workgroups gather from.
+//
+// A block's input slab (n rows x 128 B: 8 MiB at n = 65536) does not fit the
+// 4 MB L2 of an XCD, and random neighbours spread the gathers over all of it.
+// So the rows of the slab are cut into P parts (rows [p n / P, (p+1) n / P)).
+// Rows are taken in units of PG_UNIT consecutive rows; for every unit and part
+// a list holds the unit's (row, neighbour) pairs whose neighbour lies in that
+// part (the row itself counts as a neighbour; repeats are dropped), built once
+// per evaluation by k_plist.  Workgroups are persistent and grouped
+// by XCD (workgroup b runs on XCD b % 8): the workgroups of one XCD take
+// consecutive units -- the same block -- and sweep the parts in order, with
+// their rows' next values accumulated in LDS (ds_or) across the sweep.  While
+// they are in part p, the XCD's gathers fall into one n/P-row piece of the slab
+// (2 MiB at P = 4), which its L2 keeps.  The gathers need no masking (every
+// pair is a 128-byte row read) and none waits for another.  Counts, stop test
+// and listing are k_level's (the plain variant): every result is bit-identical.
+
+#define PG_MAXP 4
+#define PG_UNIT 1024         // rows of a unit: one 1024-thread workgroup, 128 KiB of LDS rows
+#define PG_WG 1024
+
+struct PgArgs {
+  const u32* list;       // per unit, parts in order: (row in unit) << 22 | neighbour
+  const int* poff;       // per unit: P + 1 offsets into its list (relative to unit * PG_UNIT * sp)
+  int n, sp, P;
+  int upt;               // units per task: ceil(n / PG_UNIT)
+  long long units;       // tasks x upt
+};
+
+// One workgroup of PG_UNIT threads per unit, a thread per row: the row's
+// distinct neighbours and the row itself (padding entries are the row), their
+// parts, a scan of the rows' run lengths per part, and the unit's pairs
+// grouped by part (in any order within a part) with the part offsets.
+// DMAX >= dpad + 1 bounds the ids a row holds in registers.
+template <int DMAX>
+__global__ __launch_bounds__(PG_UNIT) void k_plist(const int* __restrict__ nbr, int n, int dpad, int sp, int P,
+                                                   u32* __restrict__ list, int* __restrict__ poff)
+{
+  __shared__ int scan[PG_MAXP][PG_UNIT];
+  __shared__ int tot[PG_MAXP];
+  const int u = blockIdx.x, t = threadIdx.x, r = u * PG_UNIT + t;
+  int id[DMAX];
+  int run[PG_MAXP];
+#pragma unroll
+  for (int p = 0; p < PG_MAXP; p++) run[p] = 0;
+#pragma unroll
+  for (int j = 0; j < DMAX; j++) {
+    id[j] = -1;
+    if (r < n && j <= dpad) id[j] = j < dpad ? nbr[(int64_t)r * dpad + j] : r;
+  }
+  // drop repeats (an id equal to an earlier one), count the rest per part
+#pragma unroll
+  for (int j = 1; j < DMAX; j++) {
+    bool dup = false;
+#pragma unroll
+    for (int k = 0; k < j; k++) dup = dup || id[k] == id[j];
+    if (dup) id[j] = -1;
+  }
+#pragma unroll
+  for (int j = 0; j < DMAX; j++)
+    if (id[j] >= 0) {
+      const int q = (int)(((int64_t)id[j] * P) / n);
+#pragma unroll
+      for (int p = 0; p < PG_MAXP; p++) run[p] += (p == q) ? 1 : 0;
+    }
+#pragma unroll
+  for (int p = 0; p < PG_MAXP; p++) scan[p][t] = run[p];
+  __syncthreads();
+  // inclusive scans (Hillis-Steele) over the unit's rows, one per part
+  for (int o = 1; o < PG_UNIT; o <<= 1) {
+    int add[PG_MAXP];
+#pragma unroll
+    for (int p = 0; p < PG_MAXP; p++) add[p] = t >= o ? scan[p][t - o] : 0;
+    __syncthreads();
+#pragma unroll
+    for (int p = 0; p < PG_MAXP; p++) scan[p][t] += add[p];
+    __syncthreads();
+  }
+  if (t < PG_MAXP) tot[t] = scan[t][PG_UNIT - 1];
+  __syncthreads();
+  u32* ul = list + (int64_t)u * PG_UNIT * sp;
+  int pos[PG_MAXP], base = 0;
+#pragma unroll
+  for (int p = 0; p < PG_MAXP; p++) {
+    pos[p] = base + scan[p][t] - run[p];
+    if (t == 0 && p < P) poff[u * (PG_MAXP + 1) + p] = base;
+    base += p < P ? tot[p] : 0;
+  }
+  if (t == 0) poff[u * (PG_MAXP + 1) + P] = base;
+#pragma unroll
+  for (int j = 0; j < DMAX; j++)
+    if (id[j] >= 0) {
+      const int q = (int)(((int64_t)id[j] * P) / n);
+      int at = 0;
+#pragma unroll
+      for (int p = 0; p < PG_MAXP; p++)
+        if (p == q) at = pos[p]++;
+      ul[at] = ((u32)t << 22) | (u32)id[j];
+    }
+}
+
+typedef u32 u32x4v __attribute__((ext_vector_type(4)));
+
+// Persistent: workgroup b runs on XCD b % 8 and takes every (gridDim.x / 8)-th
+// unit of its XCD's contiguous range of units (a unit: PG_UNIT rows of one
+// task), so the XCD's workgroups sweep that range together; any grid that is a
+// multiple of 8 works.  The gathers are buffer loads with cache policy AUX
+// (0: default; 1: sc0, 16: sc1, 2: nt), bounded by the slab (out-of-range
+// offsets read zero).
+template <bool LS, int AUX>
+__global__ __launch_bounds__(PG_WG) void k_pgather(const LevelTask* __restrict__ tasks, int level, int ring,
+                                                   int mode, LvList lst, PgArgs a)
+{
+  __shared__ u64x2 acc[PG_UNIT * 8];                  // row r of the unit at [8 r, 8 r + 8)
+  __shared__ u64 s_pop[PG_WG / 64];
+  const int x = blockIdx.x % ODP_NXCD, wi = blockIdx.x / ODP_NXCD, nwx = gridDim.x / ODP_NXCD;
+  const long long u_lo = a.units * x / ODP_NXCD, u_hi = a.units * (x + 1) / ODP_NXCD;
+  const int tid = threadIdx.x, wave = __builtin_amdgcn_readfirstlane(tid >> 6), lane = tid & 63;
+  const int nw = PG_WG / 64;
+  const bool count = (mode & LV_COUNT) != 0, store = (mode & LV_STORE) != 0;
+  const int w = 2 * (lane & 7);                       // the lane's two words of a row
+  for (long long u = u_lo + wi; u < u_hi; u += nwx) { // uniform
+    const int t = (int)(u / a.upt), ur = (int)(u - (long long)t * a.upt);
+    const LevelTask& T = tasks[t];
+    if (*T.done) continue;
+    const u64 c1 = T.cnt[(int64_t)((level - 1) % ring) * T.cstride];
+    if (c1 == (u64)T.n * (u64)T.l || (level >= 2 && c1 == T.cnt[(int64_t)((level - 2) % ring) * T.cstride])) {
+      if (tid == 0) *T.done = 1;
+      continue;
+    }
+    if (count && ur == 0 && tid == 0) T.cnt[(int64_t)((level + 1) % ring) * T.cstride] = 0;
+    const u64* __restrict__ cur = (level & 1) ? T.buf1 : T.buf0;
+    u64* __restrict__ nxt = (level & 1) ? T.buf0 : T.buf1;
+    const __amdgpu_buffer_rsrc_t src = __builtin_amdgcn_make_buffer_rsrc((void*)cur, 0, a.n * 128, 0x00020000);
+    const u64 f0 = full_word(T.l, w), f1 = full_word(T.l, w + 1);
+#pragma unroll
+    for (int i = tid; i < PG_UNIT * 8; i += PG_WG) { acc[i].x = 0; acc[i].y = 0; }
+    __syncthreads();
+    const u32* __restrict__ ul = a.list + (int64_t)ur * PG_UNIT * a.sp;
+    const int* po = a.poff + ur * (PG_MAXP + 1);
+    // the parts in order; within a part the waves take 64-pair chunks in turn
+    for (int p = 0; p < a.P; p++) {
+      const int e0 = po[p], e1 = po[p + 1];
+      for (int c0 = e0 + wave * 64; c0 < e1; c0 += nw * 64) {          // wave-uniform
+        const u32 mine = c0 + lane < e1 ? ul[c0 + lane] : 0xffffffffu;
+        u64x2 gv[8];
+        int lr[8];
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+          const u32 e = (u32)__shfl((int)mine, q * 8 + (lane >> 3), 64);
+          lr[q] = e == 0xffffffffu ? -1 : (int)(e >> 22);
+          gv[q].x = 0; gv[q].y = 0;
+          if (lr[q] >= 0) {
+            const u32x4v v = __builtin_amdgcn_raw_buffer_load_b128(src, (int)((e & 0x3fffff) * 128u + w * 8u), 0, AUX);
+            gv[q].x = ((u64)v.y << 32) | v.x; gv[q].y = ((u64)v.w << 32) | v.z;
+          }
+        }
+        // Rows are 128 bytes apart, so the eight rows of one instruction meet the
+        // same banks; an odd row ORs its second word first, so each ds_or_b64
+        // spreads over both halves of the banks.
+#pragma unroll
+        for (int q = 0; q < 8; q++)
+          if (lr[q] >= 0) {
+            u64* d = (u64*)&acc[lr[q] * 8 + (w >> 1)];
+            const int odd = lr[q] & 1;
+            const u64 first = odd ? gv[q].y : gv[q].x, second = odd ? gv[q].x : gv[q].y;
+            if (first) atomicOr(d + odd, first);
+            if (second) atomicOr(d + (odd ^ 1), second);
+          }
+      }
+    }
+    __syncthreads();
+    // the unit's rows: written, counted, listed; wave w takes rows [64 w, 64 w + 64)
+    u64 pop = 0;
+    for (int g = 0; g < 8; g++) {
+      const int lr = wave * 64 + g * 8 + (lane >> 3), r = ur * PG_UNIT + lr;
+      const bool valid = r < a.n;
+      const u64x2 v = acc[lr * 8 + (w >> 1)];
+      if (valid) {
+        if (store) {
+          u64v2 o; o.x = v.x; o.y = v.y;
+          __builtin_nontemporal_store(o, (u64v2*)(nxt + (int64_t)r * 16 + w));   // streamed past L2
+        }
+        pop += (u64)(__popcll(v.x) + __popcll(v.y));
+      }
+      if (LS) {
+        const bool unsat = valid && !(v.x == f0 && v.y == f1);
+        const u64 m = __ballot(unsat);
+        if (m) {
+          u32 base = 0;
+          if (lane == 0) base = atomicAdd(lst.n, (u32)__popcll(m));
+          base = __builtin_amdgcn_readfirstlane(base);
+          if (unsat) {
+            const u32 e = base + (u32)__popcll(m & ((1ULL << lane) - 1));
+            if (e < lst.cap) lst.e[e] = ((u64)t << 48) | ((u64)r << 24) | (u64)w;
+          }
+        }
+      }
+    }
+    if (count) {
+      pop = wave_sum(pop);
+      if (lane == 0) s_pop[wave] = pop;
+    }
+    __syncthreads();                                  // acc and s_pop are free for the next unit
+    if (count && tid == 0) {
+      u64 sum = 0;
+      for (int i = 0; i < nw; i++) sum += s_pop[i];
+      if (sum) atomicAdd(&T.cnt[(int64_t)(level % ring) * T.cstride], sum);
+    }
   }
 }
 

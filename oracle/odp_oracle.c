@@ -16,6 +16,7 @@
  *                      237-274 (GLOBAL_ADJ_GRID), 2103-2124 (ODP_Create_itable)
  *   - Init variants  : aspl.c:223-288 (num_degrees expansion per variant)
  *   - sentinels      : aspl.c:332-336
+ *   - one rank's share of the MPI+CUDA flavour: aspl_mpi_cuda.cu:44-123
  *
  * Pinning (what checks each function; see tests/test_oracle.py):
  *   - all three kinds on s=1 graphs: the reference's golden outputs
@@ -27,7 +28,10 @@
  *     reference produced for the symmetric fixtures n24d4g4, w6h6d3r2g2,
  *     w6h6d3r2g4, w10h10d4r4g4 (tests/golden/symmetric_fixtures.json), plus
  *     oracle/_ref on generated symmetric graphs;
- *   - MATRIX vs MATRIX_SAVING vs BFS: bit-identical for every chunk width.
+ *   - MATRIX vs MATRIX_SAVING vs BFS: bit-identical for every chunk width;
+ *   - rank shares: the ranks' partials combine to the whole evaluation, and
+ *     each rank's partial equals breadth-first searches from its own sources
+ *     (sum of distances minus n-1 per source, times s; the largest eccentricity).
  * oracle_set_fault() plants known mistakes so the tests can show each check
  * above rejects them.
  */
@@ -49,7 +53,8 @@ enum {
   FAULT_ROTATE_SWAP = 3,      /* swap the 90 and 270 degree rotations             */
   FAULT_CIRCULANT_SHIFT = 4,  /* wrong orbit shift in the symmetric general case  */
   FAULT_SKIP_LAST_NBR = 5,    /* drop the last neighbour of every row             */
-  FAULT_ASPL_DENOM = 6        /* divide by n*n instead of n*(n-1)                 */
+  FAULT_ASPL_DENOM = 6,       /* divide by n*n instead of n*(n-1)                 */
+  FAULT_SHARD_CHUNK = 7       /* MATRIX shards: floor(W / procs) words, not ceil   */
 };
 static int g_fault = FAULT_NONE;
 void oracle_set_fault(int code) { g_fault = code; }
@@ -356,6 +361,102 @@ int oracle_eval(int family, int nodes_or_width, int height, int degree, const in
     *sum = LONG_MAX;
     *ASPL = DBL_MAX;
   }
+  free(c.num_degrees);
+  free(c.itable);
+  return 0;
+}
+
+/* The per-rank loop of the MPI+CUDA flavour (aspl_mpi_cuda.cu:44-80 and
+ * 90-123): column blocks t = rank, rank + procs, ... of `chunk` words each,
+ * every block's levels until it reaches all n x l pairs (or n levels), the
+ * rank-local sum += (n l - num) s starting at 0 and diameter = max(kk + 1)
+ * starting at 1 -- what each rank holds before the two Allreduces at :82-83. */
+static void o_rank_blocks(const octx *c, const int *adj, int64_t chunk, int rank, int procs, long *sum, int *diameter)
+{
+  int n = c->nodes, s = c->symmetries, based = n / s;
+  int64_t W = ((int64_t)based + 63) / 64;
+  int64_t parsize = (W + chunk - 1) / chunk;
+  uint64_t *A = malloc((size_t)n * chunk * 8), *B = malloc((size_t)n * chunk * 8);
+  *sum = 0;
+  *diameter = 1;
+  for (int64_t t = rank; t < parsize; t += procs) {
+    memset(A, 0, (size_t)n * chunk * 8);
+    memset(B, 0, (size_t)n * chunk * 8);
+    int64_t l;
+    for (l = 0; l < 64LL * chunk && 64LL * t * chunk + l < based; l++) {
+      int64_t off = (64LL * t * chunk + l) * chunk + l / 64;
+      A[off] = B[off] = 1ULL << (l % 64);
+    }
+    int kk;
+    for (kk = 0; kk < n; kk++) {
+      o_level(c, adj, A, B, chunk);
+      uint64_t num = o_popcount(B, (int64_t)n * chunk);
+      if (num == (uint64_t)n * (uint64_t)l) break;
+      uint64_t *tmp = A; A = B; B = tmp;
+      *sum += ((long)n * l - (long)num) * s;
+    }
+    if (kk + 1 > *diameter) *diameter = kk + 1;
+  }
+  free(A); free(B);
+}
+
+/*
+ * One rank's share of ODP_Set_aspl in the MPI+CUDA flavour, before the
+ * combine: MATRIX takes ceil(W / procs)-word blocks, rank r the block t = r
+ * (aspl_mpi_cuda.cu:47-54, W = ceil((n/s)/64)); MATRIX_SAVING takes the
+ * 64-word blocks t = r, r + procs, ... (GPU_CHUNK, :92-94).  sum is the
+ * rank-local excess over the n(n-1) base (the reference's *sum before the
+ * Allreduce, :83), diameter the rank-local maximum of kk + 1 (n + 1 when a
+ * block never completes).  The combine is :82-87: diameter = max over ranks,
+ * sum = sum over ranks + n(n-1), ASPL = sum / ((n-1) n), sum /= 2.
+ * Symmetric grids use the reference's row order (utils.c:2103-2124).
+ * Returns 0, or -1 on invalid parameters.
+ */
+int oracle_eval_shard(int family, int nodes_or_width, int height, int degree, const int *num_degrees,
+                      int symmetries, const int *adjacency, int kind, int rank, int procs,
+                      long *sum, int *diameter)
+{
+  octx c;
+  memset(&c, 0, sizeof c);
+  int width = nodes_or_width;
+  int nodes = family ? width * height : nodes_or_width;
+  if (symmetries < 1 || nodes % symmetries != 0 || procs < 1 || rank < 0 || rank >= procs) return -1;
+  if (kind != ORACLE_MATRIX && kind != ORACLE_MATRIX_SAVING) return -1;
+  c.nodes = nodes; c.degree = degree; c.symmetries = symmetries;
+  c.height = family ? height : -1;
+  int based = nodes / symmetries;
+  if (family && symmetries > 1) {
+    if (symmetries != 2 && symmetries != 4) return -1;
+    c.grid_s = 1;
+    c.itable = calloc((size_t)nodes, sizeof(int));
+    o_create_itable(width, height, symmetries, c.itable);
+  }
+  if (num_degrees) {
+    c.num_degrees = malloc(sizeof(int) * nodes);
+    if (symmetries == 1) {
+      memcpy(c.num_degrees, num_degrees, sizeof(int) * nodes);
+    } else if (!family) {
+      for (int i = 0; i < symmetries; i++)
+        for (int j = 0; j < based; j++) c.num_degrees[i * based + j] = num_degrees[j];
+    } else if (symmetries == 2) {
+      for (int i = 0; i < based; i++) {
+        c.num_degrees[i] = num_degrees[i];
+        c.num_degrees[o_rotate(i, width, height, 2, 180)] = num_degrees[i];
+      }
+    } else {
+      for (int i = 0; i < based; i++) {
+        int v = (i / (height / 2)) * height + (i % (height / 2));
+        c.num_degrees[v] = num_degrees[i];
+        c.num_degrees[o_rotate(v, width, height, 4, 90)]  = num_degrees[i];
+        c.num_degrees[o_rotate(v, width, height, 4, 180)] = num_degrees[i];
+        c.num_degrees[o_rotate(v, width, height, 4, 270)] = num_degrees[i];
+      }
+    }
+  }
+  int64_t W = ((int64_t)based + 63) / 64;
+  int64_t chunk = kind == ORACLE_MATRIX ? (W + procs - 1) / procs : 64;
+  if (g_fault == FAULT_SHARD_CHUNK && kind == ORACLE_MATRIX) chunk = W / procs > 0 ? W / procs : 1;
+  o_rank_blocks(&c, adjacency, chunk, rank, procs, sum, diameter);
   free(c.num_degrees);
   free(c.itable);
   return 0;

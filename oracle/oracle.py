@@ -39,6 +39,10 @@ def lib():
                                                            ctypes.POINTER(ctypes.c_double)]
         _lib.oracle_expand.restype = None
         _lib.oracle_expand.argtypes = [ctypes.c_int] * 4 + [p, ctypes.c_int, p, p]
+        _lib.oracle_eval_shard.restype = ctypes.c_int
+        _lib.oracle_eval_shard.argtypes = [ctypes.c_int] * 4 + [p, ctypes.c_int, p, ctypes.c_int, ctypes.c_int,
+                                                                 ctypes.c_int, ctypes.POINTER(ctypes.c_long),
+                                                                 ctypes.POINTER(ctypes.c_int)]
         _lib.oracle_set_fault.restype = None
         _lib.oracle_set_fault.argtypes = [ctypes.c_int]
     return _lib
@@ -67,6 +71,23 @@ def evaluate(adjacency, nodes=None, width=None, height=None, symmetries=1, num_d
     if rc != 0:
         raise ValueError("invalid parameters")
     return d.value, s.value, x.value
+
+
+def evaluate_shard(adjacency, rank, procs, nodes=None, width=None, height=None, symmetries=1, num_degrees=None,
+                   kind=MATRIX):
+    """One rank's share of the MPI+CUDA flavour (aspl_mpi_cuda.cu:44-123) -> (excess, diameter):
+    the rank-local sum over the n(n-1) base and the rank-local diameter, before the combine."""
+    adj = np.ascontiguousarray(adjacency, dtype=np.int32)
+    nd = None if num_degrees is None else np.ascontiguousarray(num_degrees, dtype=np.int32)
+    fam = GRID if width is not None else GENERAL
+    nw = width if fam == GRID else nodes
+    h = height if fam == GRID else -1
+    s, d = ctypes.c_long(), ctypes.c_int()
+    rc = lib().oracle_eval_shard(fam, nw, h, adj.shape[1], _p(nd), symmetries, _p(adj), kind, rank, procs,
+                                 ctypes.byref(s), ctypes.byref(d))
+    if rc != 0:
+        raise ValueError("invalid parameters")
+    return s.value, d.value
 
 
 def expand(adjacency, nodes=None, width=None, height=None, symmetries=1, num_degrees=None):

@@ -49,6 +49,8 @@ def case_graph(c):
 C2_PLANS = {
     "default": (None, None),
     "saving": ("MATRIX_SAVING", None),
+    "blocks16-k_level": (None, dict(plan=LEV, kw=16, upw=8, seg_major=0, xcd=1, direct=-1, parts=0)),
+    "parts2-no-direct": (None, dict(plan=LEV, kw=16, upw=8, seg_major=0, xcd=1, direct=0, parts=2)),
     "blocks64-gather": (None, dict(plan=LEV, kw=64, upw=8, seg_major=0, xcd=1, direct=0)),
     "whole-rows": (None, dict(plan=LEV, kw=0, upw=4, seg_major=1, xcd=0, direct=0)),
 }
@@ -68,6 +70,8 @@ def test_random65536(name, monkeypatch):
     try:
         if plan:
             odp.set_plan(**plan)
+        if name == "default":
+            assert odp.get_plan()["parts"] == 4
         assert same(odp.set_aspl(adj), want)
         for _ in range(3):                      # eager, captured, replayed
             assert same(odp.set_aspl_resident(), want)

@@ -38,6 +38,8 @@ PLANS = [
     dict(plan=LEV, kw=2, upw=2, seg_major=1, xcd=1, direct=2),
     dict(plan=LEV, kw=4, upw=4, seg_major=0, xcd=1, group=2, direct=3),
     dict(plan=LEV, kw=128, upw=8, seg_major=0, xcd=1, direct=6),
+    dict(plan=LEV, kw=16, upw=8, seg_major=0, xcd=1, direct=-1, parts=4),
+    dict(plan=LEV, kw=16, upw=8, seg_major=0, xcd=1, direct=0, parts=2, group=3),
 ]
 
 
@@ -504,6 +506,8 @@ SPARSE_PLANS = [
     dict(plan=LEV, kw=4, upw=4, seg_major=1, xcd=0, direct=0, batch=8),
     dict(plan=LEV, kw=64, upw=2, seg_major=0, xcd=1, direct=0, batch=16),
     dict(plan=LEV, kw=2, upw=2, seg_major=0, xcd=1, group=2, direct=3),
+    dict(plan=LEV, kw=16, upw=8, seg_major=0, xcd=1, direct=-1, parts=4),
+    dict(plan=LEV, kw=16, upw=8, seg_major=0, xcd=1, direct=0, parts=2),
 ]
 
 
@@ -701,5 +705,43 @@ def test_past_16bit_ids(n, d):
         assert odp.get_plan()["plan"] == LEV
         for _ in range(2):
             assert same(odp.set_aspl(adj), want)
+    finally:
+        odp.finalize_aspl()
+
+
+@pytest.mark.parametrize("parts", [2, 4])
+def test_part_phased_gather(parts):
+    """k_pgather (the XCD-phased gather over row parts) against the oracle:
+    graphs whose rows hold several 16-word blocks and a partial last block,
+    regular and non-regular tables, a disconnected graph, MATRIX and groups,
+    repeated evaluations (frontier first, then plain levels with the
+    count-only, listing and sparse last levels); the plan reports the parts
+    that run."""
+    cases = []
+    g = odp.generate_regular(3000, 12, 71)                       # 47 words: 2 full blocks + 15 words
+    cases.append((g, None))
+    holes, nd = _trimmed(odp.generate_regular(2200, 9, 72), 5, 200)
+    cases.append((holes, nd))
+    half = odp.generate_regular(1100, 6, 73)
+    cases.append((np.concatenate([half, half + 1100]).astype(np.int32), None))
+    for adj, deg in cases:
+        n = adj.shape[0]
+        want = oracle.evaluate(adj, nodes=n, num_degrees=deg)
+        for group in (0, 2):
+            odp.init_aspl_general(n, adj.shape[1], deg)
+            try:
+                odp.set_plan(plan=LEV, kw=16, upw=8, seg_major=0, xcd=1, direct=-1, group=group, parts=parts)
+                assert odp.get_plan()["parts"] == parts
+                for _ in range(4):
+                    assert same(odp.set_aspl(adj), want), (n, group)
+            finally:
+                odp.finalize_aspl()
+    # rows narrower than 16 words: the plan falls back to k_level and says so
+    small = odp.generate_regular(500, 4, 74)
+    odp.init_aspl_general(500, 4)
+    try:
+        odp.set_plan(plan=LEV, kw=16, upw=8, seg_major=0, xcd=1, parts=parts)
+        assert odp.get_plan()["parts"] == 0
+        assert same(odp.set_aspl(small), oracle.evaluate(small, nodes=500))
     finally:
         odp.finalize_aspl()

@@ -205,3 +205,92 @@ def test_planted_faults_are_caught(code):
     finally:
         oracle.set_fault(0)
     assert caught, FAULTS[code]
+
+
+# ---- one rank's share of the MPI+CUDA flavour --------------------------------
+
+def _bfs_dists(full, sources):
+    """Hop distances from each source (rows) to every vertex: breadth-first
+    search, scipy's compiled unweighted shortest paths."""
+    from scipy.sparse import csr_matrix
+    from scipy.sparse.csgraph import shortest_path
+    n = full.shape[0]
+    rows = np.repeat(np.arange(n), full.shape[1])
+    cols = full.ravel()
+    keep = cols >= 0
+    g = csr_matrix((np.ones(keep.sum()), (rows[keep], cols[keep])), shape=(n, n))
+    return shortest_path(g, method="D", unweighted=True, indices=sources).astype(np.int64)
+
+
+def _ref_source_vertices(nodes=None, width=None, height=None, symmetries=1):
+    """Vertex of each source column (row of the bit matrix below n/s): the row
+    itself, or for symmetric grids the base vertex the reference's itable puts
+    there (utils.c:2103-2124)."""
+    if width is None or symmetries == 1:
+        return list(range((nodes or width * height) // symmetries))
+    based = width * height // symmetries
+    if symmetries == 2:
+        return list(range(based))
+    bh = height // 2
+    return [(i // bh) * height + (i % bh) for i in range(based)]
+
+
+def _rank_sources(based, kind, rank, procs):
+    W = (based + 63) // 64
+    if kind == oracle.MATRIX:
+        chunk = (W + procs - 1) // procs
+        blocks = [rank] if rank < (W + chunk - 1) // chunk else []
+    else:
+        chunk = 64
+        blocks = list(range(rank, (W + chunk - 1) // chunk, procs))
+    return [c for t in blocks for c in range(64 * chunk * t, min(based, 64 * chunk * (t + 1)))]
+
+
+def shard_brute_ok(adj, procs_list=(1, 2, 3, 5), **kw):
+    """Each rank's partial equals breadth-first searches from its own sources,
+    and the partials combine to the whole evaluation."""
+    s = kw.get("symmetries", 1)
+    n = kw.get("nodes") or kw["width"] * kw["height"]
+    full = oracle.expand(adj, **kw) if s > 1 else adj
+    verts = _ref_source_vertices(**kw)
+    dist = _bfs_dists(full, verts)
+    whole = oracle.evaluate(adj, **kw)
+    for kind in (oracle.MATRIX, oracle.MATRIX_SAVING):
+        for procs in procs_list:
+            tot, dmax = 0, 1
+            for rank in range(procs):
+                ex, dia = oracle.evaluate_shard(adj, rank, procs, kind=kind, **kw)
+                src = _rank_sources(len(verts), kind, rank, procs)
+                want_ex = s * sum(int(dist[c].sum()) - (n - 1) for c in src)
+                want_d = max([1] + [int(dist[c].max()) for c in src])
+                if (ex, dia) != (want_ex, want_d):
+                    return False
+                tot += ex
+                dmax = max(dmax, dia)
+            ordered = tot + n * (n - 1)
+            if (dmax, ordered // 2, ordered / ((n - 1.0) * n)) != (whole[0], whole[1], whole[2]):
+                return False
+    return True
+
+
+def test_shard_partials_bfs_and_combine():
+    """oracle.evaluate_shard against breadth-first searches from each rank's
+    sources, for both kinds and 1, 2, 3, 5 ranks, on a general graph of two
+    4096-source blocks, a circulant-symmetric graph and 2- and 4-fold
+    symmetric grids; the partials also combine to the whole evaluation."""
+    adj = odp.generate_regular(4200, 3, 4)
+    assert shard_brute_ok(adj, nodes=4200)
+    base = odp.generate_general_s(1200, 4, 4, 6)
+    assert shard_brute_ok(base, nodes=1200, symmetries=4)
+    for s in (2, 4):
+        full = odp.generate_grid(24, 24, 6, 3, s, 7)
+        gb = odp.grid_base_table(24, 24, 6, s, full)
+        assert shard_brute_ok(gb, width=24, height=24, symmetries=s)
+
+
+def test_shard_planted_fault_is_caught():
+    oracle.set_fault(7)           # MATRIX shards floor(W / procs) words wide
+    try:
+        assert not shard_brute_ok(odp.generate_regular(4200, 3, 4), procs_list=(3, 5), nodes=4200)
+    finally:
+        oracle.set_fault(0)
