@@ -80,6 +80,23 @@ void ODPX_Set_aspl_resident(int *diameter, long *sum, double *ASPL);
  * range, the four ids are not distinct, or edge a-b or c-d is missing. */
 int  ODPX_Swap_device(int a, int b, int c, int d);
 
+/* ---- one process's share (the reference's MPI+CUDA division) ------------- */
+/* Evaluates, on the active Init, only the source columns that rank `rank` of
+ * `nranks` processes owns in the reference's MPI+CUDA flavour, on the adjacency
+ * already on the device (after ODP_Set_aspl or ODPX_Upload_adjacency): MATRIX,
+ * one block of ceil(W / nranks) 64-bit words per rank (aspl_mpi_cuda.cu:47-54,
+ * W = ceil((nodes/symmetries)/64)); MATRIX_SAVING, the 64-word blocks rank,
+ * rank + nranks, ... (:92-94).  Symmetric grids number the source columns as
+ * the reference does (utils.c:2103-2124).  Out: excess, the rank's share of
+ * the ordered-pair distance sum beyond the n(n-1) base (the reference's *sum
+ * before its Allreduce, :83); diameter, the rank's largest terminal level + 1
+ * (1 with no column, nodes + 1 with a block that never completes);
+ * disconnected, 1 in that case.  The combine of :82-87 gives ODP_Set_aspl's
+ * result: diameter = max over ranks, sum = sum over ranks + n(n-1), ASPL =
+ * sum / ((n-1) n), sum /= 2, and diameter > nodes gives the sentinels.  One
+ * GPU runs any rank.  Returns 0, or -1 for an invalid rank or null output. */
+int  ODPX_Set_aspl_shard(int rank, int nranks, long long *excess, int *diameter, int *disconnected);
+
 /* ---- algorithm kind (host only, no GPU) ---------------------------------- */
 /* The kind ODP_Init_aspl_* picks: ODP_ASPL if set (MATRIX, MATRIX_SAVING; BFS
  * and unknown values exit as at Init), else MATRIX when the GPU memory estimate

@@ -126,6 +126,7 @@ def load():
     _sig(lib, "ODPX_Upload_adjacency", None, [_P])
     _sig(lib, "ODPX_Set_aspl_resident", None, [ip, ctypes.POINTER(_L), ctypes.POINTER(_D)])
     _sig(lib, "ODPX_Swap_device", _I, [_I, _I, _I, _I])
+    _sig(lib, "ODPX_Set_aspl_shard", _I, [_I, _I, ctypes.POINTER(ctypes.c_longlong), ip, ip])
     _sig(lib, "ODPX_Get_kind", _I, [_I, _I, _I, _I, _I])
     _sig(lib, "ODPX_Get_mem_usage", _D, [_I, _I, _I, _I, _I, _I])
     _sig(lib, "ODPX_Anneal", _I, [_P, ctypes.c_longlong, _D, _D, _I, _I, _U64, ctypes.POINTER(AnnealResult)])
@@ -218,6 +219,25 @@ def set_aspl_resident():
 
 def swap_device(a, b, c, d):
     return load().ODPX_Swap_device(a, b, c, d)
+
+
+def set_aspl_shard(rank, nranks):
+    """ODPX_Set_aspl_shard: one rank's share of the resident adjacency's evaluation
+    -> (excess, diameter, disconnected), as the reference's MPI+CUDA rank holds it."""
+    ex, d, dc = ctypes.c_longlong(), ctypes.c_int(), ctypes.c_int()
+    if load().ODPX_Set_aspl_shard(rank, nranks, ctypes.byref(ex), ctypes.byref(d), ctypes.byref(dc)) != 0:
+        raise ValueError(f"invalid shard {rank} of {nranks}")
+    return ex.value, d.value, dc.value
+
+
+def combine_shards(nodes, partials):
+    """The reference's combine (aspl_mpi_cuda.cu:82-87) of (excess, diameter, _)
+    partials -> (diameter, sum, ASPL) with the sentinels of aspl_cuda.cu:256-260."""
+    d = max(p[1] for p in partials)
+    total = sum(p[0] for p in partials) + nodes * (nodes - 1)
+    if d > nodes:
+        return 2**31 - 1, 2**63 - 1, float.fromhex("0x1.fffffffffffffp+1023")
+    return d, int(total / 2.0), total / ((nodes - 1.0) * nodes)
 
 
 def get_kind(nodes, degree, has_num_degrees=False, symmetries=1, grid_s=False):

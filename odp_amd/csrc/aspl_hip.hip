@@ -272,6 +272,8 @@ struct ResRun {
 struct Evaluator {
   // graph
   int family = 0, n = 0, s = 1, degree = 0, dpad = 0, width = 0, height = 0, grid_s = 0, nsrc = 0;
+  // source columns evaluated: [src_lo, src_hi) of [0, nsrc) (all but for one shard's share)
+  int src_lo = 0, src_hi = 0;
   int has_numdeg = 0;
   double avg_deg = 0;
   size_t adj_cap = 0, nbr_cap = 0;
@@ -413,6 +415,7 @@ struct Evaluator {
     pend_patch.clear();
     family = fam; n = nodes; degree = deg; s = sym; width = w; height = h; grid_s = gs;
     nsrc = n / s;
+    src_lo = 0; src_hi = nsrc;
     dpad = round_up(deg > 0 ? deg : 1, 4);
     has_numdeg = numdeg_full != nullptr;
     if (adj_elems() > adj_cap) { dfree(&d_adj); dalloc(&d_adj, adj_elems()); adj_cap = adj_elems(); }
@@ -447,6 +450,7 @@ struct Evaluator {
 
   void set_itable(const int* itable)
   {
+    dfree(&d_itable);
     dalloc(&d_itable, (size_t)n);
     HIPC(hipMemcpy(d_itable, itable, sizeof(int) * n, hipMemcpyHostToDevice));
     set_geometry(itable);
@@ -696,7 +700,7 @@ struct Evaluator {
   void setup_level()
   {
     LevelRun& r = lr;
-    int W = (nsrc + 63) / 64;
+    int W = (src_hi - src_lo + 63) / 64;
     int kw = plan.kw > 0 ? std::min(plan.kw, W) : W;
     int kwp = kw <= 128 ? (1 << (ilog2(std::max(kw, 2) - 1) + 1)) : round_up(kw, 128);
     if (kwp < kw) kwp = kw;
@@ -783,8 +787,8 @@ struct Evaluator {
       T.cnt = r.cnt + t;
       T.done = r.done + t;
       T.n = n; T.dpad = dpad; T.kwp = kwp;
-      T.src0 = t * kw * 64;
-      T.l = std::min(kw * 64, nsrc - T.src0);
+      T.src0 = src_lo + t * kw * 64;
+      T.l = std::min(kw * 64, src_hi - T.src0);
       T.lpr_log2 = lpr_log2; T.segs = segs; T.rgroups = rgroups; T.units = units;
       T.seg_major = r.seg_major; T.cstride = ntasks; T.pad = 0;
       T.ccnt = r.cnt + (size_t)RING * ntasks + t;
@@ -810,7 +814,7 @@ struct Evaluator {
     ResRun& r = rr;
     r.wb = plan.wb; r.threads = plan.threads;
     int b = 32 * r.wb;
-    int ntasks = (nsrc + b - 1) / b;
+    int ntasks = (src_hi - src_lo + b - 1) / b;
     if ((size_t)ntasks > r.task_cap) {
       dfree(&r.tasks);
       dalloc(&r.tasks, (size_t)ntasks);
@@ -831,7 +835,7 @@ struct Evaluator {
     std::vector<ResTask> ht(ntasks);
     for (int t = 0; t < ntasks; t++) {
       ResTask& T = ht[t];
-      T.nbr = d_nbr16; T.n = n; T.dp = dp16; T.src0 = t * b; T.l = std::min(b, nsrc - t * b);
+      T.nbr = d_nbr16; T.n = n; T.dp = dp16; T.src0 = src_lo + t * b; T.l = std::min(b, src_hi - T.src0);
       T.s = s; T.graph = 0; T.wb = r.wb; T.dq = (std::max(degree, 1) + 3) / 4;
     }
     HIPC(hipMemcpy(r.tasks, ht.data(), sizeof(ResTask) * ntasks, hipMemcpyHostToDevice));
@@ -929,8 +933,8 @@ struct Evaluator {
       a.tstride = (int64_t)2 * n * r.kwp;
       a.cnt = r.cnt + first; a.cstride = r.ntasks;
       a.n = n; a.dpad = dpad; a.kw = r.kw; a.kwl = ilog2(r.kw);
-      a.src_lo = first * r.kw * 64;
-      a.nsrc = std::min(cnt_t * r.kw * 64, nsrc - a.src_lo);
+      a.src_lo = src_lo + first * r.kw * 64;
+      a.nsrc = std::min(cnt_t * r.kw * 64, src_hi - a.src_lo);
       a.ntask = cnt_t; a.words = cnt_t * r.kw; a.chunks = (a.words + 127) / 128;
       a.K = K; a.list = (int)direct_list(K);
       const bool id16 = direct16();
@@ -1324,6 +1328,15 @@ struct Evaluator {
     *sum = lsum;
   }
 
+  // the evaluated sources' excess over their n-1 base distances (times s),
+  // the largest terminal level + 1, and whether some block stalled
+  void run_raw(u64* excess, int* d, bool* disc)
+  {
+    *excess = 0; *d = 1; *disc = false;
+    if (plan.plan == ODPX_PLAN_LEVEL) run_level(excess, d, disc);
+    else run_resident(excess, d, disc);
+  }
+
   // full evaluation of the adjacency on the device, reference arithmetic
   void evaluate(int* diameter, long* sum, double* ASPL)
   {
@@ -1331,8 +1344,7 @@ struct Evaluator {
     u64 excess = 0;
     int d = 1;
     bool disc = false;
-    if (plan.plan == ODPX_PLAN_LEVEL) run_level(&excess, &d, &disc);
-    else run_resident(&excess, &d, &disc);
+    run_raw(&excess, &d, &disc);
     result_of(excess, d, disc, diameter, sum, ASPL);
     if (timed()) { g_stats.evals++; g_stats.eval_ms += 1e3 * (wall_time() - t0); }
   }
@@ -1415,19 +1427,23 @@ static void tile_rank(int bw, int bh, std::vector<int>& rank)
 // k * n/s.  Within the base rows the order is tiled (tile_rank) instead of the
 // reference's x-major order; results do not depend on it (rows and sources are
 // relabelled together).
-static void make_itable(int width, int height, int s, int* itable)
+static void make_itable(int width, int height, int s, int* itable, bool tiled = true)
 {
   int based = width * height / s;
   std::vector<int> rank;
+  if (!tiled) {                // the reference's order: base vertex i takes row i
+    rank.resize(based);
+    for (int i = 0; i < based; i++) rank[i] = i;
+  }
   if (s == 2) {
-    tile_rank(width / 2, height, rank);
+    if (tiled) tile_rank(width / 2, height, rank);
     for (int i = 0; i < based; i++) {
       itable[i] = rank[i];
       itable[rotate_host(i, width, height, 2, 180)] = rank[i] + based;
     }
   } else {
     int bh = height / 2;
-    tile_rank(width / 2, bh, rank);
+    if (tiled) tile_rank(width / 2, bh, rank);
     for (int i = 0; i < based; i++) {
       int v = (i / bh) * height + (i % bh);
       itable[v] = rank[i];
@@ -1673,6 +1689,58 @@ extern "C" int ODPX_Swap_device(int a, int b, int c, int d)
     auto cheb = [&](int p, int q) { return std::max(abs(p / v.height - q / v.height), abs(p % v.height - q % v.height)); };
     v.geo_lmax = std::max(v.geo_lmax, std::max(cheb(a, c), cheb(b, d)));
   }
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// one process's share of an evaluation, as the reference's MPI+CUDA flavour
+// divides it (aspl_mpi_cuda.cu:44-123): the source columns of one rank
+
+extern "C" int ODPX_Set_aspl_shard(int rank, int nranks, long long* excess, int* diameter, int* disconnected)
+{
+  RandFence rand_fence;
+  need_active("ODPX_Set_aspl_shard");
+  if (nranks < 1 || rank < 0 || rank >= nranks || !excess || !diameter || !disconnected) return -1;
+  Evaluator& v = E.ev;
+  const int nsrc = v.nsrc, W = (nsrc + 63) / 64;
+  // MATRIX: ceil(W / P)-word blocks, rank r the block r (aspl_mpi_cuda.cu:47-54);
+  // MATRIX_SAVING: 64-word blocks r, r + P, ... (aspl_mpi_cuda.cu:92-94)
+  std::vector<std::pair<int, int>> ranges;
+  if (E.kind == KIND_MATRIX) {
+    const int chunk = (W + nranks - 1) / nranks, parsize = (W + chunk - 1) / chunk;
+    if (rank < parsize) ranges.push_back({64 * rank * chunk, std::min(nsrc, 64 * (rank + 1) * chunk)});
+  } else {
+    const int b = 64 * GPU_CHUNK_WORDS, parsize = (W + GPU_CHUNK_WORDS - 1) / GPU_CHUNK_WORDS;
+    for (int t = rank; t < parsize; t += nranks) ranges.push_back({b * t, std::min(nsrc, b * (t + 1))});
+  }
+  // symmetric grids: the reference's row order, so that a source column is the
+  // same base vertex as in the reference (utils.c:2103-2124)
+  std::vector<int> itable;
+  if (v.grid_s) {
+    itable.resize(v.n);
+    make_itable(v.width, v.height, v.s, itable.data(), false);
+    v.set_itable(itable.data());
+  }
+  const ODPX_Plan saved = v.plan;
+  u64 tot = 0;
+  int dmax = 1;
+  bool disc = false;
+  for (const auto& r : ranges) {
+    v.src_lo = r.first; v.src_hi = r.second;
+    v.apply_plan(saved);
+    u64 ex; int d; bool dc;
+    v.run_raw(&ex, &d, &dc);
+    tot += ex; dmax = std::max(dmax, d); disc = disc || dc;
+  }
+  v.src_lo = 0; v.src_hi = nsrc;
+  if (v.grid_s) {
+    make_itable(v.width, v.height, v.s, itable.data(), true);
+    v.set_itable(itable.data());
+  }
+  v.apply_plan(saved);
+  *excess = (long long)tot;
+  *diameter = disc ? v.n + 1 : dmax;
+  *disconnected = disc ? 1 : 0;
   return 0;
 }
 
