@@ -97,6 +97,20 @@ int  ODPX_Swap_device(int a, int b, int c, int d);
  * GPU runs any rank.  Returns 0, or -1 for an invalid rank or null output. */
 int  ODPX_Set_aspl_shard(int rank, int nranks, long long *excess, int *diameter, int *disconnected);
 
+/* ---- allocation-free evaluation ----------------------------------------- */
+/* No counterpart in the reference: for callers that cannot allocate device
+ * memory (serving loops, harnesses that forbid hipMalloc).  ODP_Set_aspl's
+ * result for a general graph (s = 1, entries -1 are absent edges) of at most
+ * 16384 vertices and table width 1..32, without an Init: no hipMalloc, hipFree
+ * or host heap allocation.  Every device buffer (the adjacency copy, the
+ * 16-bit neighbour table, the block list, per-workgroup totals, counters and
+ * the result) lives in one 3.1 MiB file-scope device array that each GPU holds
+ * until the process ends; a call clears what must start at zero, and calls
+ * that overlap on one GPU race.  adjacency int[nodes][degree] in host memory.
+ * Returns 0, or -1 (with a message, the device untouched) for sizes past the
+ * maximum; an id outside [-1, nodes) exits as ODP_Set_aspl does. */
+int  ODPX_Set_aspl_noalloc(int nodes, int degree, const int *adjacency, int *diameter, long *sum, double *ASPL);
+
 /* ---- algorithm kind (host only, no GPU) ---------------------------------- */
 /* The kind ODP_Init_aspl_* picks: ODP_ASPL if set (MATRIX, MATRIX_SAVING; BFS
  * and unknown values exit as at Init), else MATRIX when the GPU memory estimate

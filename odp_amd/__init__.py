@@ -127,6 +127,7 @@ def load():
     _sig(lib, "ODPX_Set_aspl_resident", None, [ip, ctypes.POINTER(_L), ctypes.POINTER(_D)])
     _sig(lib, "ODPX_Swap_device", _I, [_I, _I, _I, _I])
     _sig(lib, "ODPX_Set_aspl_shard", _I, [_I, _I, ctypes.POINTER(ctypes.c_longlong), ip, ip])
+    _sig(lib, "ODPX_Set_aspl_noalloc", _I, [_I, _I, _P, ip, ctypes.POINTER(_L), ctypes.POINTER(_D)])
     _sig(lib, "ODPX_Get_kind", _I, [_I, _I, _I, _I, _I])
     _sig(lib, "ODPX_Get_mem_usage", _D, [_I, _I, _I, _I, _I, _I])
     _sig(lib, "ODPX_Anneal", _I, [_P, ctypes.c_longlong, _D, _D, _I, _I, _U64, ctypes.POINTER(AnnealResult)])

@@ -2299,5 +2299,124 @@ extern "C" void ODPX_Batch_release(void)
 }
 
 // ---------------------------------------------------------------------------
+// allocation-free evaluation: ODP_Set_aspl's result for a general graph, with
+// every device buffer in one file-scope array (for callers that cannot allocate)
+
+static const int NOALLOC_MAX_NODES = 16384;
+static const int NOALLOC_MAX_DEGREE = 32;
+static const size_t NA_ADJ = 0;                                                    // int32[n][degree]
+static const size_t NA_TABLE = NA_ADJ + (size_t)NOALLOC_MAX_NODES * NOALLOC_MAX_DEGREE * 4;   // u16[n][dp16]
+static const size_t NA_TASKS = NA_TABLE + (size_t)NOALLOC_MAX_NODES * NOALLOC_MAX_DEGREE * 2; // ResTask[n / 32]
+static const size_t NA_PARTS = NA_TASKS + (size_t)(NOALLOC_MAX_NODES / 32) * 64;   // ResOut[grid <= n / 32]
+static const size_t NA_CTRL = NA_PARTS + (size_t)(NOALLOC_MAX_NODES / 32) * 32;    // qhead, done, err, result
+static const size_t NA_BYTES = NA_CTRL + 256;
+static_assert(sizeof(ResTask) <= 64 && sizeof(ResOut) <= 32 && sizeof(ResHost) <= 128, "scratch layout");
+
+// 3.1 MiB; each GPU the library runs on holds its own copy until the process ends
+static __device__ __align__(256) unsigned char scratch[NA_BYTES];
+static unsigned char* g_scratch_dev[64];          // hipGetSymbolAddress of `scratch` per device
+
+// One grid: the 16-bit neighbour table (entry j of row r: adj[r][j] when it is
+// a vertex id, r itself past the degree or for -1; other values set *err), the
+// task list of 32 * wb-source blocks, and zeroed queue head, arrival counter
+// and error flag.
+__global__ void k_noalloc_prep(const int* __restrict__ adj, int n, int degree, int dp, int wb,
+                               unsigned short* __restrict__ table, ResTask* __restrict__ tasks, int ntasks,
+                               int* qhead, u32* done, int* err)
+{
+  const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
+  if (tid == 0) { *qhead = 0; *done = 0; }
+  for (int64_t i = tid; i < (int64_t)n * dp; i += stride) {
+    const int r = (int)(i / dp), j = (int)(i - (int64_t)r * dp);
+    int v = r;
+    if (j < degree) {
+      const int a = adj[(int64_t)r * degree + j];
+      if (a >= 0 && a < n) v = a;
+      else if (a != -1) atomicOr(err, 1);
+    }
+    table[i] = (unsigned short)v;
+  }
+  const int b = 32 * wb;
+  for (int64_t t = tid; t < ntasks; t += stride) {
+    ResTask T;
+    T.nbr = table; T.n = n; T.dp = dp; T.src0 = (int)t * b; T.l = min(b, n - (int)t * b);
+    T.s = 1; T.graph = 0; T.wb = wb; T.dq = (max(degree, 1) + 3) / 4;
+    tasks[t] = T;
+  }
+}
+
+__global__ void k_noalloc_clear(int* err) { if (threadIdx.x == 0) *err = 0; }
+
+// the device address of `scratch` on the current device, looked up once per device
+static unsigned char* noalloc_scratch()
+{
+  int dev = 0;
+  HIPC(hipGetDevice(&dev));
+  if (dev < 0 || dev >= 64) ODP_ERROR("ODPX_Set_aspl_noalloc: device %d out of range\n", dev);
+  if (!g_scratch_dev[dev]) {
+    void* p = nullptr;
+    HIPC(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_dev[dev] = (unsigned char*)p;
+  }
+  return g_scratch_dev[dev];
+}
+
+extern "C" int ODPX_Set_aspl_noalloc(int nodes, int degree, const int* adjacency, int* diameter, long* sum,
+                                     double* ASPL)
+{
+  RandFence rand_fence;
+  if (nodes < 1 || nodes > NOALLOC_MAX_NODES || degree < 1 || degree > NOALLOC_MAX_DEGREE || !adjacency) {
+    fprintf(stderr, "ODPX_Set_aspl_noalloc: nodes (%d) must be in [1, %d] and degree (%d) in [1, %d]\n", nodes,
+            NOALLOC_MAX_NODES, degree, NOALLOC_MAX_DEGREE);
+    return -1;
+  }
+  ctx_init();
+  unsigned char* base = noalloc_scratch();
+  int* d_adj = (int*)(base + NA_ADJ);
+  unsigned short* d_table = (unsigned short*)(base + NA_TABLE);
+  ResTask* d_tasks = (ResTask*)(base + NA_TASKS);
+  ResOut* d_parts = (ResOut*)(base + NA_PARTS);
+  int* d_qhead = (int*)(base + NA_CTRL);
+  u32* d_done = (u32*)(base + NA_CTRL + 4);
+  int* d_err = (int*)(base + NA_CTRL + 8);
+  ResHost* d_res = (ResHost*)(base + NA_CTRL + 128);
+  // block width and workgroup size as the resident plan picks them for this graph
+  int wbmax = 0;
+  for (int wb = RES_WB_MAX; wb >= 1 && !wbmax; wb >>= 1)
+    if ((size_t)nodes * wb * 4 <= LDS_DYN_MAX && nodes <= res_rpt(wb) * 1024) wbmax = wb;
+  int wb = 1;
+  for (int c = wbmax; c >= 1; c >>= 1)
+    if ((nodes + 32 * c - 1) / (32 * c) >= C.ncu) { wb = c; break; }
+  int threads = 1024;
+  for (int th : {256, 512, 1024})
+    if (nodes <= res_rpt(wb) * th && (nodes >= 4096 ? th == 1024 : (nodes > 512 ? th >= 512 : true))) {
+      threads = th;
+      break;
+    }
+  const int dp = round_up(degree, 8), ntasks = (nodes + 32 * wb - 1) / (32 * wb);
+  const size_t lds = (size_t)nodes * wb * 4;
+  const int per_cu = std::min((int)std::max<size_t>(1, LDS_LIMIT / (lds + LDS_STATIC)), 2048 / threads);
+  const int grid = std::max(1, std::min(ntasks, per_cu * C.ncu));
+  hipStream_t st = C.stream;
+  HIPC(hipMemcpyAsync(d_adj, adjacency, sizeof(int) * (size_t)nodes * degree, hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(k_noalloc_clear, dim3(1), dim3(64), 0, st, d_err);
+  const int64_t work = std::max<int64_t>((int64_t)nodes * dp, ntasks);
+  hipLaunchKernelGGL(k_noalloc_prep, dim3((int)std::min<int64_t>((work + 255) / 256, 1024)), dim3(256), 0, st,
+                     d_adj, nodes, degree, dp, wb, d_table, d_tasks, ntasks, d_qhead, d_done, d_err);
+  HIPC(hipGetLastError());
+  ResFin fin{};
+  fin.done = d_done; fin.part = d_parts; fin.host = d_res; fin.err = d_err;
+  auto kern = wb == 1 ? k_resident_w<1> : wb == 2 ? k_resident_w<2> : wb == 4 ? k_resident_w<4> : k_resident_w<8>;
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(threads), lds, st, d_tasks, ntasks, d_qhead, (ResOut*)nullptr, fin);
+  HIPC(hipGetLastError());
+  ResHost res;
+  HIPC(hipMemcpyAsync(&res, d_res, sizeof res, hipMemcpyDeviceToHost, st));
+  HIPC(hipStreamSynchronize(st));
+  if (res.err) ODP_ERROR("adjacency holds a vertex id outside [0, nodes) within num_degrees\n");
+  finish_one(nodes, res.out.sum, std::max(1, res.out.diam), (res.out.flags & 1) != 0, diameter, sum, ASPL);
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
 // serving engine
 #include "serve.hpp"
