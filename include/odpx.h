@@ -14,6 +14,7 @@
 #ifndef ODP_HIP_ODPX_H
 #define ODP_HIP_ODPX_H
 #include <stdint.h>
+#include "odp.h"
 
 #ifdef __cplusplus
 extern "C" {
@@ -195,14 +196,30 @@ typedef struct {
   int    optimum;             /* 1: the lower bounds of ODP_Set_lbounds_general reached */
   double seconds;
 } ODPX_AnnealResult;
+/* One 2-opt move of the reference's kind (the move of ODP_Mutate_adjacency_*,
+ * utils.c:571-990, with its rejections and symmetric images) on a base table,
+ * drawn from a seeded generator instead of libc rand(): family 0 general
+ * (nodes_or_width = nodes), 1 grid (width, height; edges at most `length` long).
+ * The undo record goes to r, for ODP_Restore_adjacency_general / _grid.
+ * Returns 0, or -1 for invalid parameters. */
+int ODPX_Mutate(int family, int nodes_or_width, int height, int degree, int length, int symmetries,
+                void *adjacency, uint64_t rng[4], ODP_Restore *r);
 /* The annealing loop of the reference's sample/SA/general.c:149-181 for a general
- * graph (needs an active ODP_Init_aspl_general(nodes, degree, ...)): ncalcs times,
+ * graph (an active ODP_Init_aspl_general(nodes, degree, ...)): ncalcs times,
  * one random 2-opt move (ODPX_Two_opt, generator seeded by seed) applied to the
  * device copy, an exact evaluation (as ODP_Set_aspl), the best graph kept, and the
  * acceptance test of sample/SA/common.c:15-43 (temperature from max_temp to
  * min_temp geometrically; hill_climbing accepts no worse ASPL) -- a rejected move
  * is undone on the device.  adjacency int[nodes][degree]: in, the start graph;
- * out, the best graph.  Stops early at the lower bounds.  0, or -1 if invalid. */
+ * out, the best graph.  Stops early at the lower bounds.  0, or -1 if invalid.
+ * Under the resident plan, a regular table whose block and neighbour table fit
+ * one workgroup's LDS runs many steps per cooperative launch with no host round
+ * trip per step (k_anneal: every workgroup draws the same move from a copy of
+ * the generator); ODP_HIP_ANNEAL=host keeps one launch per step.
+ * Symmetric general (general_s) and grid Inits anneal the base table the Init
+ * takes with ODPX_Mutate's moves (grid edges kept within the start graph's
+ * longest Manhattan length), the acceptance test accept_s (sample/SA/common.c:
+ * 15-37, the difference divided by symmetries) and the reference's restores. */
 int ODPX_Anneal(void *adjacency, long long ncalcs, double max_temp, double min_temp, int hill_climbing,
                 int ASPL_priority, uint64_t seed, ODPX_AnnealResult *result);
 

@@ -42,6 +42,11 @@ class AnnealResult(ctypes.Structure):
         return {name: getattr(self, name) for name, _ in self._fields_}
 
 
+class Restore(ctypes.Structure):
+    """ODP_Restore (include/odp.h): the undo record of one 2-edge swap."""
+    _fields_ = [("u", ctypes.c_int * 2), ("v", ctypes.c_int * 2), ("u_d", ctypes.c_int * 2), ("v_d", ctypes.c_int * 2)]
+
+
 class Stats(ctypes.Structure):
     """ODPX_Stats (include/odpx.h)."""
     _fields_ = [("evals", ctypes.c_longlong), ("eval_ms", ctypes.c_double),
@@ -131,6 +136,9 @@ def load():
     _sig(lib, "ODPX_Get_kind", _I, [_I, _I, _I, _I, _I])
     _sig(lib, "ODPX_Get_mem_usage", _D, [_I, _I, _I, _I, _I, _I])
     _sig(lib, "ODPX_Anneal", _I, [_P, ctypes.c_longlong, _D, _D, _I, _I, _U64, ctypes.POINTER(AnnealResult)])
+    _sig(lib, "ODPX_Mutate", _I, [_I, _I, _I, _I, _I, _I, _P, _P, ctypes.POINTER(Restore)])
+    _sig(lib, "ODP_Restore_adjacency_general", None, [Restore, _P])
+    _sig(lib, "ODP_Restore_adjacency_grid", None, [Restore, _P])
     _sig(lib, "ODPX_Stats_enable", None, [_I])
     _sig(lib, "ODPX_Stats_reset", None, [])
     _sig(lib, "ODPX_Stats_get", None, [ctypes.POINTER(Stats)])
@@ -431,13 +439,33 @@ def two_opt(adjacency, rng):
     return tuple(int(v) for v in sw)
 
 
+def mutate(adjacency, rng, nodes=None, width=None, height=None, length=0, symmetries=1):
+    """ODPX_Mutate in place on a base table: one 2-opt move of the reference's kind
+    drawn from rng -> the ODP_Restore record."""
+    assert adjacency.dtype == np.int32 and adjacency.flags.c_contiguous
+    r = Restore()
+    fam, nw = (0, nodes) if width is None else (1, width)
+    if load().ODPX_Mutate(fam, nw, height or 0, adjacency.shape[1], length, symmetries, _ptr(adjacency),
+                          _ptr(rng.state), ctypes.byref(r)) != 0:
+        raise ValueError("ODPX_Mutate: invalid parameters")
+    return r
+
+
+def restore_general(r, adjacency):
+    load().ODP_Restore_adjacency_general(r, _ptr(adjacency))
+
+
+def restore_grid(r, adjacency):
+    load().ODP_Restore_adjacency_grid(r, _ptr(adjacency))
+
+
 def anneal(adjacency, ncalcs, max_temp=100.0, min_temp=0.217147, hill_climbing=False, aspl_priority=False, seed=1):
     """ODPX_Anneal on the active general Init: returns (best adjacency, result dict)."""
     a = np.array(adjacency, dtype=np.int32, order="C", copy=True)
     r = AnnealResult()
     if load().ODPX_Anneal(_ptr(a), int(ncalcs), float(max_temp), float(min_temp), int(bool(hill_climbing)),
                           int(bool(aspl_priority)), int(seed), ctypes.byref(r)) != 0:
-        raise ValueError("ODPX_Anneal: invalid arguments or no general s = 1 Init")
+        raise ValueError("ODPX_Anneal: invalid arguments or no active Init")
     return a, r.as_dict()
 
 

@@ -128,6 +128,10 @@ static void ctx_init()
   HIPC(hipFuncSetAttribute((const void*)k_resident_w<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_DYN_MAX));
   HIPC(hipFuncSetAttribute((const void*)k_resident_w<4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_DYN_MAX));
   HIPC(hipFuncSetAttribute((const void*)k_resident_w<8>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_DYN_MAX));
+  HIPC(hipFuncSetAttribute((const void*)k_anneal<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_DYN_MAX));
+  HIPC(hipFuncSetAttribute((const void*)k_anneal<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_DYN_MAX));
+  HIPC(hipFuncSetAttribute((const void*)k_anneal<4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_DYN_MAX));
+  HIPC(hipFuncSetAttribute((const void*)k_anneal<8>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_DYN_MAX));
   HIPC(hipFuncSetAttribute((const void*)k_direct<true, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_DYN_MAX));
   HIPC(hipFuncSetAttribute((const void*)k_direct<true, false>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_DYN_MAX));
   HIPC(hipFuncSetAttribute((const void*)k_direct<false, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_DYN_MAX));
@@ -1762,13 +1766,193 @@ static bool sa_accept(int nodes, int cur_d, int d, double cur_a, double a, doubl
   return exp(diff / temp) > u;
 }
 
+// The device loop (k_anneal) applies to a regular general graph under the
+// resident plan whose block and whole 16-bit table fit one workgroup's LDS, with
+// one workgroup per block and every block's workgroup resident at once
+// (a cooperative launch).  ODP_HIP_ANNEAL=host keeps the host-stepped loop.
+static size_t anneal_lds(const Evaluator& v)
+{
+  return (size_t)((v.n * v.rr.wb + 3) & ~3) * 4 + (size_t)v.n * v.dp16 * 2;
+}
+
+static bool anneal_device_ok(const Evaluator& v, const std::vector<int>& cur)
+{
+  const char* e = getenv("ODP_HIP_ANNEAL");
+  if (e && strcmp(e, "host") == 0) return false;
+  if (v.plan.plan != ODPX_PLAN_RESIDENT || v.s != 1 || v.family != 0 || v.has_numdeg || v.degree < 1) return false;
+  for (int x : cur) if (x < 0 || x >= v.n) return false;
+  const int G = v.rr.ntasks;
+  if (G < 1 || G > C.ncu || anneal_lds(v) > LDS_DYN_MAX) return false;
+  int dev = 0, coop = 0;
+  HIPC(hipGetDevice(&dev));
+  HIPC(hipDeviceGetAttribute(&coop, hipDeviceAttributeCooperativeLaunch, dev));
+  if (!coop) return false;
+  int occ = 0;
+  const int wb = v.rr.wb;
+  const void* f = wb == 1 ? (const void*)k_anneal<1> : wb == 2 ? (const void*)k_anneal<2>
+                : wb == 4 ? (const void*)k_anneal<4> : (const void*)k_anneal<8>;
+  HIPC(hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, f, v.rr.threads, anneal_lds(v)));
+  return occ * C.ncu >= G;
+}
+
+static long long anneal_chunk_env()
+{
+  const char* e = getenv("ODP_HIP_ANNEAL_CHUNK");
+  const long long c = e ? atoll(e) : 0;
+  return c > 0 ? c : 4096;
+}
+
+static void anneal_on_device(Evaluator& v, int* adj_out, const std::vector<int>& start, const uint64_t rng[4],
+                             long long ncalcs, double max_temp, double cooling, bool hill, bool prio, int low_d,
+                             double low_a, int& cur_d, double& cur_a, int& best_d, long& best_sum, double& best_a,
+                             long long& steps, long long& accepted, bool& optimum)
+{
+  const int n = v.n, deg = v.degree, G = v.rr.ntasks, wb = v.rr.wb;
+  v.flush_adj();
+  AnnealState S{};
+  for (int i = 0; i < 4; i++) S.rng[i] = rng[i];
+  S.cur_a = cur_a; S.best_a = best_a; S.temp = max_temp; S.cooling = cooling; S.low_a = low_a;
+  S.steps = 0; S.accepted = 0; S.best_sum = best_sum;
+  S.cur_d = cur_d; S.best_d = best_d; S.low_d = low_d;
+  S.hill = hill ? 1 : 0; S.prio = prio ? 1 : 0;
+  AnnealState* d_st = nullptr;
+  ResOut* d_slots = nullptr;
+  u32* d_bar = nullptr;
+  int* d_best = nullptr;
+  dalloc(&d_st, 1); dalloc(&d_slots, (size_t)2 * G); dalloc(&d_bar, 1); dalloc(&d_best, (size_t)n * deg);
+  HIPC(hipMemcpyAsync(d_st, &S, sizeof S, hipMemcpyHostToDevice, C.stream));
+  const size_t lds = anneal_lds(v);
+  const void* f = wb == 1 ? (const void*)k_anneal<1> : wb == 2 ? (const void*)k_anneal<2>
+                : wb == 4 ? (const void*)k_anneal<4> : (const void*)k_anneal<8>;
+  const long long chunk = anneal_chunk_env();
+  const double t_dev = wall_time();
+  while (S.steps < ncalcs && !S.optimum && !S.stop) {
+    AnnealArgs aa;
+    aa.adj = v.d_adj; aa.best = d_best; aa.st = d_st; aa.slots = d_slots; aa.bar = d_bar;
+    aa.n = n; aa.degree = deg; aa.dp = v.dp16; aa.ntasks = G;
+    aa.steps = (int)std::min<long long>(chunk, ncalcs - S.steps);
+    hipLaunchKernelGGL(k_clear, dim3(1), dim3(64), 0, C.stream, (u64*)nullptr, 0, (int*)d_bar, 1);
+    void* args[] = {&aa};
+    HIPC(hipLaunchCooperativeKernel(f, dim3(G), dim3(v.rr.threads), args, (unsigned)lds, C.stream));
+    HIPC(hipMemcpyAsync(&S, d_st, sizeof S, hipMemcpyDeviceToHost, C.stream));
+    HIPC(hipStreamSynchronize(C.stream));
+    if (S.err) ODP_ERROR("libodp_hip: internal error: an annealing step's grid barrier timed out\n");
+  }
+  E.elapsed += wall_time() - t_dev;
+  E.times += (unsigned)S.steps;
+  // the caller's array receives the best graph (unchanged when nothing improved)
+  if (S.best_d != best_d || S.best_a != best_a)
+    HIPC(hipMemcpy(adj_out, d_best, sizeof(int) * (size_t)n * deg, hipMemcpyDeviceToHost));
+  else
+    memcpy(adj_out, start.data(), sizeof(int) * (size_t)n * deg);
+  dfree(&d_st); dfree(&d_slots); dfree(&d_bar); dfree(&d_best);
+  // the device copy now holds the chain's current graph; the 16-bit table is rebuilt from it
+  v.stage_is_device = false;
+  v.table_fresh = false;
+  steps = S.steps; accepted = S.accepted;
+  cur_d = S.cur_d; cur_a = S.cur_a;
+  best_d = S.best_d; best_sum = (long)S.best_sum; best_a = S.best_a;
+  optimum = S.optimum != 0;
+}
+
+// acceptance test of sample/SA/common.c:15-43 (accept_s: the ASPL difference
+// scaled by n(n-1)/symmetries); the uniform draw comes from the run's generator
+static bool sa_accept_s(int nodes, int cur_d, int d, double cur_a, double a, double temp, bool hill, bool aspl_prio,
+                        int symmetries, uint64_t rng[4])
+{
+  if (hill) return a <= cur_a;
+  if (d < cur_d && !aspl_prio) return true;
+  if (d > cur_d && !aspl_prio) return false;
+  if (a <= cur_a) return true;
+  const double diff = ((cur_a - a) * nodes * (nodes - 1)) / symmetries;
+  const double u = ((double)(ODPX_Rng_next(rng) >> 11) + 0.5) / 9007199254740992.0;   // (0, 1)
+  return exp(diff / temp) > u;
+}
+
+// Longest Manhattan length of a grid table's edges (base rows: the vertices
+// ODP_Conv_edge2adjacency_grid_s keeps, utils.c:1116-1151)
+static int grid_table_length(const Evaluator& v, const int* base)
+{
+  const int based = v.n / v.s, bh = v.height / 2;
+  int len = 0;
+  for (int i = 0; i < based; i++) {
+    const int x = v.s == 4 ? (i / bh) * v.height + (i % bh) : i;
+    for (int j = 0; j < v.degree; j++) {
+      const int u = base[(size_t)i * v.degree + j];
+      if (u < 0 || u >= v.n) continue;
+      len = std::max(len, abs(u / v.height - x / v.height) + abs(u % v.height - x % v.height));
+    }
+  }
+  return len;
+}
+
+// The annealing loop of sample/SA/general_s.c:160-194 and grid(_s).c:170-199 for
+// symmetric general and grid Inits: each step one move of the reference's kind
+// (ODPX_Mutate, drawn from the run's generator) on the host table, ODP_Set_aspl,
+// the best graph kept, accept_s, and ODP_Restore_adjacency_* on a rejection.
+// Grid moves keep every edge within the start graph's longest Manhattan length.
+static int anneal_symmetric(int* adj, long long ncalcs, double max_temp, double min_temp, bool hill, bool prio,
+                            uint64_t seed, ODPX_AnnealResult* res)
+{
+  Evaluator& v = E.ev;
+  const double t0 = wall_time();
+  const int n = v.n, deg = v.degree, s = v.s, based = n / s;
+  const bool grid = v.family == 1;
+  std::vector<int> cur(adj, adj + (size_t)based * deg);
+  const int length = grid ? grid_table_length(v, cur.data()) : 0;
+  uint64_t rng[4];
+  ODPX_Rng_seed(seed, rng);
+  int diameter; long sum; double ASPL;
+  ODP_Set_aspl(cur.data(), &diameter, &sum, &ASPL);
+  int cur_d = diameter, best_d = diameter;
+  double cur_a = ASPL, best_a = ASPL;
+  long best_sum = sum;
+  int low_d; double low_a;
+  if (grid) ODP_Set_lbounds_grid(v.width, v.height, deg, length, &low_d, &low_a);
+  else ODP_Set_lbounds_general(n, deg, &low_d, &low_a);
+  long long steps = 0, accepted = 0;
+  bool optimum = diameter == low_d && ASPL == low_a;
+  const double cooling = hill ? 0.0 : pow(min_temp / max_temp, 1.0 / (double)ncalcs);
+  double temp = max_temp;
+  for (long long i = 0; i < ncalcs && !optimum; i++) {
+    ODP_Restore r;
+    if (ODPX_Mutate(grid ? 1 : 0, grid ? v.width : n, v.height, deg, length, s, cur.data(), rng, &r) != 0) break;
+    ODP_Set_aspl(cur.data(), &diameter, &sum, &ASPL);
+    steps++;
+    if (diameter < best_d || (diameter == best_d && ASPL < best_a)) {
+      best_d = diameter; best_sum = sum; best_a = ASPL;
+      memcpy(adj, cur.data(), sizeof(int) * (size_t)based * deg);
+      if (diameter == low_d && ASPL == low_a) { optimum = true; break; }
+    }
+    if (sa_accept_s(n, cur_d, diameter, cur_a, ASPL, temp, hill, prio, s, rng)) {
+      cur_d = diameter; cur_a = ASPL;
+      accepted++;
+    } else if (grid) {
+      ODP_Restore_adjacency_grid(r, cur.data());
+    } else {
+      ODP_Restore_adjacency_general(r, cur.data());
+    }
+    temp *= cooling;
+  }
+  if (res) {
+    res->steps = steps; res->accepted = accepted;
+    res->best_diameter = best_d; res->best_sum = best_sum; res->best_ASPL = best_a;
+    res->diameter = cur_d; res->ASPL = cur_a; res->optimum = optimum ? 1 : 0;
+    res->seconds = wall_time() - t0;
+  }
+  return 0;
+}
+
 extern "C" int ODPX_Anneal(void* adjacency, long long ncalcs, double max_temp, double min_temp, int hill_climbing,
                            int ASPL_priority, uint64_t seed, ODPX_AnnealResult* res)
 {
   RandFence rand_fence;
   need_active("ODPX_Anneal");
   Evaluator& v = E.ev;
-  if (v.s != 1 || v.family != 0 || ncalcs < 0 || (!hill_climbing && (max_temp <= 0 || min_temp <= 0))) return -1;
+  if (ncalcs < 0 || (!hill_climbing && (max_temp <= 0 || min_temp <= 0))) return -1;
+  if (v.s != 1 || v.family != 0)
+    return anneal_symmetric((int*)adjacency, ncalcs, max_temp, min_temp, hill_climbing != 0, ASPL_priority != 0, seed,
+                            res);
   const double t0 = wall_time();
   const int n = v.n, deg = v.degree;
   int* adj = (int*)adjacency;
@@ -1802,6 +1986,18 @@ extern "C" int ODPX_Anneal(void* adjacency, long long ncalcs, double max_temp, d
     const int e[12] = {a, pos_of(a, c), c, b, pos_of(b, d), d, c, pos_of(c, a), a, d, pos_of(d, b), b};
     v.pend_patch.insert(v.pend_patch.end(), e, e + 12);
   };
+  if (!optimum && ncalcs > 0 && anneal_device_ok(v, cur)) {
+    // every step on the device: a cooperative launch runs many of them
+    anneal_on_device(v, adj, cur, rng, ncalcs, max_temp, cooling, hill_climbing != 0, ASPL_priority != 0, low_d, low_a,
+                     cur_d, cur_a, best_d, best_sum, best_a, steps, accepted, optimum);
+    if (res) {
+      res->steps = steps; res->accepted = accepted;
+      res->best_diameter = best_d; res->best_sum = best_sum; res->best_ASPL = best_a;
+      res->diameter = cur_d; res->ASPL = cur_a; res->optimum = optimum ? 1 : 0;
+      res->seconds = wall_time() - t0;
+    }
+    return 0;
+  }
   for (long long i = 0; i < ncalcs && !optimum; i++) {
     int sw[4];
     if (ODPX_Two_opt(n, deg, cur.data(), rng, sw) != 0) break;      // no valid move left

@@ -1609,3 +1609,199 @@ __global__ __launch_bounds__(1024) void k_resident(const ResTask* __restrict__ t
     res_commit(out, T.graph, b, lane, wave);
   }
 }
+
+// ---------------------------------------------------------------------------
+// annealing on the device (k_anneal): many steps of the annealing loop of
+// sample/SA/general.c:149-181 in one cooperative launch, for a regular general
+// graph (s = 1) whose blocks fit LDS together with its neighbour table.
+//
+// Every workgroup holds the whole 16-bit neighbour table in LDS (the first
+// `degree` entries of a row are its adjacency row) and evaluates one block of
+// 32 * WB sources.  Thread 0 of every workgroup keeps the same copy of the
+// chain's state -- generator, current and best results, temperature -- and
+// draws the same 2-opt move the host's ODPX_Two_opt would draw from it, so the
+// tables stay identical without any exchange.  One grid barrier per step: each
+// workgroup publishes its block's totals, waits for all of them, adds them up
+// in the same order and makes the same acceptance decision (the test of
+// sample/SA/common.c:15-43); a rejected move is undone in every table.
+
+struct AnnealState {
+  uint64_t rng[4];
+  double cur_a, best_a, temp, cooling, low_a;
+  long long steps, accepted, best_sum;
+  int cur_d, best_d, low_d;
+  int optimum;          // the lower bounds were reached
+  int stop;             // no valid move was found
+  int hill, prio;
+  int err;              // a grid barrier timed out
+};
+
+struct AnnealArgs {
+  int* adj;             // [n][degree]: the chain's current graph (read at start, written at the end)
+  int* best;            // [n][degree]: the best graph, written on every improvement
+  AnnealState* st;
+  ResOut* slots;        // [2][ntasks] block totals of a step (double buffered by step parity)
+  u32* bar;             // grid barrier counter (zero at launch)
+  int n, degree, dp, ntasks, steps;
+};
+
+#define ANNEAL_SPIN_LIMIT (1ull << 24)
+
+__device__ __forceinline__ u32 anneal_rnd(uint64_t s[4], u32 bound)
+{
+  // xoshiro256** step and Lemire's multiply-shift, as odp_gen.c's rnd()
+  const uint64_t r = ((s[1] * 5) << 7 | (s[1] * 5) >> 57) * 9, t = s[1] << 17;
+  s[2] ^= s[0]; s[3] ^= s[1]; s[1] ^= s[2]; s[0] ^= s[3]; s[2] ^= t; s[3] = (s[3] << 45) | (s[3] >> 19);
+  return (u32)(((r >> 32) * (uint64_t)bound) >> 32);
+}
+
+__device__ __forceinline__ uint64_t anneal_next(uint64_t s[4])
+{
+  const uint64_t r = ((s[1] * 5) << 7 | (s[1] * 5) >> 57) * 9, t = s[1] << 17;
+  s[2] ^= s[0]; s[3] ^= s[1]; s[1] ^= s[2]; s[0] ^= s[3]; s[2] ^= t; s[3] = (s[3] << 45) | (s[3] >> 19);
+  return r;
+}
+
+__device__ __forceinline__ bool anneal_has(const unsigned short* tab, int dp, int degree, int a, int b)
+{
+  for (int j = 0; j < degree; j++) if (tab[a * dp + j] == b) return true;
+  return false;
+}
+
+__device__ __forceinline__ void anneal_replace(unsigned short* tab, int dp, int degree, int row, int from, int to)
+{
+  for (int j = 0; j < degree; j++) if (tab[row * dp + j] == from) { tab[row * dp + j] = (unsigned short)to; return; }
+}
+
+// a-b, c-d -> a-c, b-d in the table (ODPX_Swap_edges' order of replacements)
+__device__ __forceinline__ void anneal_swap(unsigned short* tab, int dp, int degree, int a, int b, int c, int d)
+{
+  anneal_replace(tab, dp, degree, a, b, c);
+  anneal_replace(tab, dp, degree, b, a, d);
+  anneal_replace(tab, dp, degree, c, d, a);
+  anneal_replace(tab, dp, degree, d, c, b);
+}
+
+template <int WB>
+__global__ __launch_bounds__(1024) void k_anneal(AnnealArgs a)
+{
+  extern __shared__ __attribute__((aligned(16))) u32 lds[];
+  __shared__ u64 s_part[2][16];
+  __shared__ int s_mv[5], s_dec[3];
+  __shared__ u64 s_sum;
+  __shared__ int s_diam, s_flags;
+  __shared__ AnnealState S;          // the chain's state; only thread 0 reads or writes it
+  const int tid = threadIdx.x, nth = blockDim.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6), nw = nth >> 6;
+  const int g = blockIdx.x, G = gridDim.x, n = a.n, deg = a.degree, dp = a.dp;
+  u32* bits = lds;
+  unsigned short* tab = (unsigned short*)(lds + ((n * WB + 3) & ~3));
+  for (int i = tid; i < n * dp; i += nth) {
+    const int r = i / dp, j = i - r * dp;
+    tab[i] = (unsigned short)(j < deg ? a.adj[(int64_t)r * deg + j] : r);
+  }
+  if (tid == 0) S = *a.st;
+  __syncthreads();
+  ResTask T;
+  T.nbr = tab; T.n = n; T.dp = dp; T.src0 = g * 32 * WB; T.l = min(32 * WB, n - T.src0);
+  T.s = 1; T.graph = 0; T.wb = WB; T.dq = (max(deg, 1) + 3) / 4;
+  for (int k = 0; k < a.steps; k++) {                 // uniform: every exit below is uniform
+    if (tid == 0) {
+      int found = 0, ma = 0, mb = 0, mc = 0, md = 0;
+      for (int tries = 0; tries < 1000000 && !found; tries++) {
+        const int x = (int)anneal_rnd(S.rng, (u32)n), y = (int)anneal_rnd(S.rng, (u32)n);
+        const int bx = tab[x * dp + anneal_rnd(S.rng, (u32)deg)];
+        const int dy = tab[y * dp + anneal_rnd(S.rng, (u32)deg)];
+        if (x == y || x == dy || bx == y || bx == dy) continue;
+        if (anneal_has(tab, dp, deg, x, y) || anneal_has(tab, dp, deg, bx, dy)) continue;
+        anneal_swap(tab, dp, deg, x, bx, y, dy);
+        found = 1; ma = x; mb = bx; mc = y; md = dy;
+      }
+      s_mv[0] = found; s_mv[1] = ma; s_mv[2] = mb; s_mv[3] = mc; s_mv[4] = md;
+    }
+    __syncthreads();
+    if (!s_mv[0]) {
+      if (tid == 0) S.stop = 1;
+      break;
+    }
+    const ResOut b = res_run<WB>(T, bits, s_part, tid, nth, lane, wave, nw);
+    const int par = k & 1;                             // slots of this step (k is uniform; S.steps is thread 0's)
+    if (tid == 0) {
+      a.slots[par * G + g] = b;
+      __threadfence();
+      atomicAdd(a.bar, 1u);
+      const u32 target = (u32)G * (u32)(k + 1);
+      unsigned long long spins = 0;
+      int abort = 0;
+      while (__hip_atomic_load(a.bar, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) < target) {
+        __builtin_amdgcn_s_sleep(2);
+        if (++spins > ANNEAL_SPIN_LIMIT) { atomicOr(&a.st->err, 1); abort = 1; break; }
+      }
+      s_dec[2] = abort;
+    }
+    __syncthreads();
+    if (s_dec[2]) break;
+    // every block's totals, added in the same order in every workgroup
+    if (wave == 0) {
+      u64 sum = 0;
+      int diam = 0;
+      u32 flags = 0;
+      for (int i = lane; i < G; i += 64) {
+        const ResOut* p = a.slots + par * G + i;
+        sum += __hip_atomic_load(&p->sum, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        diam = max(diam, __hip_atomic_load(&p->diam, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+        flags |= __hip_atomic_load(&p->flags, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      sum = wave_sum(sum); diam = wave_max(diam); flags = wave_or(flags);
+      if (lane == 0) { s_sum = sum; s_diam = diam; s_flags = (int)flags; }
+    }
+    __syncthreads();
+    if (tid == 0) {
+      // the result as ODP_Set_aspl reports it (aspl.c:73-74, 332-336)
+      long lsum = (long)n * (n - 1) + (long)s_sum;
+      int d = max(1, s_diam);
+      if (s_flags & 1) d = n + 1;
+      double aspl = lsum / (((double)n - 1) * n);
+      lsum /= 2.0;
+      if (d > n) { d = INT_MAX; lsum = LONG_MAX; aspl = DBL_MAX; }
+      S.steps++;
+      int improved = 0, accept;
+      if (d < S.best_d || (d == S.best_d && aspl < S.best_a)) {
+        improved = 1;
+        S.best_d = d; S.best_sum = lsum; S.best_a = aspl;
+        if (d == S.low_d && aspl == S.low_a) S.optimum = 1;
+      }
+      if (S.optimum) accept = 1;              // the loop ends here; the move stays
+      else if (S.hill) accept = aspl <= S.cur_a;
+      else if (d < S.cur_d && !S.prio) accept = 1;
+      else if (d > S.cur_d && !S.prio) accept = 0;
+      else if (aspl <= S.cur_a) accept = 1;
+      else {
+        const double diff = (S.cur_a - aspl) * n * (n - 1);
+        const double u = ((double)(anneal_next(S.rng) >> 11) + 0.5) / 9007199254740992.0;
+        accept = exp(diff / S.temp) > u;
+      }
+      if (!S.optimum) {
+        if (accept) { S.cur_d = d; S.cur_a = aspl; S.accepted++; }
+        S.temp *= S.cooling;
+      }
+      s_dec[0] = improved; s_dec[1] = accept;
+    }
+    __syncthreads();
+    if (s_dec[0] && g == 0)                            // the best graph: workgroup 0's table
+      for (int i = tid; i < n * deg; i += nth) a.best[i] = tab[(i / deg) * dp + (i % deg)];
+    __syncthreads();
+    if (!s_dec[1] && tid == 0) anneal_swap(tab, dp, deg, s_mv[1], s_mv[3], s_mv[2], s_mv[4]);   // undo
+    if (tid == 0) s_dec[2] = S.optimum;               // thread 0's state decides, for every thread
+    __syncthreads();
+    if (s_dec[2]) break;
+  }
+  __syncthreads();
+  if (g == 0) {
+    for (int i = tid; i < n * deg; i += nth) a.adj[i] = tab[(i / deg) * dp + (i % deg)];
+    if (tid == 0) {
+      S.err = __hip_atomic_load(&a.st->err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      *a.st = S;
+    }
+  }
+}

@@ -21,12 +21,20 @@
 #include <stdlib.h>
 #include <stdbool.h>
 #include "odp.h"
+#include "odpx.h"
 
 #define MU_FAIL(...) do { fprintf(stderr, __VA_ARGS__); exit(1); } while (0)
 #define GEN_ROUNDS 10      /* mutations per edge when generating (parameter.h:17) */
 
-/* uniform integer in [0, max) from libc rand() (utils.c:356-359) */
-static int draw(int max) { return (int)(rand() * ((double)max) / (1.0 + RAND_MAX)); }
+/* uniform integer in [0, max): from libc rand() (utils.c:356-359), or, inside
+ * ODPX_Mutate, from the caller's seeded generator (multiply-shift of its high
+ * 32 bits) */
+static uint64_t *g_rng = NULL;
+static int draw(int max)
+{
+  if (g_rng) return (int)(((ODPX_Rng_next(g_rng) >> 32) * (uint64_t)max) >> 32);
+  return (int)(rand() * ((double)max) / (1.0 + RAND_MAX));
+}
 
 static int wrapn(int x, int n)
 {
@@ -418,6 +426,33 @@ void ODP_Mutate_adjacency_grid(int width, int height, int degree, int *num_degre
   check_parity(width * height, degree);
   Space S = grid_space(width, height, degree, num_degrees, length, 1, adjacency);
   two_opt(&S, r);
+}
+
+/* One 2-opt move of the reference's kind (the move of ODP_Mutate_adjacency_*,
+ * utils.c:571-990) drawn from a seeded generator instead of libc rand();
+ * family 0 general (nodes_or_width = nodes), 1 grid (width, height, length).
+ * It records the table for ODP_Restore_adjacency_general / _grid in every
+ * case.  Returns 0, or -1 for invalid parameters. */
+int ODPX_Mutate(int family, int nodes_or_width, int height, int degree, int length, int symmetries,
+                void *adjacency, uint64_t rng[4], ODP_Restore *r)
+{
+  if (!adjacency || !rng || !r || degree < 1 || symmetries < 1) return -1;
+  Space S;
+  if (family == 0) {
+    if (nodes_or_width % symmetries != 0) return -1;
+    last.nodes = nodes_or_width; last.degree = degree; last.s = symmetries;
+    S = general_space(nodes_or_width, degree, NULL, symmetries, adjacency);
+  } else {
+    const int w = nodes_or_width;
+    if (symmetries != 1 && symmetries != 2 && symmetries != 4) return -1;
+    if ((symmetries == 2 && w % 2) || (symmetries == 4 && (w % 2 || height % 2 || w != height))) return -1;
+    last.width = w; last.height = height; last.degree = degree; last.s = symmetries;
+    S = grid_space(w, height, degree, NULL, length, symmetries, adjacency);
+  }
+  g_rng = rng;
+  two_opt(&S, r);
+  g_rng = NULL;
+  return 0;
 }
 
 /* utils.c:500-507: puts back the two edges a general mutation replaced */
