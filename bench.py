@@ -205,7 +205,7 @@ def c5_chain(i):
     return odp.generate_regular(C5_NODES[i % 3], C5_DEGREES[(i // 3) % 3], 5000 + i)
 
 
-def served_extra(nchains=64, saturate_s=4.0, poisson_s=6.0, load=0.6, seed=77, small_slots=2, large_slots=4, rate=None):
+def served_extra(nchains=64, saturate_s=4.0, poisson_s=6.0, load=0.6, seed=77, small_slots=1, large_slots=2, rate=None):
     """64 annealing chains, n in {4096, 16384, 65536} x d in {8, 16, 32}, each request one
     random 2-opt swap (accepted: the next request applies to the swapped graph)."""
     t0 = time.perf_counter()
@@ -250,6 +250,7 @@ def served_extra(nchains=64, saturate_s=4.0, poisson_s=6.0, load=0.6, seed=77, s
         # open loop: Poisson arrivals, chains uniform; the mix saturates when its slowest
         # class does, so the rate is `load` of min over classes of capacity / class share
         cap_mix = min(x_large * nchains / max(1, n_large), x_small * nchains / max(1, nchains - n_large))
+        fixed_rate = rate is not None
         rate = max(1.0, load * cap_mix) if rate is None else rate
         arr = np.random.default_rng(seed)
         lat, lat_large, starts = [], [], {}
@@ -289,10 +290,11 @@ def served_extra(nchains=64, saturate_s=4.0, poisson_s=6.0, load=0.6, seed=77, s
                  "p50_ms": float(np.percentile(lat_ms, 50)), "p99_ms": float(np.percentile(lat_ms, 99)),
                  "p50_ms_small": float(np.percentile(lat_s_ms, 50)), "p99_ms_small": float(np.percentile(lat_s_ms, 99)),
                  "p50_ms_large": float(np.percentile(lat_l_ms, 50)), "p99_ms_large": float(np.percentile(lat_l_ms, 99)),
-                 "requests": len(results), "failed": bad,
+                 "requests": len(results), "failed": bad, "slots": {"small": small_slots, "large": large_slots},
+                 "poisson_rate_source": "fixed" if fixed_rate else "60% of the measured mix capacity",
                  "workload": "64 chains, n in {4096,16384,65536} x d in {8,16,32}, one accepted 2-opt swap per "
                              "request; closed loop for the saturated rates, then Poisson arrivals over uniform "
-                             "chains at 60% of the mix's capacity"},
+                             "chains"},
                 sample_checks)
     finally:
         srv.stop()
@@ -433,6 +435,13 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-extras", action="store_true")
     ap.add_argument("--no-served", action="store_true", help="skip the serving-engine workload")
+    ap.add_argument("--served-slots", default="1,2", metavar="SMALL,LARGE",
+                    help="serving engine slots: concurrent small-chain batches, concurrent large-chain evaluations")
+    ap.add_argument("--served-rate", type=float, default=548.0,
+                    help="Poisson arrivals per second for the served workload (0: 60%% of the measured capacity); "
+                         "548/s is 60%% of the capacity measured with 2 small + 4 large slots, kept so that "
+                         "latencies compare at one load")
+    ap.add_argument("--served-only", action="store_true", help="run the served workload alone (no timed evaluation)")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic.json"))
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first run one device-to-device copy of MIB MiB (a plain copy kernel of known "
@@ -445,6 +454,11 @@ def main():
               f"by source-column blocks, one share per GPU, with one 16-byte reduction of the sums)", file=sys.stderr)
         sys.exit(2)
 
+    if args.served_only:
+        sm, lg = (int(x) for x in args.served_slots.split(","))
+        res, _ = served_extra(small_slots=sm, large_slots=lg, rate=args.served_rate or None)
+        print(json.dumps({"served": res, "slots": [sm, lg]}), flush=True)
+        return
     gkind, n, d, desc = WORKLOADS[args.workload]
     if gkind == "batch":
         return batch_workload(args, desc)
@@ -617,7 +631,8 @@ def main():
         bchecks, out["batched"] = batched_extra(1024, 3)
         checks += bchecks
         if not args.no_served:
-            out["served"], schecks = served_extra()
+            sm, lg = (int(x) for x in args.served_slots.split(","))
+            out["served"], schecks = served_extra(small_slots=sm, large_slots=lg, rate=args.served_rate or None)
             checks += schecks
     if not args.no_cpu:
         cb, cres = cpu_baseline(adj, gkw, kind_name, checks=checks)

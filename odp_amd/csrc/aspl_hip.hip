@@ -339,6 +339,7 @@ struct Evaluator {
   LevelRun lr;
   ResRun rr;
   int guess_levels = 0;                 // levels the last level-plan evaluation needed
+  int batch_cap = MAX_BATCH_LEVELS;     // gather levels launched between two host reads (serving: few)
   hipStream_t st = nullptr;             // launch stream (the library's main stream unless set)
   bool timed() const { return g_stats_on && st == C.stream; }
 
@@ -1090,7 +1091,8 @@ struct Evaluator {
   int first_batch_end() const
   {
     if (lv.tk) return tile_batch_end(lv.L1, guess_levels > 0 ? guess_levels - lv.L1 : 8);
-    return std::min(n, std::max(2, std::min(MAX_BATCH_LEVELS, guess_levels > 0 ? guess_levels : 8)));
+    const int be = std::min(n, std::max(2, std::min(MAX_BATCH_LEVELS, guess_levels > 0 ? guess_levels : 8)));
+    return std::min(be, std::max(lv.L1 + batch_cap, 2));
   }
 
   // tile mode: batches are whole launches of lv.tk levels, short enough that
@@ -1193,7 +1195,7 @@ struct Evaluator {
       lv.spec = -1;
     }
     if (lv.running > 0 && lv.launched < n) {
-      const int step = std::min(MAX_BATCH_LEVELS, std::max(lv.launched, 8));
+      const int step = std::min(batch_cap, std::min(MAX_BATCH_LEVELS, std::max(lv.launched, 8)));
       lv.batch_end = lv.tk ? tile_batch_end(lv.launched, step) : std::min(n, lv.launched + step);
       lv_levels(lv.launched, lv.batch_end);
       lv.launched = std::max(lv.launched, lv.batch_end);

@@ -30,6 +30,16 @@
 #include <chrono>
 
 static const int SV_MAX_BATCH = 64;       // requests per small-slot launch
+// Gather levels a large slot enqueues per progress step.  Slots share HIP's
+// hardware queues (4 per process), so a small slot's batch can sit behind a
+// large slot's queued work; one level at a time bounds that wait to about one
+// level (env ODP_HIP_SERVE_LEVELS overrides)
+static int sv_large_levels()
+{
+  const char* v = getenv("ODP_HIP_SERVE_LEVELS");
+  const int k = v ? atoi(v) : 1;
+  return k >= 1 ? k : 1;
+}
 
 static double mono_time()
 {
@@ -254,6 +264,7 @@ static void sv_large_launch(SvLargeSlot& sl, int ci)
     sl.shaped_n = c.n; sl.shaped_d = c.degree;
   }
   ev.ext_adj = c.d_adj;
+  ev.batch_cap = sv_large_levels();
   ev.lv_begin();
   HIPC(hipEventRecord(sl.done, ev.st));
   sl.busy = true;
