@@ -169,6 +169,23 @@ static bool graphs_env()
   return env != 0;
 }
 
+// the first pulled level of a resident block (res_walk_levels; env
+// ODP_HIP_RES_WALK=0 builds level 0 from the table and pulls every later
+// level, N walks the levels below N at most)
+static int res_walk_cap()
+{
+  static int cap = -2;
+  if (cap == -2) { const char* v = getenv("ODP_HIP_RES_WALK"); cap = v ? atoi(v) : -1; }
+  return cap < 0 ? 1 << 20 : std::max(cap, 0);
+}
+
+static int res_walk_env(int n, int deg, int l)
+{
+  static double cost = -1;
+  if (cost < 0) { const char* v = getenv("ODP_HIP_RES_WALK_COST"); cost = v ? atof(v) : 4.0; }
+  return std::min(res_walk_levels(n, deg, l, cost), res_walk_cap());
+}
+
 // cache policy of k_pgather's gathers (env ODP_HIP_PG_AUX: 0 default, 1 sc0, 16 sc1, 2 nt)
 static int pg_aux_env()
 {
@@ -842,6 +859,7 @@ struct Evaluator {
       ResTask& T = ht[t];
       T.nbr = d_nbr16; T.n = n; T.dp = dp16; T.src0 = src_lo + t * b; T.l = std::min(b, src_hi - T.src0);
       T.s = s; T.graph = 0; T.wb = r.wb; T.dq = (std::max(degree, 1) + 3) / 4;
+      T.deg = std::max(degree, 1); T.walk = res_walk_env(n, T.deg, T.l);
     }
     HIPC(hipMemcpy(r.tasks, ht.data(), sizeof(ResTask) * ntasks, hipMemcpyHostToDevice));
     r.ntasks = ntasks;
@@ -1832,6 +1850,7 @@ static void anneal_on_device(Evaluator& v, int* adj_out, const std::vector<int>&
     AnnealArgs aa;
     aa.adj = v.d_adj; aa.best = d_best; aa.st = d_st; aa.slots = d_slots; aa.bar = d_bar;
     aa.n = n; aa.degree = deg; aa.dp = v.dp16; aa.ntasks = G;
+    aa.walk_cap = res_walk_cap();
     aa.steps = (int)std::min<long long>(chunk, ncalcs - S.steps);
     hipLaunchKernelGGL(k_clear, dim3(1), dim3(64), 0, C.stream, (u64*)nullptr, 0, (int*)d_bar, 1);
     void* args[] = {&aa};
@@ -2115,6 +2134,50 @@ extern "C" int ODPX_Autotune(void* adjacency, int repeats)
 
 struct BatchGraph { long long adj_off, nbr_off, ent_start; int n, degree, dp, pad; };
 
+// Orders each row's neighbour list so that the rows a wave reads in one LDS
+// pass (g consecutive rows) fetch rows of different bank classes (id mod m,
+// m = banks / words per row) at every list position: thread t takes group t
+// of the concatenated graphs (gstart: first group of each graph, gstart[ng]
+// the total) and fills the positions in turn, each row taking the id of its
+// remaining ones whose class is the least used at that position so far.  A
+// row's list only changes order, so every level ORs the same rows.
+__global__ void k_balance(const BatchGraph* __restrict__ gs, const int* __restrict__ gstart, int ng, int g,
+                          unsigned short* __restrict__ nbr)
+{
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= gstart[ng]) return;
+  int lo = 0, hi = ng - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (gstart[mid] <= t) lo = mid; else hi = mid - 1;
+  }
+  const BatchGraph G = gs[lo];
+  const u32 m = (u32)max(1, min(G.pad, 64));
+  const int r0 = (t - gstart[lo]) * g, r1 = min(r0 + g, G.n);
+  unsigned short* base = nbr + G.nbr_off;
+  for (int j = 0; j < G.degree; j++) {
+    u64 used[8] = {0, 0, 0, 0, 0, 0, 0, 0};          // used[k] bit c: class c taken more than k times
+    for (int r = r0; r < r1; r++) {
+      unsigned short* row = base + (int64_t)r * G.dp;
+      int best = j, bc = 9;
+      for (int q = j; q < G.degree; q++) {
+        const u32 c = (u32)row[q] % m;
+        int cnt = 0;
+#pragma unroll
+        for (int k = 0; k < 8; k++) cnt += (int)((used[k] >> c) & 1);
+        if (cnt < bc) { bc = cnt; best = q; if (cnt == 0) break; }
+      }
+      const unsigned short x = row[best];
+      row[best] = row[j];
+      row[j] = x;
+      const u32 c = (u32)x % m;
+#pragma unroll
+      for (int k = 0; k < 8; k++)
+        if (!((used[k] >> c) & 1)) { used[k] |= 1ULL << c; break; }
+    }
+  }
+}
+
 __global__ void k_prepare_batch(const BatchGraph* __restrict__ gs, int ng, long long total,
                                 const int* __restrict__ adj, unsigned short* __restrict__ nbr, int* err)
 {
@@ -2157,6 +2220,8 @@ static struct {
   int *d_adj = nullptr, *d_err = nullptr, *d_qhead = nullptr, *h_err = nullptr;
   unsigned short* d_nbr = nullptr;
   BatchGraph* d_g = nullptr;
+  int* d_gstart = nullptr;   // [small graphs + 1] first balance group of each graph
+  size_t gstart_cap = 0;
   ResOut *d_out = nullptr, *h_out = nullptr;
   ResTask* d_t = nullptr;
   Evaluator single;          // graphs too large for the LDS path, and the looped path
@@ -2217,6 +2282,23 @@ static void single_eval_dev(Evaluator& v, int n, int degree, const int* dev_adj,
   if (*diameter > n) { *diameter = INT_MAX; *sum = LONG_MAX; *ASPL = DBL_MAX; }
 }
 
+// neighbour-list order of batched graphs (env ODP_HIP_BALANCE="banks,rows":
+// LDS banks of 4 bytes and rows per LDS pass, default 64,32; "0" keeps the
+// given order)
+static void balance_env(int* banks, int* group)
+{
+  static int b = -1, g = 0;
+  if (b < 0) {
+    b = 64; g = 32;
+    if (const char* v = getenv("ODP_HIP_BALANCE")) {
+      g = 32;
+      if (sscanf(v, "%d,%d", &b, &g) < 1) b = 0;
+      if (b < 0 || b > 64 || (b & (b - 1)) || g < 1 || g > 64) b = 0;
+    }
+  }
+  *banks = b; *group = g;
+}
+
 // the resident launch's task table under the current policy: per graph the
 // widest block its size class allows, longest blocks first (short queue tail)
 static void batch_build_tasks()
@@ -2236,6 +2318,7 @@ static void batch_build_tasks()
       ResTask T;
       T.nbr = B.d_nbr + G.nbr_off; T.n = G.n; T.dp = G.dp; T.src0 = t * b; T.l = std::min(b, G.n - t * b);
       T.s = 1; T.graph = (int)k; T.wb = wb; T.dq = (std::max(G.degree, 1) + 3) / 4;
+      T.deg = std::max(G.degree, 1); T.walk = res_walk_env(G.n, T.deg, T.l);
       tasks.push_back(T);
       cost.push_back((double)G.n * G.dp * (1.0 + log((double)G.n) / log(std::max(2.0, G.degree - 1.0))));
     }
@@ -2257,6 +2340,23 @@ static void batch_build_tasks()
   if (sorted.size() > B.t_cap) { dfree(&B.d_t); dalloc(&B.d_t, sorted.size()); B.t_cap = sorted.size(); }
   B.ntasks = (int)sorted.size();
   HIPC(hipMemcpyAsync(B.d_t, sorted.data(), sizeof(ResTask) * sorted.size(), hipMemcpyHostToDevice, C.stream));
+  int banks = 0, group = 0;
+  balance_env(&banks, &group);
+  if (banks > 0) {
+    // neighbour lists ordered for the chosen block widths (bank classes id mod banks / wb)
+    std::vector<int> gstart(B.small.size() + 1, 0);
+    for (size_t k = 0; k < B.small.size(); k++) {
+      B.hg[k].pad = std::max(1, banks / B.wb[k]);
+      gstart[k + 1] = gstart[k] + (B.hg[k].n + group - 1) / group;
+    }
+    if (gstart.size() > B.gstart_cap) { dfree(&B.d_gstart); dalloc(&B.d_gstart, gstart.size()); B.gstart_cap = gstart.size(); }
+    HIPC(hipMemcpyAsync(B.d_g, B.hg.data(), sizeof(BatchGraph) * B.hg.size(), hipMemcpyHostToDevice, C.stream));
+    HIPC(hipMemcpyAsync(B.d_gstart, gstart.data(), sizeof(int) * gstart.size(), hipMemcpyHostToDevice, C.stream));
+    const int total = gstart.back();
+    hipLaunchKernelGGL(k_balance, dim3((total + 255) / 256), dim3(256), 0, C.stream, B.d_g, B.d_gstart,
+                       (int)B.small.size(), group, B.d_nbr);
+    HIPC(hipGetLastError());
+  }
   HIPC(hipStreamSynchronize(C.stream));
 }
 
@@ -2300,11 +2400,11 @@ extern "C" int ODPX_Batch_load(int ngraphs, const int* nodes, const int* degrees
       B.g_cap = B.small.size();
     }
     HIPC(hipMemcpyAsync(B.d_g, B.hg.data(), sizeof(BatchGraph) * B.hg.size(), hipMemcpyHostToDevice, C.stream));
-    batch_build_tasks();
     int pgrid = (int)std::min<long long>(std::max<long long>(1, (nbr_tot + 255) / 256), 8192);
     hipLaunchKernelGGL(k_prepare_batch, dim3(pgrid), dim3(256), 0, C.stream, B.d_g, (int)B.hg.size(), nbr_tot,
                        B.d_adj, B.d_nbr, B.d_err);
     HIPC(hipGetLastError());
+    batch_build_tasks();
   }
   HIPC(hipMemcpyAsync(B.h_err, B.d_err, sizeof(int), hipMemcpyDeviceToHost, C.stream));
   HIPC(hipStreamSynchronize(C.stream));
@@ -2489,9 +2589,9 @@ extern "C" void ODPX_Batch_release(void)
 {
   RandFence rand_fence;
   batch_drop_graph();
-  dfree(&B.d_adj); dfree(&B.d_nbr); dfree(&B.d_err); dfree(&B.d_qhead); hfree(&B.h_err);
+  dfree(&B.d_adj); dfree(&B.d_nbr); dfree(&B.d_err); dfree(&B.d_qhead); hfree(&B.h_err); dfree(&B.d_gstart);
   dfree(&B.d_g); dfree(&B.d_out); hfree(&B.h_out); dfree(&B.d_t);
-  B.adj_cap = B.nbr_cap = B.g_cap = B.t_cap = 0;
+  B.adj_cap = B.nbr_cap = B.g_cap = B.t_cap = B.gstart_cap = 0;
   B.ng = 0; B.small.clear(); B.large.clear();
   B.single.release();
 }
@@ -2520,7 +2620,7 @@ static unsigned char* g_scratch_dev[64];          // hipGetSymbolAddress of `scr
 // and error flag.
 __global__ void k_noalloc_prep(const int* __restrict__ adj, int n, int degree, int dp, int wb,
                                unsigned short* __restrict__ table, ResTask* __restrict__ tasks, int ntasks,
-                               int* qhead, u32* done, int* err)
+                               int* qhead, u32* done, int* err, int walk_cap)
 {
   const int64_t tid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x, stride = (int64_t)gridDim.x * blockDim.x;
   if (tid == 0) { *qhead = 0; *done = 0; }
@@ -2539,6 +2639,7 @@ __global__ void k_noalloc_prep(const int* __restrict__ adj, int n, int degree, i
     ResTask T;
     T.nbr = table; T.n = n; T.dp = dp; T.src0 = (int)t * b; T.l = min(b, n - (int)t * b);
     T.s = 1; T.graph = 0; T.wb = wb; T.dq = (max(degree, 1) + 3) / 4;
+    T.deg = max(degree, 1); T.walk = min(res_walk_levels(n, T.deg, T.l), walk_cap);
     tasks[t] = T;
   }
 }
@@ -2600,7 +2701,7 @@ extern "C" int ODPX_Set_aspl_noalloc(int nodes, int degree, const int* adjacency
   hipLaunchKernelGGL(k_noalloc_clear, dim3(1), dim3(64), 0, st, d_err);
   const int64_t work = std::max<int64_t>((int64_t)nodes * dp, ntasks);
   hipLaunchKernelGGL(k_noalloc_prep, dim3((int)std::min<int64_t>((work + 255) / 256, 1024)), dim3(256), 0, st,
-                     d_adj, nodes, degree, dp, wb, d_table, d_tasks, ntasks, d_qhead, d_done, d_err);
+                     d_adj, nodes, degree, dp, wb, d_table, d_tasks, ntasks, d_qhead, d_done, d_err, res_walk_cap());
   HIPC(hipGetLastError());
   ResFin fin{};
   fin.done = d_done; fin.part = d_parts; fin.host = d_res; fin.err = d_err;

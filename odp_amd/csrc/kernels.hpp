@@ -1236,7 +1236,28 @@ struct ResTask {
   int graph;                   // index into the per-graph outputs
   int wb;                      // 32-bit words per row of the block: 1, 2, 4 or 8
   int dq;                      // groups of 4 ids a row reads: ceil(table width / 4) <= dp / 4
+  int deg;                     // table width (entries a row holds, padding included)
+  int walk;                    // levels 0 .. walk - 1 from walks out of the block's sources (0: none)
 };
+
+// Levels a block builds from walks out of its l sources instead of pulling
+// every row's neighbours: level L (sources within L + 1 hops) adds the walks
+// of exactly L + 1 steps, l * deg^(L+1) LDS bit-sets with L + 1 dependent table
+// reads each, against n * (deg + 1) row reads for a pulled level; a level is
+// walked while its walks cost less than half a pulled level (counting a table
+// read as two row reads).  Returns the first pulled level (1: level 0 alone,
+// whose l * (deg + 1) walks of at most one step always beat reading the table).
+__host__ __device__ inline int res_walk_levels(int n, int deg, int l, double cost = 4.0)
+{
+  int K = 1;
+  double w = (double)l * deg;                        // walks of one step
+  for (int L = 1; L < 8; L++) {
+    w *= deg;                                        // walks of L + 1 steps
+    if (w * cost * (L + 1) > (double)n * (deg + 1)) break;
+    K = L + 1;
+  }
+  return K;
+}
 
 struct ResOut {     // per-graph accumulators
   u64 sum;          // sum over blocks of sum_L (n*l - count_L) * s   (the n(n-1) base excluded)
@@ -1342,6 +1363,30 @@ __device__ __forceinline__ ResOut res_run(const ResTask& T, u32* cur, u64 (*s_pa
   }
   // level 0 straight from the identity: row r holds r and its neighbours
   u64 pop = 0;
+  if (T.walk >= 1) {
+    // from the sources' side: clear every row, then each source sets its bit
+    // in its own row and in its neighbours' rows (the table is symmetric)
+    for (int r = tid; r < T.n; r += nth) {
+      RowT<WB> row;
+#pragma unroll
+      for (int k = 0; k < WB; k++) row.w[k] = 0;
+      lds_store<WB>(cur + r * WB, row);
+    }
+    __syncthreads();
+    const int per = T.deg + 1;
+    for (int it = tid; it < T.l * per; it += nth) {
+      const int c0 = it / per, j = it - c0 * per;
+      const int src = T.src0 + c0;
+      const int v = j == T.deg ? src : (int)T.nbr[src * T.dp + j];
+      atomicOr(&cur[v * WB + (c0 >> 5)], 1u << (c0 & 31));
+    }
+    __syncthreads();
+    for (int r = tid; r < T.n; r += nth) {
+      const RowT<WB> row = lds_row<WB>(cur + r * WB);
+#pragma unroll
+      for (int k = 0; k < WB; k++) pop += (u64)__popc(row.w[k]);
+    }
+  } else
   for (int r = tid; r < T.n; r += nth) {
     RowT<WB> row;
 #pragma unroll
@@ -1372,6 +1417,38 @@ __device__ __forceinline__ ResOut res_run(const ResTask& T, u32* cur, u64 (*s_pa
   u64 prev = 0, sum = 0;
   int L = 0, D = 0;
   bool disc = false;
+  // levels 1 .. T.walk - 1: the walks of exactly L + 1 steps from each source
+  // set its bit in the rows they end at (the rows already hold level L - 1,
+  // every shorter walk); the exits are left to the pulled levels' loop below
+  while (L + 1 < T.walk && c != target && !(L >= 1 && c == prev) && L + 1 < T.n) {
+    sum += (target - c) * (u64)T.s;
+    prev = c;
+    L++;
+    const u32 deg = (u32)T.deg, steps = (u32)L + 1;
+    u32 per = 1;
+    for (u32 i = 0; i < steps; i++) per *= deg;
+    const u32 items = (u32)T.l * per;
+    for (u32 it = (u32)tid; it < items; it += (u32)nth) {
+      const u32 c0 = it / per;
+      u32 rem = it - c0 * per;
+      int v = T.src0 + (int)c0;
+      for (u32 i = 0; i < steps; i++) {
+        const u32 j = rem % deg;
+        rem /= deg;
+        v = T.nbr[v * T.dp + (int)j];
+      }
+      atomicOr(&cur[v * WB + (int)(c0 >> 5)], 1u << (c0 & 31));
+    }
+    __syncthreads();
+    pop = 0;
+    for (int r = tid; r < T.n; r += nth) {
+      const RowT<WB> row = lds_row<WB>(cur + r * WB);
+#pragma unroll
+      for (int k = 0; k < WB; k++) pop += (u64)__popc(row.w[k]);
+    }
+    par ^= 1;
+    c = block_total(pop, s_part[par], wave, nw);
+  }
   for (;;) {
     if (c == target) { D = L + 1; break; }
     if ((L >= 1 && c == prev) || L + 1 >= T.n) { disc = true; break; }
@@ -1643,6 +1720,7 @@ struct AnnealArgs {
   ResOut* slots;        // [2][ntasks] block totals of a step (double buffered by step parity)
   u32* bar;             // grid barrier counter (zero at launch)
   int n, degree, dp, ntasks, steps;
+  int walk_cap;         // walked levels stop below this level (res_walk_levels decides below it)
 };
 
 #define ANNEAL_SPIN_LIMIT (1ull << 24)
@@ -1705,6 +1783,7 @@ __global__ __launch_bounds__(1024) void k_anneal(AnnealArgs a)
   ResTask T;
   T.nbr = tab; T.n = n; T.dp = dp; T.src0 = g * 32 * WB; T.l = min(32 * WB, n - T.src0);
   T.s = 1; T.graph = 0; T.wb = WB; T.dq = (max(deg, 1) + 3) / 4;
+  T.deg = deg; T.walk = min(res_walk_levels(n, deg, T.l), a.walk_cap);
   for (int k = 0; k < a.steps; k++) {                 // uniform: every exit below is uniform
     if (tid == 0) {
       int found = 0, ma = 0, mb = 0, mc = 0, md = 0;

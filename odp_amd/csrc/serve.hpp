@@ -193,6 +193,7 @@ static void sv_small_launch(SvSmallSlot& sl)
       ResTask T;
       T.nbr = c.d_nbr16; T.n = c.n; T.dp = c.dp16; T.src0 = t * b; T.l = std::min(b, c.n - t * b);
       T.s = 1; T.graph = k; T.wb = c.wb; T.dq = (std::max(c.degree, 1) + 3) / 4;
+      T.deg = std::max(c.degree, 1); T.walk = res_walk_env(c.n, T.deg, T.l);
       sl.tasks.push_back(T);
     }
   }
