@@ -1468,14 +1468,19 @@ __device__ __forceinline__ ResOut res_run(const ResTask& T, u32* cur, u64 (*s_pa
         if (!sat) {
           const uint4* ids = (const uint4*)T.nbr;
           const u32 off = (u32)r * (u32)(T.dp >> 3);
+          bool done = false;
           for (int j = 0; j < (T.dq >> 1); j++) {
             const uint4 v = ids[off + j];
             row_or<WB>(row, cur, v.x & 0xffff); row_or<WB>(row, cur, v.x >> 16);
             row_or<WB>(row, cur, v.y & 0xffff); row_or<WB>(row, cur, v.y >> 16);
             row_or<WB>(row, cur, v.z & 0xffff); row_or<WB>(row, cur, v.z >> 16);
             row_or<WB>(row, cur, v.w & 0xffff); row_or<WB>(row, cur, v.w >> 16);
+            done = true;                           // a row filled by 8 neighbours reads no more
+#pragma unroll
+            for (int k = 0; k < WB; k++) done = done && (row.w[k] == full.w[k]);
+            if (done) break;
           }
-          if (T.dq & 1) {
+          if ((T.dq & 1) && !done) {
             const uint2 v = ((const uint2*)T.nbr)[2 * off + (T.dq - 1)];
             row_or<WB>(row, cur, v.x & 0xffff); row_or<WB>(row, cur, v.x >> 16);
             row_or<WB>(row, cur, v.y & 0xffff); row_or<WB>(row, cur, v.y >> 16);
