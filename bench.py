@@ -578,7 +578,10 @@ def main():
     # is resident.  achieved = compulsory HBM bytes of its launches (each row of
     # a working block read once and written once, the ids once per launch) over
     # their duration; the operand-model rate is reported apart as work-equivalent.
-    kern = ("k_tile" if plan.get("tile") else "k_level") if st["level_launches"] > 0 else "k_resident_w"
+    if st["level_launches"] > 0:
+        kern = "k_tile" if plan.get("tile") else ("k_pgather" if plan.get("parts") else "k_level")
+    else:
+        kern = "k_resident_w"
     pre = "resident" if kern == "k_resident_w" else "level"
     nl = max(1, st[pre + "_launches"])
     launch_ms = st[pre + "_ms"] / nl
@@ -593,7 +596,7 @@ def main():
     traffic, measured = None, None
     if os.path.exists(args.traffic):
         tr = json.load(open(args.traffic)).get(args.workload)
-        if tr:
+        if tr and tr.get("kernel") == kern:         # counters of the kernel this run measured
             traffic = tr["bytes_per_launch"]
             measured = {"bytes_per_launch": traffic, "GBs_at_this_launch_time": round(traffic / (launch_ms * 1e-3) / 1e9, 1),
                         "source": tr.get("source"),
