@@ -24,12 +24,14 @@ typedef struct {
 
 /* ---- ASPL / diameter lifecycle: the hot path ----------------------------
  * One evaluator per process, as in the reference (aspl_cuda.cu:2-8).
- * The bit-matrix method is unchanged (aspl.c:35-119; aspl_cuda.cu:53-126);
- * results are bit-identical to the reference's serial library.
+ * The results are the reference's bit-matrix method's (aspl.c:35-119;
+ * aspl_cuda.cu:53-126), bit-identical to the reference's serial library.
  * Init may be called again after Finalize (the reference double-frees then,
  * aspl.c:295-296; this library resets all state at Finalize).
- * Environment, read at Init: ODP_ASPL=MATRIX|MATRIX_SAVING (utils.c:1238-1266;
- * BFS exits with an error on the GPU, utils.c:1256-1259), ODP_PROFILE=1
+ * Environment, read at Init: ODP_ASPL=MATRIX|MATRIX_SAVING|BFS (utils.c:1238-1266;
+ * BFS, which the reference runs on the CPU only (aspl.c:166-221) and whose GPU
+ * flavour refuses (utils.c:1256-1259), runs one breadth-first search per
+ * source on the GPU here, with the same results), ODP_PROFILE=1
  * (utils.c:142-150, printed at Finalize as utils.c:1268-1294 does). */
 
 /* include/odp.h:8 ; aspl_cuda.cu:159-162 */

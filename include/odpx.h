@@ -113,10 +113,10 @@ int  ODPX_Set_aspl_shard(int rank, int nranks, long long *excess, int *diameter,
 int  ODPX_Set_aspl_noalloc(int nodes, int degree, const int *adjacency, int *diameter, long *sum, double *ASPL);
 
 /* ---- algorithm kind (host only, no GPU) ---------------------------------- */
-/* The kind ODP_Init_aspl_* picks: ODP_ASPL if set (MATRIX, MATRIX_SAVING; BFS
- * and unknown values exit as at Init), else MATRIX when the GPU memory estimate
+/* The kind ODP_Init_aspl_* picks: ODP_ASPL if set (MATRIX, MATRIX_SAVING, BFS;
+ * unknown values exit as at Init), else MATRIX when the GPU memory estimate
  * of MATRIX is at most 2048 MB, MATRIX_SAVING otherwise (utils.c:1238-1266 with
- * is_cpu = false, procs = 1).  Returns 1 (MATRIX) or 2 (MATRIX_SAVING). */
+ * is_cpu = false, procs = 1).  Returns 1 (MATRIX), 2 (MATRIX_SAVING) or 3 (BFS). */
 int    ODPX_Get_kind(int nodes, int degree, int has_num_degrees, int symmetries, int grid_s);
 /* The memory estimate in MB of that rule and of the ODP_PROFILE report
  * (utils.c:1206-1236 with is_cpu = false, procs = 1). */

@@ -250,7 +250,7 @@ def combine_shards(nodes, partials):
 
 
 def get_kind(nodes, degree, has_num_degrees=False, symmetries=1, grid_s=False):
-    """The algorithm kind ODP_Init_aspl_* picks (1 MATRIX, 2 MATRIX_SAVING); host only."""
+    """The algorithm kind ODP_Init_aspl_* picks (1 MATRIX, 2 MATRIX_SAVING, 3 BFS); host only."""
     return load().ODPX_Get_kind(nodes, degree, int(bool(has_num_degrees)), symmetries, int(bool(grid_s)))
 
 

@@ -30,7 +30,7 @@
 #define HIPC(x) do { hipError_t e_ = (x); if (e_ != hipSuccess) \
   ODP_ERROR("libodp_hip: %s failed: %s (%s:%d)\n", #x, hipGetErrorString(e_), __FILE__, __LINE__); } while (0)
 
-static const int KIND_MATRIX = 1, KIND_SAVING = 2;
+static const int KIND_MATRIX = 1, KIND_SAVING = 2, KIND_BFS = 3;   // the reference's ASPL_* (common.h:39-41)
 static const int GPU_CHUNK_WORDS = 64;          // column-block width of MATRIX_SAVING (parameter.h:13)
 static const int SAVING_GROUP = 32;             // MATRIX_SAVING blocks resident at once (4 per XCD)
 static const int LEVEL_BLOCK_WORDS = 16;        // default column-block width of the level plan
@@ -300,6 +300,12 @@ struct Evaluator {
   size_t adj_cap = 0, nbr_cap = 0;
   int *d_adj = nullptr, *d_numdeg = nullptr, *d_itable = nullptr, *d_nbr = nullptr, *d_err = nullptr;
   unsigned short* d_nbr16 = nullptr;    // 16-bit row-neighbour table of the resident plan
+  bool bfs = false;                     // the BFS kind: evaluations run k_bfs
+  int* d_bfs_q = nullptr;               // [workgroups][2][n] frontier queues of k_bfs
+  u32* d_bfs_vis = nullptr;             // [workgroups][n/32] visited bitmaps when LDS cannot hold one
+  BfsOut *d_bfs_out = nullptr, *h_bfs_out = nullptr;
+  int* d_bfs_next = nullptr;            // next source to take
+  size_t bfs_q_cap = 0, bfs_vis_cap = 0;
   const int* ext_adj = nullptr;         // when set, the adjacency is read from this device copy
   int dp16 = 8;
   size_t nbr16_cap = 0;
@@ -421,6 +427,8 @@ struct Evaluator {
     dfree(&lr.plist); dfree(&lr.ppoff);
     dfree(&rr.tasks); dfree(&rr.out); dfree(&rr.qhead); dfree(&rr.done); hfree(&rr.host); rr.host_dev = nullptr;
     dfree(&rr.part); rr.part_cap = 0;
+    dfree(&d_bfs_q); dfree(&d_bfs_vis); dfree(&d_bfs_out); dfree(&d_bfs_next); hfree(&h_bfs_out);
+    bfs_q_cap = bfs_vis_cap = 0; bfs = false;
     lr = LevelRun(); rr = ResRun();
     adj_cap = nbr_cap = stage_cap = 0;
     guess_levels = 0;
@@ -1361,6 +1369,65 @@ struct Evaluator {
     else run_resident(excess, d, disc);
   }
 
+  // ---- the BFS kind ----
+  // the smallest level plan: one 1-word block resident, so the kind allocates
+  // no bit matrix (its evaluations never run the level plan)
+  ODPX_Plan bfs_plan() const
+  {
+    ODPX_Plan p{};
+    p.plan = ODPX_PLAN_LEVEL; p.kw = 1; p.upw = 4; p.group = 1;
+    return p;
+  }
+
+  // One breadth-first search per source row (k_bfs), over the 32-bit
+  // row-neighbour table; the sources are rows [0, n/s), one per symmetry orbit
+  // (circulant: vertices 0..n/s-1; grid: the base region's cells), so the
+  // ordered-pair distance sum is s times theirs.  Returns what run_raw returns:
+  // the excess over n(n-1), the diameter, whether some source missed a vertex.
+  void run_bfs(u64* excess, int* d, bool* disc)
+  {
+    flush_adj();
+    drop_graph();
+    table_fresh = false;                       // the resident plan's 16-bit table is not kept current
+    PrepArgs a{};
+    a.adj = ext_adj ? ext_adj : d_adj;
+    a.numdeg = d_numdeg; a.itable = d_itable; a.err = d_err;
+    a.nbr = d_nbr; a.nbr16 = nullptr;
+    a.n = n; a.s = s; a.degree = degree; a.dpad = dpad;
+    a.width = width; a.height = height; a.grid_s = grid_s;
+    const int64_t total = (int64_t)n * dpad;
+    hipLaunchKernelGGL(k_prepare, dim3((int)std::min<int64_t>(std::max<int64_t>(1, (total + 255) / 256), 4096)),
+                       dim3(256), 0, st, a);
+    HIPC(hipGetLastError());
+    const size_t words = ((size_t)n + 31) / 32;
+    const bool vis_lds = words * 4 <= LDS_DYN_MAX;
+    // workgroups: up to 8 per CU, fewer when the queues (and bitmaps) of that
+    // many would pass 8 GiB
+    const size_t per_wg = 8ull * n + (vis_lds ? 0 : words * 4);
+    int G = (int)std::min<size_t>({(size_t)nsrc, (size_t)C.ncu * 8, std::max<size_t>(1, (8ull << 30) / per_wg)});
+    if ((size_t)G * 2 * n > bfs_q_cap) { dfree(&d_bfs_q); dalloc(&d_bfs_q, (size_t)G * 2 * n); bfs_q_cap = (size_t)G * 2 * n; }
+    if (!vis_lds && (size_t)G * words > bfs_vis_cap) {
+      dfree(&d_bfs_vis); dalloc(&d_bfs_vis, (size_t)G * words); bfs_vis_cap = (size_t)G * words;
+    }
+    if (!d_bfs_out) { dalloc(&d_bfs_out, 1); dalloc(&d_bfs_next, 1); halloc(&h_bfs_out, 1); }
+    hipLaunchKernelGGL(k_bfs_clear, dim3(1), dim3(64), 0, st, d_bfs_out, d_bfs_next);
+    if (vis_lds)
+      hipLaunchKernelGGL(k_bfs<true>, dim3(G), dim3(256), words * 4, st, d_nbr, n, dpad, nsrc, d_bfs_q,
+                         (u32*)nullptr, d_bfs_next, d_bfs_out);
+    else
+      hipLaunchKernelGGL(k_bfs<false>, dim3(G), dim3(256), 0, st, d_nbr, n, dpad, nsrc, d_bfs_q, d_bfs_vis,
+                         d_bfs_next, d_bfs_out);
+    HIPC(hipGetLastError());
+    HIPC(hipMemcpyAsync(h_bfs_out, d_bfs_out, sizeof(BfsOut), hipMemcpyDeviceToHost, st));
+    HIPC(hipMemcpyAsync(h_err, d_err, sizeof(int), hipMemcpyDeviceToHost, st));
+    HIPC(hipStreamSynchronize(st));
+    check_err();
+    const BfsOut o = *h_bfs_out;
+    *disc = o.disc != 0;
+    *d = std::max(1, o.ecc);
+    *excess = o.sum * (u64)s - (u64)n * (u64)(n - 1);
+  }
+
   // full evaluation of the adjacency on the device, reference arithmetic
   void evaluate(int* diameter, long* sum, double* ASPL)
   {
@@ -1368,7 +1435,8 @@ struct Evaluator {
     u64 excess = 0;
     int d = 1;
     bool disc = false;
-    run_raw(&excess, &d, &disc);
+    if (bfs) run_bfs(&excess, &d, &disc);
+    else run_raw(&excess, &d, &disc);
     result_of(excess, d, disc, diameter, sum, ASPL);
     if (timed()) { g_stats.evals++; g_stats.eval_ms += 1e3 * (wall_time() - t0); }
   }
@@ -1381,8 +1449,9 @@ static double ref_mem_mb(int kind, int nodes, int degree, int symmetries, bool n
 {
   double ab;
   if (kind == KIND_MATRIX) ab = nodes * ((double)nodes / (4.0 * symmetries));
-  else ab = 16.0 * nodes * GPU_CHUNK_WORDS;
-  if (grid_s) ab += 4.0 * nodes;
+  else if (kind == KIND_SAVING) ab = 16.0 * nodes * GPU_CHUNK_WORDS;
+  else ab = nodes * (3.0 * sizeof(int) + 1.0);                // the BFS kind (utils.c:1220-1224)
+  if (grid_s && kind != KIND_BFS) ab += 4.0 * nodes;
   double res = 8.0 * 448, adj = 4.0 * (nodes / symmetries) * degree, deg = numdeg ? 4.0 * nodes : 0;
   return (ab + res + adj + deg) / (1024.0 * 1024.0);
 }
@@ -1394,7 +1463,7 @@ static int ref_kind(int nodes, int degree, int symmetries, bool numdeg, bool gri
                    ? KIND_MATRIX : KIND_SAVING;
   if (strcmp(val, "MATRIX") == 0) return KIND_MATRIX;
   if (strcmp(val, "MATRIX_SAVING") == 0) return KIND_SAVING;
-  if (strcmp(val, "BFS") == 0) ODP_ERROR("BFS for HIP is not implemented.\n");
+  if (strcmp(val, "BFS") == 0) return KIND_BFS;   // the reference's GPU flavour refuses it (utils.c:1256-1259)
   ODP_ERROR("Unknown ASPL value (%s)\n", val);
   return 0;
 }
@@ -1493,7 +1562,8 @@ static void lifecycle_init(int family, int nodes, int degree, const int* numdeg_
   E.kind = ref_kind(nodes, degree, s, numdeg_full != nullptr, grid_s);
   E.ev.shape(family, nodes, degree, s, width, height, numdeg_full, grid_s ? 1 : 0);
   if (grid_s) E.ev.set_itable(itable);
-  E.ev.apply_plan(E.ev.default_plan(E.kind));
+  E.ev.bfs = E.kind == KIND_BFS;
+  E.ev.apply_plan(E.kind == KIND_BFS ? E.ev.bfs_plan() : E.ev.default_plan(E.kind));
   E.profile = profile_env();
   E.elapsed = 0;
   E.times = 0;
@@ -1608,7 +1678,7 @@ extern "C" void ODP_Finalize_aspl(void)
     printf("Number of Times = %u\n", E.times);
     printf("Total Time      = %f sec.\n", E.elapsed);
     printf("Average Time    = %f sec.\n", E.elapsed / E.times);   // as utils.c:1284 (nan when never called)
-    printf("Algorithm       = %s (HIP)\n", E.kind == KIND_MATRIX ? "MATRIX" : "MATRIX_SAVING");
+    printf("Algorithm       = %s (HIP)\n", E.kind == KIND_MATRIX ? "MATRIX" : E.kind == KIND_SAVING ? "MATRIX_SAVING" : "BFS");
     printf("Symmetries      = %d\n", E.ev.s);
     printf("Memory Usage    = %.3f MB\n", mem);
     printf("Num of Procs    = 1\n");
@@ -1725,6 +1795,7 @@ extern "C" int ODPX_Set_aspl_shard(int rank, int nranks, long long* excess, int*
   RandFence rand_fence;
   need_active("ODPX_Set_aspl_shard");
   if (nranks < 1 || rank < 0 || rank >= nranks || !excess || !diameter || !disconnected) return -1;
+  if (E.kind == KIND_BFS) return -1;               // the MPI flavours divide bit-matrix columns only
   Evaluator& v = E.ev;
   const int nsrc = v.nsrc, W = (nsrc + 63) / 64;
   // MATRIX: ceil(W / P)-word blocks, rank r the block r (aspl_mpi_cuda.cu:47-54);

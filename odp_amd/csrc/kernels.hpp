@@ -734,6 +734,80 @@ __global__ __launch_bounds__(PG_WG) void k_pgather(const LevelTask* __restrict__
 }
 
 // ---------------------------------------------------------------------------
+// The BFS kind (the reference's CPU-only aspl_bfs, aspl.c:166-221, with the
+// top-down step of utils.c:1999-2100): one breadth-first search per source
+// over the row-neighbour table.  Workgroups are persistent and take sources
+// from a counter; each keeps its visited bitmap in LDS (VIS_LDS) or in its own
+// slice of device memory, and its two frontier queues (n ids each) in device
+// memory.  A level's frontier entries (vertex, table position) are dealt over
+// the threads; a test-and-set on the bitmap admits each new vertex once.  Per
+// source the distances' sum, the eccentricity and the number reached are
+// folded into per-workgroup totals, added into the result once at the end.
+struct BfsOut {
+  u64 sum;          // sum over the sources of sum_v dist(source, v)
+  int ecc;          // largest eccentricity
+  int disc;         // a source did not reach every vertex
+};
+
+__global__ void k_bfs_clear(BfsOut* out, int* next_src)
+{
+  if (threadIdx.x == 0) { out->sum = 0; out->ecc = 0; out->disc = 0; *next_src = 0; }
+}
+
+template <bool VIS_LDS>
+__global__ __launch_bounds__(256) void k_bfs(const int* __restrict__ nbr, int n, int dpad, int nsrc,
+                                             int* __restrict__ queues, u32* __restrict__ gvis,
+                                             int* __restrict__ next_src, BfsOut* __restrict__ out)
+{
+  extern __shared__ u32 dyn_vis[];
+  __shared__ int s_src, s_cnt;
+  const int tid = threadIdx.x, nth = blockDim.x;
+  const int words = (n + 31) >> 5;
+  u32* vis = VIS_LDS ? dyn_vis : gvis + (size_t)blockIdx.x * words;
+  int* fq = queues + (size_t)blockIdx.x * 2 * n;
+  int* nq = fq + n;
+  u64 tot = 0;
+  int ecc_max = 0, disc = 0;
+  for (;;) {                                            // uniform: s_src is read after a barrier
+    if (tid == 0) s_src = atomicAdd(next_src, 1);
+    __syncthreads();
+    const int src = __builtin_amdgcn_readfirstlane(s_src);
+    if (src >= nsrc) break;
+    for (int i = tid; i < words; i += nth) vis[i] = 0;
+    __syncthreads();
+    if (tid == 0) { vis[src >> 5] = 1u << (src & 31); fq[0] = src; }
+    int fcount = 1, level = 0, reached = 1;
+    u64 dsum = 0;
+    while (fcount > 0) {                                // uniform
+      level++;
+      if (tid == 0) s_cnt = 0;
+      __syncthreads();                                  // the frontier, the bitmap and s_cnt are set
+      const int entries = fcount * dpad;
+      for (int e = tid; e < entries; e += nth) {
+        const int f = e / dpad, j = e - f * dpad;
+        const int u = nbr[(size_t)fq[f] * dpad + j];
+        const u32 b = 1u << (u & 31);
+        if (!(atomicOr(&vis[u >> 5], b) & b)) nq[atomicAdd(&s_cnt, 1)] = u;
+      }
+      __syncthreads();                                  // every new vertex is queued and counted
+      fcount = __builtin_amdgcn_readfirstlane(s_cnt);
+      dsum += (u64)level * (u64)fcount;
+      reached += fcount;
+      int* t = fq; fq = nq; nq = t;
+      __syncthreads();                                  // s_cnt is read before it is reset
+    }
+    tot += dsum;
+    ecc_max = max(ecc_max, level - 1);
+    if (reached < n) disc = 1;
+  }
+  if (tid == 0) {
+    if (tot) atomicAdd(&out->sum, tot);
+    atomicMax(&out->ecc, ecc_max);
+    if (disc) atomicOr(&out->disc, 1);
+  }
+}
+
+// ---------------------------------------------------------------------------
 // k_sparse: the count of a level expected to be the last one (LV_SPARSE), from
 // the list of chunks the level before it left unsaturated.  A chunk saturated
 // at level L-1 keeps its bits at level L, so the count of level L is that of

@@ -168,9 +168,13 @@ def test_error_nodes_not_divisible():
     assert r.returncode == 1 and "must be divisible by symmetries" in r.stderr
 
 
-def test_error_bfs_on_gpu_build():
-    r = _run_expect_exit("o.init_aspl_general(10, 3)", env={"ODP_ASPL": "BFS"})
-    assert r.returncode == 1 and "BFS for HIP is not implemented" in r.stderr
+def test_bfs_kind_accepted(monkeypatch):
+    """ODP_ASPL=BFS selects the BFS kind (3, common.h:41) instead of exiting as
+    the reference's GPU flavour does (utils.c:1256-1259); its memory estimate is
+    the reference's BFS formula (utils.c:1220-1224, no OpenMP term)."""
+    monkeypatch.setenv("ODP_ASPL", "BFS")
+    assert odp.get_kind(1000, 4) == 3
+    assert odp.get_mem_usage(3, 1000, 4) == pytest.approx((1000 * 13 + 8 * 448 + 4 * 1000 * 4) / 2**20)
 
 
 def test_error_unknown_kind():
