@@ -67,7 +67,8 @@ int  ODPX_Get_plan(ODPX_Plan *plan);
 int  ODPX_Set_plan(const ODPX_Plan *plan);
 /* Times every candidate plan on `adjacency` (same layout as ODP_Set_aspl),
  * checks that all of them return identical results (exit(1) otherwise), and
- * keeps the fastest.  Returns the number of candidates tried. */
+ * keeps the fastest.  Returns the number of candidates tried, or -1 under the
+ * BFS kind (which runs no plan). */
 int  ODPX_Autotune(void *adjacency, int repeats);
 
 /* ---- device-resident evaluation ---------------------------------------- */

@@ -2151,6 +2151,7 @@ extern "C" int ODPX_Autotune(void* adjacency, int repeats)
 {
   RandFence rand_fence;
   need_active("ODPX_Autotune");
+  if (E.kind == KIND_BFS) return -1;               // the BFS kind runs no plan
   Evaluator& v = E.ev;
   std::vector<ODPX_Plan> cand;
   for (int wb = 1; wb <= RES_WB_MAX; wb <<= 1)

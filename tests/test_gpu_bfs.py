@@ -144,6 +144,7 @@ def test_device_swaps_and_profile(capfd, monkeypatch):
             assert same(odp.set_aspl_resident(), oracle.evaluate(cur, nodes=n, kind=oracle.BFS))
         with pytest.raises(ValueError):
             odp.set_aspl_shard(0, 2)                # the MPI flavours divide bit-matrix columns only
+        assert odp.autotune(cur) == -1              # the kind runs no plan
     finally:
         odp.finalize_aspl()
     out = capfd.readouterr().out
