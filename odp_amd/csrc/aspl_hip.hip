@@ -128,6 +128,10 @@ static void ctx_init()
   HIPC(hipFuncSetAttribute((const void*)k_resident_w<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_DYN_MAX));
   HIPC(hipFuncSetAttribute((const void*)k_resident_w<4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_DYN_MAX));
   HIPC(hipFuncSetAttribute((const void*)k_resident_w<8>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_DYN_MAX));
+  HIPC(hipFuncSetAttribute((const void*)k_resident_w<1, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_DYN_MAX));
+  HIPC(hipFuncSetAttribute((const void*)k_resident_w<2, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_DYN_MAX));
+  HIPC(hipFuncSetAttribute((const void*)k_resident_w<4, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_DYN_MAX));
+  HIPC(hipFuncSetAttribute((const void*)k_resident_w<8, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_DYN_MAX));
   HIPC(hipFuncSetAttribute((const void*)k_anneal<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_DYN_MAX));
   HIPC(hipFuncSetAttribute((const void*)k_anneal<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_DYN_MAX));
   HIPC(hipFuncSetAttribute((const void*)k_anneal<4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_DYN_MAX));
@@ -2499,7 +2503,8 @@ static void batch_enqueue()
     int per_cu = (int)std::max<size_t>(1, LDS_LIMIT / (B.grp_lds[g] + LDS_STATIC));
     per_cu = std::min(per_cu, B.per_cu);
     const int grid = std::max(1, std::min(B.grp_n[g], per_cu * C.ncu));
-    auto kern = g == 0 ? k_resident_w<8> : g == 1 ? k_resident_w<4> : g == 2 ? k_resident_w<2> : k_resident_w<1>;
+    auto kern = g == 0 ? k_resident_w<8, true> : g == 1 ? k_resident_w<4, true> : g == 2 ? k_resident_w<2, true>
+                                                                                       : k_resident_w<1, true>;
     hipLaunchKernelGGL(kern, dim3(grid), dim3(1024), B.grp_lds[g], C.stream, B.d_t + B.grp_start[g], B.grp_n[g],
                        B.d_qhead + g, B.d_out, ResFin{});
     HIPC(hipGetLastError());
