@@ -1925,7 +1925,6 @@ static void anneal_on_device(Evaluator& v, int* adj_out, const std::vector<int>&
     AnnealArgs aa;
     aa.adj = v.d_adj; aa.best = d_best; aa.st = d_st; aa.slots = d_slots; aa.bar = d_bar;
     aa.n = n; aa.degree = deg; aa.dp = v.dp16; aa.ntasks = G;
-    aa.walk_cap = res_walk_cap();
     aa.steps = (int)std::min<long long>(chunk, ncalcs - S.steps);
     hipLaunchKernelGGL(k_clear, dim3(1), dim3(64), 0, C.stream, (u64*)nullptr, 0, (int*)d_bar, 1);
     void* args[] = {&aa};

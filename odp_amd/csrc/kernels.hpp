@@ -1453,7 +1453,7 @@ __device__ __forceinline__ u64 block_total(u64 v, u64* part, int wave, int nw)
 
 // All levels of one block of T.l <= 32*WB sources; returns the block's totals
 // (the same value in every thread).
-template <int WB, bool STOP = false>
+template <int WB, bool STOP = false, bool WALK = true>
 __device__ __forceinline__ ResOut res_run(const ResTask& T, u32* cur, u64 (*s_part)[16],
                                           int tid, int nth, int lane, int wave, int nw)
 {
@@ -1467,7 +1467,7 @@ __device__ __forceinline__ ResOut res_run(const ResTask& T, u32* cur, u64 (*s_pa
   }
   // level 0 straight from the identity: row r holds r and its neighbours
   u64 pop = 0;
-  if (T.walk >= 1) {
+  if (WALK && T.walk >= 1) {
     // from the sources' side: clear every row, then each source sets its bit
     // in its own row and in its neighbours' rows (the table is symmetric)
     for (int r = tid; r < T.n; r += nth) {
@@ -1524,7 +1524,7 @@ __device__ __forceinline__ ResOut res_run(const ResTask& T, u32* cur, u64 (*s_pa
   // levels 1 .. T.walk - 1: the walks of exactly L + 1 steps from each source
   // set its bit in the rows they end at (the rows already hold level L - 1,
   // every shorter walk); the exits are left to the pulled levels' loop below
-  while (L + 1 < T.walk && c != target && !(L >= 1 && c == prev) && L + 1 < T.n) {
+  while (WALK && L + 1 < T.walk && c != target && !(L >= 1 && c == prev) && L + 1 < T.n) {
     sum += (target - c) * (u64)T.s;
     prev = c;
     L++;
@@ -1833,7 +1833,6 @@ struct AnnealArgs {
   ResOut* slots;        // [2][ntasks] block totals of a step (double buffered by step parity)
   u32* bar;             // grid barrier counter (zero at launch)
   int n, degree, dp, ntasks, steps;
-  int walk_cap;         // walked levels stop below this level (res_walk_levels decides below it)
 };
 
 #define ANNEAL_SPIN_LIMIT (1ull << 24)
@@ -1896,7 +1895,7 @@ __global__ __launch_bounds__(1024) void k_anneal(AnnealArgs a)
   ResTask T;
   T.nbr = tab; T.n = n; T.dp = dp; T.src0 = g * 32 * WB; T.l = min(32 * WB, n - T.src0);
   T.s = 1; T.graph = 0; T.wb = WB; T.dq = (max(deg, 1) + 3) / 4;
-  T.deg = deg; T.walk = min(res_walk_levels(n, deg, T.l), a.walk_cap);
+  T.deg = deg; T.walk = 0;                            // res_run below is built without walks
   for (int k = 0; k < a.steps; k++) {                 // uniform: every exit below is uniform
     if (tid == 0) {
       int found = 0, ma = 0, mb = 0, mc = 0, md = 0;
@@ -1916,7 +1915,7 @@ __global__ __launch_bounds__(1024) void k_anneal(AnnealArgs a)
       if (tid == 0) S.stop = 1;
       break;
     }
-    const ResOut b = res_run<WB>(T, bits, s_part, tid, nth, lane, wave, nw);
+    const ResOut b = res_run<WB, false, false>(T, bits, s_part, tid, nth, lane, wave, nw);   // no walks (measured faster)
     const int par = k & 1;                             // slots of this step (k is uniform; S.steps is thread 0's)
     if (tid == 0) {
       a.slots[par * G + g] = b;
