@@ -79,7 +79,10 @@ void ODPX_Upload_adjacency(void *adjacency);
 void ODPX_Set_aspl_resident(int *diameter, long *sum, double *ASPL);
 /* Applies edge swap a-b, c-d -> a-c, b-d to the device copy (s == 1 only),
  * all or nothing: 0 on success; -1, with the graph unchanged, if an id is out of
- * range, the four ids are not distinct, or edge a-b or c-d is missing. */
+ * range, the four ids are not distinct, or edge a-b or c-d is missing.  After
+ * an upload the library keeps a host copy of the device adjacency, checks the
+ * swap on it and returns without waiting for the device; otherwise (after
+ * device-only changes such as ODPX_Anneal) it reads the device's verdict. */
 int  ODPX_Swap_device(int a, int b, int c, int d);
 
 /* ---- one process's share (the reference's MPI+CUDA division) ------------- */

@@ -334,6 +334,12 @@ struct Evaluator {
   // h_stage holds what the device copy holds (or will, once the pending entries are stored):
   // false after moves made on the device only
   bool stage_is_device = false;
+  // h_mirror holds what the device copy holds once the enqueued work is done:
+  // taken from h_stage by the first device swap after an upload and kept by
+  // the swaps that follow, so they are checked on the host and launched
+  // without waiting; anything else that changes the device copy clears it
+  std::vector<int> h_mirror;
+  bool mirror_ok = false;
   // the pending table entries as launch arguments (taken: the list is emptied)
   ResPatch make_patch()
   {
@@ -421,11 +427,13 @@ struct Evaluator {
     drop_graph();
     table_fresh = false;
     stage_is_device = false;
+    mirror_ok = false;
     pend_patch.clear();
     dfree(&d_adj); dfree(&d_numdeg); dfree(&d_itable); dfree(&d_nbr); dfree(&d_err); dfree(&d_nbr16);
     dfree(&d_xy2row); dfree(&d_row2xy); dfree(&d_toff);
     nbr16_cap = 0; toff_cap = 0; geo_lmax = -1;
     hfree(&h_stage); hfree(&h_err); h_stage_dev = nullptr; adj_on_host = false;
+    std::vector<int>().swap(h_mirror);
     dfree(&lr.bufs); dfree(&lr.cnt); dfree(&lr.done); dfree(&lr.tasks); dfree(&lr.wg_start); hfree(&lr.h_cnt);
     dfree(&lr.flags); hfree(&lr.h_acnt); dfree(&lr.wgm); dfree(&lr.list); dfree(&lr.list_n); dfree(&lr.tq);
     dfree(&lr.plist); dfree(&lr.ppoff);
@@ -446,6 +454,7 @@ struct Evaluator {
     drop_graph();
     table_fresh = false;
     stage_is_device = false;
+    mirror_ok = false;
     pend_patch.clear();
     family = fam; n = nodes; degree = deg; s = sym; width = w; height = h; grid_s = gs;
     nsrc = n / s;
@@ -548,6 +557,7 @@ struct Evaluator {
           idx[k++] = (int)i;
         }
     }
+    mirror_ok = false;
     for (int q = 0; q < k; q++) {
       const int i = idx[q];
       h_stage[i] = a[i];
@@ -564,6 +574,7 @@ struct Evaluator {
     if (upload_as_patch((const int*)adjacency)) return;
     table_fresh = false;
     stage_is_device = true;
+    mirror_ok = false;
     memcpy(h_stage, adjacency, sizeof(int) * adj_elems());
     if (plan.plan == ODPX_PLAN_RESIDENT && !ext_adj && sizeof(int) * adj_elems() <= ZC_MAX_BYTES) {
       adj_on_host = true;          // the next table build reads it from host memory
@@ -1770,17 +1781,40 @@ extern "C" int ODPX_Swap_device(int a, int b, int c, int d)
   for (int x : ids) if (x < 0 || x >= v.n) return -1;
   if (a == b || a == c || a == d || b == c || b == d || c == d) return -1;   // not a 2-opt move
   v.flush_adj();
-  v.stage_is_device = false;         // the device copy moves on its own
-  hipLaunchKernelGGL(k_swap, dim3(1), dim3(64), 0, C.stream, v.d_adj, v.degree, a, b, c, d, v.d_err, v.swap_table(),
-                     v.dp16);
-  HIPC(hipGetLastError());
-  HIPC(hipMemcpyAsync(v.h_err, v.d_err, sizeof(int), hipMemcpyDeviceToHost, C.stream));
-  HIPC(hipStreamSynchronize(C.stream));
-  if (*v.h_err & 4) {
-    *v.h_err = 0;
-    HIPC(hipMemsetAsync(v.d_err, 0, sizeof(int), C.stream));
+  if (!v.mirror_ok && v.stage_is_device && !v.ext_adj) {   // h_stage holds the device copy
+    v.h_mirror.assign(v.h_stage, v.h_stage + v.adj_elems());
+    v.mirror_ok = true;
+  }
+  if (v.mirror_ok) {
+    // checked on the host copy with k_swap's rule (the first entry of each row
+    // equal to the old neighbour), applied to it, and launched without waiting
+    const int rows[4] = {a, b, c, d}, from[4] = {b, a, d, c}, to[4] = {c, d, a, b};
+    int pos[4];
+    for (int e = 0; e < 4; e++) {
+      const int* r = v.h_mirror.data() + (size_t)rows[e] * v.degree;
+      pos[e] = -1;
+      for (int j = 0; j < v.degree; j++)
+        if (r[j] == from[e]) { pos[e] = j; break; }
+      if (pos[e] < 0) return -1;                   // nothing changed
+    }
+    for (int e = 0; e < 4; e++) v.h_mirror[(size_t)rows[e] * v.degree + pos[e]] = to[e];
+    v.stage_is_device = false;
+    hipLaunchKernelGGL(k_swap, dim3(1), dim3(64), 0, C.stream, v.d_adj, v.degree, a, b, c, d, v.d_err,
+                       v.swap_table(), v.dp16);
+    HIPC(hipGetLastError());
+  } else {
+    v.stage_is_device = false;         // the device copy moves on its own
+    hipLaunchKernelGGL(k_swap, dim3(1), dim3(64), 0, C.stream, v.d_adj, v.degree, a, b, c, d, v.d_err,
+                       v.swap_table(), v.dp16);
+    HIPC(hipGetLastError());
+    HIPC(hipMemcpyAsync(v.h_err, v.d_err, sizeof(int), hipMemcpyDeviceToHost, C.stream));
     HIPC(hipStreamSynchronize(C.stream));
-    return -1;
+    if (*v.h_err & 4) {
+      *v.h_err = 0;
+      HIPC(hipMemsetAsync(v.d_err, 0, sizeof(int), C.stream));
+      HIPC(hipStreamSynchronize(C.stream));
+      return -1;
+    }
   }
   if (v.geo_lmax >= 0) {
     // a grid's new edges a-c and b-d may be longer than any before (k_tile's halo)
@@ -1943,6 +1977,7 @@ static void anneal_on_device(Evaluator& v, int* adj_out, const std::vector<int>&
   dfree(&d_st); dfree(&d_slots); dfree(&d_bar); dfree(&d_best);
   // the device copy now holds the chain's current graph; the 16-bit table is rebuilt from it
   v.stage_is_device = false;
+  v.mirror_ok = false;
   v.table_fresh = false;
   steps = S.steps; accepted = S.accepted;
   cur_d = S.cur_d; cur_a = S.cur_a;
@@ -2067,6 +2102,7 @@ extern "C" int ODPX_Anneal(void* adjacency, long long ncalcs, double max_temp, d
   double temp = max_temp;
   v.flush_adj();
   v.stage_is_device = false;         // the moves change the device copy only
+  v.mirror_ok = false;
   // Under the resident plan (its table current) a move is not a launch of its
   // own: its four table entries ride along with the next evaluation's launch
   // (the host copy gives their positions).  Otherwise k_swap applies it.
