@@ -1201,6 +1201,7 @@ struct DirectArgs {
   unsigned qmul;         // floor(2^32 / q) + 1 for q = 16-byte loads per table row > 1:
                          // idx / q == umulhi(idx, qmul) for idx < 2^32 / q (idx < 2^24, q <= 256 here)
   int pw[8];             // dpad^i
+  int nt;                // the rows of level K are streamed past L2 (the next level gathers from other XCDs)
 };
 
 #define DIRECT_KMAX 6
@@ -1301,8 +1302,15 @@ __global__ __launch_bounds__(512) void k_direct(DirectArgs a)
           const u64x2 x = *(const u64x2*)(bm + w);
           const u32 c = (u32)(__popcll(x.x) + __popcll(x.y));
           if (c) atomicAdd(&lcd[64 * j], c);
-          if (d == a.K)
-            *(u64x2*)(a.out + (int64_t)(w >> a.kwl) * a.tstride + (int64_t)v * a.kw + (w & (a.kw - 1))) = x;
+          if (d == a.K) {
+            u64* dst = a.out + (int64_t)(w >> a.kwl) * a.tstride + (int64_t)v * a.kw + (w & (a.kw - 1));
+            if (a.nt) {
+              u64v2 o; o.x = x.x; o.y = x.y;
+              __builtin_nontemporal_store(o, (u64v2*)dst);
+            } else {
+              *(u64x2*)dst = x;
+            }
+          }
         }
       }
       __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
