@@ -53,7 +53,7 @@ def plan_fits(plan, n):
     if plan is None or plan["plan"] != RES:
         return True
     wb, th = plan["wb"], plan["threads"]
-    rpt = 16 if wb == 1 else (2 if wb == 8 else 32 // wb)
+    rpt = {1: 16, 2: 16, 4: 10, 8: 2}[wb]
     return n <= 65535 and n * wb * 4 <= LDS_DYN_MAX and n <= rpt * th
 
 
