@@ -20,6 +20,7 @@
 #include <unistd.h>
 #include <sys/time.h>
 #include <vector>
+#include <atomic>
 #include <algorithm>
 #include <mutex>
 #include "odp.h"
@@ -50,6 +51,7 @@ static const double MEM_THRESHOLD_MB = 2048.0;  // MATRIX -> MATRIX_SAVING switc
 static const size_t LDS_LIMIT = 163840;         // LDS a gfx950 workgroup may allocate
 static const size_t LDS_STATIC = 512;           // k_resident's own __shared__ data, rounded up
 static const size_t LDS_DYN_MAX = LDS_LIMIT - LDS_STATIC;
+static const double RES_SPIN_S = 200e-6;         // a resident evaluation's wait watches its result this long
 static const int RING = 260;                    // per-level counter slots kept on the device
 static const int MAX_BATCH_LEVELS = 256;        // levels launched between two host reads (< RING - 2)
 static const int PG_PARTS = 4;                  // row parts of the XCD-phased gather (2 MiB of a 65536-row slab)
@@ -292,6 +294,7 @@ struct ResRun {
   size_t part_cap = 0;
   ResHost* host = nullptr;              // pinned coherent host memory the launch writes its result to
   ResHost* host_dev = nullptr;          // its device address
+  u32 seq = 0;                          // sequence number of the last launch
 };
 
 struct Evaluator {
@@ -424,6 +427,7 @@ struct Evaluator {
 
   void release()
   {
+    if (st) HIPC(hipStreamSynchronize(st));    // a resident launch may still be finishing
     drop_graph();
     table_fresh = false;
     stage_is_device = false;
@@ -872,6 +876,7 @@ struct Evaluator {
       // launches (their totals go through the per-workgroup slots)
       dalloc(&r.out, 1); dalloc(&r.qhead, 1); dalloc(&r.done, 1);
       halloc_coherent(&r.host, &r.host_dev, 1);
+      r.host->seq = r.seq;             // no launch has written it yet
       HIPC(hipMemsetAsync(r.out, 0, sizeof(ResOut), st));
       HIPC(hipMemsetAsync(r.qhead, 0, sizeof(int), st));
       HIPC(hipMemsetAsync(r.done, 0, sizeof(u32), st));
@@ -1333,6 +1338,7 @@ struct Evaluator {
       ResFin fin;
       fin.done = r.done; fin.part = r.part; fin.host = r.host_dev; fin.err = d_err;
       fin.patch = make_patch();
+      fin.seq = ++r.seq;
       hipLaunchKernelGGL(kern, dim3(r.grid), dim3(r.threads), r.lds, st, r.tasks, r.ntasks, r.qhead, r.out, fin);
       HIPC(hipGetLastError());
       timed_end(tl);
@@ -1341,9 +1347,22 @@ struct Evaluator {
     // longer (43.5 against 39.3 us per evaluation of n=1024, d=6)
     enqueue();
     trace("resident launched");
-    HIPC(hipStreamSynchronize(st));
+    // the kernel stores its sequence number after the result: watch for it a
+    // while (cheaper than a stream synchronisation for small graphs), else, and
+    // for timed launches, synchronize
+    bool seen = false;
+    if (!timed()) {
+      const volatile u32* sq = &r.host->seq;
+      const double t0 = wall_time();
+      for (int i = 0; !seen; i++) {
+        seen = *sq == r.seq;
+        if (!seen && (i & 63) == 63 && wall_time() - t0 > RES_SPIN_S) break;
+      }
+      if (seen) std::atomic_thread_fence(std::memory_order_acquire);
+    }
+    if (!seen) HIPC(hipStreamSynchronize(st));
     trace("resident synchronized");
-    const ResHost res = *r.host;      // written by the launch; the stream is synchronized
+    const ResHost res = *r.host;      // written by the launch (its sequence number seen, or the stream synchronized)
     *h_err = res.err;
     check_err();
     *excess = res.out.sum;

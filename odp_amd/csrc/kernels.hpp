@@ -1638,7 +1638,7 @@ __device__ __forceinline__ void res_commit(ResOut* out, int graph, const ResOut&
 struct ResHost {
   ResOut out;
   int err;          // the device error flag as it stood at the end of the launch
-  int pad;
+  u32 seq;          // the launch's sequence number, written after out and err
 };
 
 // Finishing step of a single-graph launch.  Each workgroup stores its totals
@@ -1680,6 +1680,7 @@ struct ResFin {
   // table it reads (the same values: no conflict), workgroup 0 also into the
   // adjacency -- a move folded into the evaluation's launch
   ResPatch patch;
+  u32 seq;             // stored into host->seq once the result is in host memory
 };
 
 __device__ __forceinline__ u32 wave_or(u32 v)
@@ -1715,11 +1716,12 @@ __device__ __forceinline__ void res_finish(const ResFin& fin, int* qhead, int la
   flags = wave_or(flags);
   diam = wave_max(diam);
   if (lane == 0) {
-    ResHost h;
-    h.out.sum = sum; h.out.diam = diam; h.out.flags = flags; h.out.levels = (u32)levels; h.out.pad = 0;
-    h.err = __hip_atomic_load(fin.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    h.pad = 0;
-    *fin.host = h;
+    ResOut o;
+    o.sum = sum; o.diam = diam; o.flags = flags; o.levels = (u32)levels; o.pad = 0;
+    fin.host->out = o;
+    fin.host->err = __hip_atomic_load(fin.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __threadfence_system();                           // the result before its sequence number
+    __hip_atomic_store(&fin.host->seq, fin.seq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
     __hip_atomic_store(qhead, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __hip_atomic_store(fin.done, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     __threadfence_system();
