@@ -509,8 +509,15 @@ __global__ __launch_bounds__(LEVEL_WG) void k_level(const LevelTask* __restrict_
 // and listing are k_level's (the plain variant): every result is bit-identical.
 
 #define PG_MAXP 4
-#define PG_UNIT 1024         // rows of a unit: one 1024-thread workgroup, 128 KiB of LDS rows
-#define PG_WG 1024
+#ifndef PG_SLOTS
+#define PG_SLOTS 12          // gathers a wave keeps in flight (registers: 4 per slot)
+#endif
+#define PG_CH (8 * PG_SLOTS) // pairs of a chunk
+#define PG_NL ((PG_CH + 63) / 64)
+#define PG_UNIT 1024         // rows of a unit, 128 KiB of LDS rows: one workgroup per CU
+#ifndef PG_WG
+#define PG_WG 1024           // threads of the workgroup (a divisor of PG_UNIT)
+#endif
 
 struct PgArgs {
   const u32* list;       // per unit, parts in order: (row in unit) << 22 | neighbour
@@ -631,11 +638,12 @@ __global__ __launch_bounds__(PG_WG) void k_pgather(const LevelTask* __restrict__
     __syncthreads();
     const u32* __restrict__ ul = a.list + (int64_t)ur * PG_UNIT * a.sp;
     const int* po = a.poff + ur * (PG_MAXP + 1);
-    // the parts in order; within a part the waves take 64-pair chunks in turn.
-    // A chunk's eight gathers (eight rows each) are in flight while the wave
-    // ORs in the previous chunk's rows: a slot's next gather is issued as soon
-    // as its rows are ORed in, and a chunk's pairs are loaded a chunk ahead.
-    // (Waiting slot by slot instead of once per chunk was measured slower.)
+    // the parts in order; within a part the waves take chunks of PG_CH pairs in
+    // turn.  A chunk's PG_SLOTS gathers (eight rows each) are in flight while
+    // the wave ORs in the previous chunk's rows: a slot's next gather is issued
+    // as soon as its rows are ORed in, and a chunk's pairs are loaded a chunk
+    // ahead.  (Waiting slot by slot instead of once per chunk was measured
+    // slower.)  The in-flight rows live in registers: PG_SLOTS sets how many.
     const u32 SENT = 0xffffffffu;
     // the unit's part offsets, read once (no memory read in the chunk loop may
     // make the wave wait for the gathers in flight)
@@ -646,41 +654,49 @@ __global__ __launch_bounds__(PG_WG) void k_pgather(const LevelTask* __restrict__
     auto pbeg = [&](int pp) { return pp == 0 ? b0 : pp == 1 ? b1 : pp == 2 ? b2 : pp == 3 ? b3 : b4; };
     auto pend = [&](int pp) { return pp == 0 ? b1 : pp == 1 ? b2 : pp == 2 ? b3 : b4; };
     auto advance = [&](int& pp, int& cc) {                 // uniform
-      while (pp < a.P && cc >= pend(pp)) { pp++; if (pp < a.P) cc = pbeg(pp) + wave * 64; }
+      while (pp < a.P && cc >= pend(pp)) { pp++; if (pp < a.P) cc = pbeg(pp) + wave * PG_CH; }
     };
-    auto pairs = [&](int pp, int cc) -> u32 {              // every lane loads (an idle one the unit's first entry)
-      const bool ok = pp < a.P && cc + lane < pend(pp);
-      const u32 v = ul[ok ? cc + lane : 0];
-      return ok ? v : SENT;
+    // a chunk's pairs, 64 per register (every lane loads: an idle one the unit's first entry)
+    struct Pairs { u32 v[PG_NL]; };
+    auto pairs = [&](int pp, int cc) -> Pairs {
+      Pairs r;
+#pragma unroll
+      for (int h = 0; h < PG_NL; h++) {
+        const int i = cc + 64 * h + lane;
+        const bool ok = pp < a.P && 64 * h + lane < PG_CH && i < pend(pp);
+        const u32 v = ul[ok ? i : 0];
+        r.v[h] = ok ? v : SENT;
+      }
+      return r;
     };
-    u64x2 gv[8];
-    int lr[8];
+    u64x2 gv[PG_SLOTS];
+    int lr[PG_SLOTS];
     // Every lane loads (a pair-less lane past the slab's end, which reads zero):
     // loads under a lane mask would make the compiler wait for all of them.
-    auto issue = [&](u32 chunk, int q) {
-      const u32 e = (u32)__shfl((int)chunk, q * 8 + (lane >> 3), 64);
+    auto issue = [&](const Pairs& chunk, int q) {
+      const u32 e = (u32)__shfl((int)chunk.v[q >> 3], (q & 7) * 8 + (lane >> 3), 64);
       lr[q] = e == SENT ? -1 : (int)(e >> 22);
       const u32 off = e == SENT ? 0xfffffff0u : (e & 0x3fffff) * 128u + w * 8u;
       const u32x4v v = __builtin_amdgcn_raw_buffer_load_b128(src, (int)off, 0, AUX);
       gv[q].x = ((u64)v.y << 32) | v.x; gv[q].y = ((u64)v.w << 32) | v.z;
     };
-    int p0 = 0, k0 = b0 + wave * 64;
+    int p0 = 0, k0 = b0 + wave * PG_CH;
     advance(p0, k0);
-    int p1 = p0, k1 = k0 + nw * 64;
+    int p1 = p0, k1 = k0 + nw * PG_CH;
     advance(p1, k1);
-    const u32 first = pairs(p0, k0);
-    u32 next = pairs(p1, k1);
+    const Pairs first = pairs(p0, k0);
+    Pairs next = pairs(p1, k1);
 #pragma unroll
-    for (int q = 0; q < 8; q++) issue(first, q);
+    for (int q = 0; q < PG_SLOTS; q++) issue(first, q);
     while (p0 < a.P) {                                    // wave-uniform: this chunk's gathers are in flight
-      int p2 = p1, k2 = k1 + nw * 64;
+      int p2 = p1, k2 = k1 + nw * PG_CH;
       advance(p2, k2);
-      const u32 after = pairs(p2, k2);
+      const Pairs after = pairs(p2, k2);
       // Rows are 128 bytes apart, so the eight rows of one instruction meet the
       // same banks; an odd row ORs its second word first, so each ds_or_b64
       // spreads over both halves of the banks.
 #pragma unroll
-      for (int q = 0; q < 8; q++) {
+      for (int q = 0; q < PG_SLOTS; q++) {
         if (lr[q] >= 0) {
           u64* d = (u64*)&acc[lr[q] * 8 + (w >> 1)];
           const int odd = lr[q] & 1;
@@ -693,10 +709,10 @@ __global__ __launch_bounds__(PG_WG) void k_pgather(const LevelTask* __restrict__
       p0 = p1; k0 = k1; p1 = p2; k1 = k2; next = after;
     }
     __syncthreads();
-    // the unit's rows: written, counted, listed; wave w takes rows [64 w, 64 w + 64)
+    // the unit's rows: written, counted, listed; wave w takes rows [R w, R w + R), R = PG_UNIT / nw
     u64 pop = 0;
-    for (int g = 0; g < 8; g++) {
-      const int lr = wave * 64 + g * 8 + (lane >> 3), r = ur * PG_UNIT + lr;
+    for (int g = 0; g < PG_UNIT / nw / 8; g++) {
+      const int lr = wave * (PG_UNIT / nw) + g * 8 + (lane >> 3), r = ur * PG_UNIT + lr;
       const bool valid = r < a.n;
       const u64x2 v = acc[lr * 8 + (w >> 1)];
       if (valid) {
