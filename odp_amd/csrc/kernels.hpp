@@ -656,14 +656,20 @@ __global__ __launch_bounds__(PG_WG) void k_pgather(const LevelTask* __restrict__
     auto advance = [&](int& pp, int& cc) {                 // uniform
       while (pp < a.P && cc >= pend(pp)) { pp++; if (pp < a.P) cc = pbeg(pp) + wave * PG_CH; }
     };
-    // a chunk's pairs, 64 per register (every lane loads: an idle one the unit's first entry)
+    // a chunk's pairs, 64 per register (every lane loads: an idle one the unit's
+    // first entry).  Lane group g (eight lanes, one row) takes the chunk's pairs
+    // [g PG_SLOTS, (g + 1) PG_SLOTS), one per slot: a row's pairs of a part are
+    // consecutive in the list, so the eight groups of a slot meet eight
+    // different rows and their ORs no two lanes' same LDS word.  Register h,
+    // lane l holds the pair of slot 8 h + l / 8, group l % 8.
     struct Pairs { u32 v[PG_NL]; };
     auto pairs = [&](int pp, int cc) -> Pairs {
       Pairs r;
 #pragma unroll
       for (int h = 0; h < PG_NL; h++) {
-        const int i = cc + 64 * h + lane;
-        const bool ok = pp < a.P && 64 * h + lane < PG_CH && i < pend(pp);
+        const int q = 8 * h + (lane >> 3);
+        const int i = cc + (lane & 7) * PG_SLOTS + q;
+        const bool ok = pp < a.P && q < PG_SLOTS && i < pend(pp);
         const u32 v = ul[ok ? i : 0];
         r.v[h] = ok ? v : SENT;
       }
@@ -692,17 +698,19 @@ __global__ __launch_bounds__(PG_WG) void k_pgather(const LevelTask* __restrict__
       int p2 = p1, k2 = k1 + nw * PG_CH;
       advance(p2, k2);
       const Pairs after = pairs(p2, k2);
-      // Rows are 128 bytes apart, so the eight rows of one instruction meet the
-      // same banks; an odd row ORs its second word first, so each ds_or_b64
-      // spreads over both halves of the banks.
+      // Rows are 128 bytes apart: a row's eight lanes meet 16 of the 64 banks
+      // with the first word of their pieces, the other 16 of the row's half of
+      // the banks with the second.  A row with bit 1 set ORs its second word
+      // first, so the rows of an instruction spread over four classes of 16
+      // banks (row mod 4) instead of two.
 #pragma unroll
       for (int q = 0; q < PG_SLOTS; q++) {
         if (lr[q] >= 0) {
           u64* d = (u64*)&acc[lr[q] * 8 + (w >> 1)];
-          const int odd = lr[q] & 1;
-          const u64 f = odd ? gv[q].y : gv[q].x, g = odd ? gv[q].x : gv[q].y;
-          if (f) atomicOr(d + odd, f);
-          if (g) atomicOr(d + (odd ^ 1), g);
+          const int sw = (lr[q] >> 1) & 1;
+          const u64 f = sw ? gv[q].y : gv[q].x, g = sw ? gv[q].x : gv[q].y;
+          if (f) atomicOr(d + sw, f);
+          if (g) atomicOr(d + (sw ^ 1), g);
         }
         issue(next, q);                                     // all SENT past the last chunk
       }
