@@ -651,8 +651,13 @@ __global__ __launch_bounds__(PG_WG) void k_pgather(const LevelTask* __restrict__
     const int b2 = a.P > 1 ? __builtin_amdgcn_readfirstlane(po[2]) : b1;
     const int b3 = a.P > 2 ? __builtin_amdgcn_readfirstlane(po[3]) : b2;
     const int b4 = a.P > 3 ? __builtin_amdgcn_readfirstlane(po[4]) : b3;
-    auto pbeg = [&](int pp) { return pp == 0 ? b0 : pp == 1 ? b1 : pp == 2 ? b2 : pp == 3 ? b3 : b4; };
-    auto pend = [&](int pp) { return pp == 0 ? b1 : pp == 1 ? b2 : pp == 2 ? b3 : b4; };
+    // The XCD's workgroups take its units in rounds; every other round sweeps
+    // the parts backwards, so it starts with the part the round before ended
+    // with, whose rows are still in L2.
+    const bool rev = (((u - u_lo - wi) / nwx) & 1) != 0;
+    auto boff = [&](int p) { return p == 0 ? b0 : p == 1 ? b1 : p == 2 ? b2 : p == 3 ? b3 : b4; };
+    auto pbeg = [&](int pp) { return boff(rev ? a.P - 1 - pp : pp); };
+    auto pend = [&](int pp) { return boff(rev ? a.P - pp : pp + 1); };
     auto advance = [&](int& pp, int& cc) {                 // uniform
       while (pp < a.P && cc >= pend(pp)) { pp++; if (pp < a.P) cc = pbeg(pp) + wave * PG_CH; }
     };
@@ -686,7 +691,7 @@ __global__ __launch_bounds__(PG_WG) void k_pgather(const LevelTask* __restrict__
       const u32x4v v = __builtin_amdgcn_raw_buffer_load_b128(src, (int)off, 0, AUX);
       gv[q].x = ((u64)v.y << 32) | v.x; gv[q].y = ((u64)v.w << 32) | v.z;
     };
-    int p0 = 0, k0 = b0 + wave * PG_CH;
+    int p0 = 0, k0 = pbeg(0) + wave * PG_CH;
     advance(p0, k0);
     int p1 = p0, k1 = k0 + nw * PG_CH;
     advance(p1, k1);
