@@ -327,11 +327,15 @@ struct Evaluator {
   // build reads it there and copies it to d_adj (small tables: no DMA command)
   bool adj_on_host = false;
   size_t stage_cap = 0;
-  // the resident plan's 16-bit table matches the adjacency at `table_src`: set by
-  // its preparation, kept by device swaps (they patch the table as well), cleared
-  // by an upload, a new shape or a new plan
+  // the current plan's table (the resident plan's 16-bit table; the level plan's
+  // 32-bit table and its 16-bit copy) matches the adjacency at `table_src`: set
+  // by its preparation, kept by device swaps (they patch the table as well),
+  // cleared by an upload, a new shape or a new plan
   bool table_fresh = false;
   const int* table_src = nullptr;
+  // the level plan's pair lists (k_plist) match its table (device swaps rebuild
+  // the units of the rows they change); meaningful while table_fresh holds
+  bool pl_fresh = false;
   // table entries the next resident launch changes first (triples row, position, id)
   std::vector<int> pend_patch;
   // h_stage holds what the device copy holds (or will, once the pending entries are stored):
@@ -370,6 +374,41 @@ struct Evaluator {
     if (table_fresh && table_src == d_adj && plan.plan == ODPX_PLAN_RESIDENT && s == 1 && !grid_s) return d_nbr16;
     table_fresh = false;
     return nullptr;
+  }
+
+  // A 2-opt move a-b, c-d -> a-c, b-d on the device copy (k_swap, all or
+  // nothing).  A current table is patched with it: the resident plan's 16-bit
+  // table, or the level plan's 32-bit table and 16-bit copy, whose pair lists
+  // are then rebuilt for the units of the four rows (k_plist, at most four
+  // workgroups).  Otherwise the next evaluation rebuilds the table.
+  void swap_launch(int a, int b, int c, int d)
+  {
+    SwapTables tb{};
+    const bool cur = table_fresh && table_src == d_adj && s == 1 && !grid_s;
+    if (cur && plan.plan == ODPX_PLAN_RESIDENT) {
+      tb.t16 = d_nbr16; tb.tdp = dp16;
+    } else if (cur && plan.plan == ODPX_PLAN_LEVEL) {
+      tb.t32 = d_nbr; tb.t16 = direct16() ? d_nbr16 : nullptr; tb.tdp = dpad;
+    } else {
+      table_fresh = false;
+    }
+    hipLaunchKernelGGL(k_swap, dim3(1), dim3(64), 0, C.stream, d_adj, degree, a, b, c, d, d_err, tb);
+    HIPC(hipGetLastError());
+    if (tb.t32 && pl_fresh) {
+      PlUnits sel{};
+      const int rows[4] = {a, b, c, d};
+      for (int r : rows) {
+        const int u = r / PG_UNIT;
+        bool seen = false;
+        for (int i = 0; i < sel.cnt; i++) seen = seen || sel.u[i] == u;
+        if (!seen) sel.u[sel.cnt++] = u;
+      }
+      LevelRun& q = lr;
+      auto kern = dpad + 1 <= 20 ? k_plist<20> : dpad + 1 <= 36 ? k_plist<36> : k_plist<64>;
+      hipLaunchKernelGGL(kern, dim3(sel.cnt), dim3(PG_UNIT), 0, C.stream, d_nbr, n, dpad, q.psp, q.parts, q.plist,
+                         q.ppoff, sel);
+      HIPC(hipGetLastError());
+    }
   }
   ODPX_Plan plan{};
   LevelRun lr;
@@ -910,7 +949,8 @@ struct Evaluator {
   }
 
   // ---- evaluation of the adjacency on the device ----
-  void prepare(u64* z64, int nz64, u32* z32, int nz32)
+  // zeroes what it is given and (table) builds the current plan's table
+  void prepare(u64* z64, int nz64, u32* z32, int nz32, bool table = true)
   {
     PrepArgs a;
     const bool res = plan.plan == ODPX_PLAN_RESIDENT;
@@ -925,7 +965,8 @@ struct Evaluator {
     a.n = n; a.s = s; a.degree = degree; a.dpad = res ? dp16 : dpad;
     a.width = width; a.height = height; a.grid_s = grid_s;
     a.zero64 = z64; a.nzero64 = nz64; a.zero32 = z32; a.nzero32 = nz32;
-    int64_t total = (int64_t)n * a.dpad;
+    if (!table) { a.nbr = nullptr; a.nbr16 = nullptr; a.dpad = 0; }
+    const int64_t total = std::max<int64_t>((int64_t)n * a.dpad, std::max(nz64, nz32));
     int grid = (int)std::min<int64_t>(std::max<int64_t>(1, (total + 255) / 256), 4096);
     TimedLaunch tl;
     timed_begin(tl, T_OTHER, -1, st);
@@ -964,6 +1005,7 @@ struct Evaluator {
     int tside = TILE_MIN;               // k_tile's tile side (halo = (64 - side) / 2)
     int spec = -1;                      // level of the current group launched count-only (expected last)
     bool graphed = false;
+    bool fresh = false, plfresh = false; // the preparation keeps the table / the pair lists
     int first = 0, cnt_t = 0;
     int launched = 0, examined = 0, running = 0, batch_end = 0;
     std::vector<LvTask> sts;
@@ -1166,7 +1208,7 @@ struct Evaluator {
   void lv_prepare()
   {
     LevelRun& r = lr;
-    prepare(r.cnt, lv.nzero, r.done, r.ntasks + 1);
+    prepare(r.cnt, lv.nzero, r.done, r.ntasks + 1, !lv.fresh);
     if (lv.tk) {
       const int64_t total = (int64_t)n * dpad;
       const int grid = (int)std::min<int64_t>(std::max<int64_t>(1, (total + 255) / 256), 4096);
@@ -1179,19 +1221,21 @@ struct Evaluator {
       hipLaunchKernelGGL(k_symcheck, dim3(grid), dim3(256), 0, st, d_nbr, n, dpad, r.done + r.ntasks);
       HIPC(hipGetLastError());
     }
-    if (pg_on()) {
+    if (pg_on() && !lv.plfresh) {
       // the units' (row, neighbour) pairs grouped by part for k_pgather
       auto kern = dpad + 1 <= 20 ? k_plist<20> : dpad + 1 <= 36 ? k_plist<36> : k_plist<64>;
-      hipLaunchKernelGGL(kern, dim3(r.pupt), dim3(PG_UNIT), 0, st, d_nbr, n, dpad, r.psp, r.parts, r.plist, r.ppoff);
+      hipLaunchKernelGGL(kern, dim3(r.pupt), dim3(PG_UNIT), 0, st, d_nbr, n, dpad, r.psp, r.parts, r.plist, r.ppoff,
+                         PlUnits{});
       HIPC(hipGetLastError());
     }
   }
 
-  // graph key of a first batch ending at be: its length, the kernel variant and
-  // whether its last level is count-only
+  // graph key of a first batch ending at be: its length, the kernel variant,
+  // whether its last level is count-only and which tables the preparation builds
   int lv_key(int be) const
   {
-    return 64 * (16 * (4 * be + (lv.fr ? 1 : 0) + (spec_at(be) ? 2 : 0)) + lv.tk) + lv.tside;
+    return 4 * (64 * (16 * (4 * be + (lv.fr ? 1 : 0) + (spec_at(be) ? 2 : 0)) + lv.tk) + lv.tside) +
+           (lv.fresh ? 1 : 0) + (lv.plfresh ? 2 : 0);
   }
 
   void lv_begin()
@@ -1199,6 +1243,9 @@ struct Evaluator {
     flush_adj();
     LevelRun& r = lr;
     trace("run_level");
+    // the table (and the pair lists) the last evaluation built still match the adjacency
+    lv.fresh = table_fresh && table_src == cur_adj();
+    if (!lv.fresh) pl_fresh = false;
     lv.K = r.direct;                      // levels 0..K-1 come from k_direct (0: k_level0 does level 0)
     lv.L1 = lv.K ? lv.K : 1;              // first level the gather kernel runs
     lv.nzero = 3 * RING * r.ntasks;       // the rings (counts, changed, processed) start at zero
@@ -1212,6 +1259,7 @@ struct Evaluator {
     // frontier mode pays off over long level sequences; with few levels every
     // level is dense and the plain kernel is faster
     lv.fr = !lv.tk && (guess_levels == 0 || guess_levels > FRONTIER_MIN_LEVELS);
+    lv.plfresh = lv.fresh && pl_fresh;
     lv.graphed = r.group == r.ntasks && use_graph(lv_key(lv.be0));
     if (!lv.graphed) lv_prepare();
     lv.sts.assign(r.ntasks, LvTask{1, 0, 0, 0});
@@ -1219,6 +1267,11 @@ struct Evaluator {
     lv.max_d = 1; lv.lmax = 0;
     lv.first = 0;
     lv_group_start();
+    // the preparation is enqueued: the table matches the adjacency, and the
+    // pair lists when this evaluation's gather levels use them
+    table_fresh = true;
+    table_src = cur_adj();
+    if (pg_on()) pl_fresh = true;
   }
 
   bool lv_step()
@@ -1766,33 +1819,6 @@ extern "C" void ODPX_Set_aspl_resident(int* diameter, long* sum, double* ASPL)
   E.times++;
 }
 
-// 2-opt swap a-b, c-d -> a-c, b-d on the device adjacency (s = 1), all or
-// nothing: one thread finds the four entries first (rows a, b, c, d holding b,
-// a, d, c) and writes only when all four exist; otherwise it changes nothing
-// and sets bit 4 of *err.  The ids must be distinct (the host checks), so the
-// four entries are four different cells.  With `table` set it also patches the
-// resident plan's 16-bit table, entry for entry: the table holds the
-// adjacency's entry where it is a neighbour and the row itself where it is
-// padding, so an entry equal to `from` is the same edge.
-__global__ void k_swap(int* adj, int degree, int a, int b, int c, int d, int* err, unsigned short* table, int tdp)
-{
-  if (threadIdx.x != 0) return;
-  const int rows[4] = {a, b, c, d}, from[4] = {b, a, d, c}, to[4] = {c, d, a, b};
-  int pos[4];
-  for (int e = 0; e < 4; e++) {
-    const int* r = adj + (int64_t)rows[e] * degree;
-    pos[e] = -1;
-    for (int j = 0; j < degree; j++)
-      if (r[j] == from[e]) { pos[e] = j; break; }
-    if (pos[e] < 0) { atomicOr(err, 4); return; }
-  }
-  for (int e = 0; e < 4; e++) {
-    adj[(int64_t)rows[e] * degree + pos[e]] = to[e];
-    unsigned short* t = table ? table + (int64_t)rows[e] * tdp + pos[e] : nullptr;
-    if (t && *t == (unsigned short)from[e]) *t = (unsigned short)to[e];
-  }
-}
-
 extern "C" int ODPX_Swap_device(int a, int b, int c, int d)
 {
   RandFence rand_fence;
@@ -1821,14 +1847,10 @@ extern "C" int ODPX_Swap_device(int a, int b, int c, int d)
     }
     for (int e = 0; e < 4; e++) v.h_mirror[(size_t)rows[e] * v.degree + pos[e]] = to[e];
     v.stage_is_device = false;
-    hipLaunchKernelGGL(k_swap, dim3(1), dim3(64), 0, C.stream, v.d_adj, v.degree, a, b, c, d, v.d_err,
-                       v.swap_table(), v.dp16);
-    HIPC(hipGetLastError());
+    v.swap_launch(a, b, c, d);
   } else {
     v.stage_is_device = false;         // the device copy moves on its own
-    hipLaunchKernelGGL(k_swap, dim3(1), dim3(64), 0, C.stream, v.d_adj, v.degree, a, b, c, d, v.d_err,
-                       v.swap_table(), v.dp16);
-    HIPC(hipGetLastError());
+    v.swap_launch(a, b, c, d);
     HIPC(hipMemcpyAsync(v.h_err, v.d_err, sizeof(int), hipMemcpyDeviceToHost, C.stream));
     HIPC(hipStreamSynchronize(C.stream));
     if (*v.h_err & 4) {
@@ -2158,9 +2180,7 @@ extern "C" int ODPX_Anneal(void* adjacency, long long ncalcs, double max_temp, d
     if (fold) {
       queue_move(sw[0], sw[1], sw[2], sw[3]);
     } else {
-      hipLaunchKernelGGL(k_swap, dim3(1), dim3(64), 0, C.stream, v.d_adj, deg, sw[0], sw[1], sw[2], sw[3], v.d_err,
-                         v.swap_table(), v.dp16);
-      HIPC(hipGetLastError());
+      v.swap_launch(sw[0], sw[1], sw[2], sw[3]);
     }
     const double t = wall_time();
     v.evaluate(&diameter, &sum, &ASPL);
@@ -2182,9 +2202,7 @@ extern "C" int ODPX_Anneal(void* adjacency, long long ncalcs, double max_temp, d
       if (fold) {
         queue_move(sw[0], sw[2], sw[1], sw[3]);
       } else {
-        hipLaunchKernelGGL(k_swap, dim3(1), dim3(64), 0, C.stream, v.d_adj, deg, sw[0], sw[2], sw[1], sw[3], v.d_err,
-                           v.swap_table(), v.dp16);
-        HIPC(hipGetLastError());
+        v.swap_launch(sw[0], sw[2], sw[1], sw[3]);
       }
     }
     temp *= cooling;

@@ -168,6 +168,41 @@ __global__ void k_prepare(PrepArgs a)
   }
 }
 
+// the row-neighbour tables a device swap patches along with the adjacency
+struct SwapTables {
+  int* t32;                 // 32-bit table or null
+  unsigned short* t16;      // 16-bit table or null
+  int tdp;                  // their row stride
+};
+
+// 2-opt swap a-b, c-d -> a-c, b-d on the device adjacency (s = 1), all or
+// nothing: one thread finds the four entries first (rows a, b, c, d holding b,
+// a, d, c) and writes only when all four exist; otherwise it changes nothing
+// and sets bit 4 of *err.  The ids must be distinct (the host checks), so the
+// four entries are four different cells.  The tables set in `tb` are patched
+// entry for entry: a table row holds the adjacency's entry where it is a
+// neighbour and the row itself where it is padding, so an entry equal to
+// `from` is the same edge.
+__global__ void k_swap(int* adj, int degree, int a, int b, int c, int d, int* err, SwapTables tb)
+{
+  if (threadIdx.x != 0) return;
+  const int rows[4] = {a, b, c, d}, from[4] = {b, a, d, c}, to[4] = {c, d, a, b};
+  int pos[4];
+  for (int e = 0; e < 4; e++) {
+    const int* r = adj + (int64_t)rows[e] * degree;
+    pos[e] = -1;
+    for (int j = 0; j < degree; j++)
+      if (r[j] == from[e]) { pos[e] = j; break; }
+    if (pos[e] < 0) { atomicOr(err, 4); return; }
+  }
+  for (int e = 0; e < 4; e++) {
+    adj[(int64_t)rows[e] * degree + pos[e]] = to[e];
+    const int64_t k = (int64_t)rows[e] * tb.tdp + pos[e];
+    if (tb.t16 && tb.t16[k] == (unsigned short)from[e]) tb.t16[k] = (unsigned short)to[e];
+    if (tb.t32 && tb.t32[k] == from[e]) tb.t32[k] = to[e];
+  }
+}
+
 // Sets *asym when some row u is listed among the neighbours of row r while r
 // is not among u's (entries equal to the row itself are padding).
 __global__ void k_symcheck(const int* __restrict__ nbr, int n, int dpad, u32* asym)
@@ -498,8 +533,9 @@ __global__ __launch_bounds__(LEVEL_WG) void k_level(const LevelTask* __restrict_
 // So the rows of the slab are cut into P parts (rows [p n / P, (p+1) n / P)).
 // Rows are taken in units of PG_UNIT consecutive rows; for every unit and part
 // a list holds the unit's (row, neighbour) pairs whose neighbour lies in that
-// part (the row itself counts as a neighbour; repeats are dropped), built once
-// per evaluation by k_plist.  Workgroups are persistent and grouped
+// part (the row itself counts as a neighbour; repeats are dropped), built by
+// k_plist with the table (a device swap rebuilds the units of the rows it
+// changes).  Workgroups are persistent and grouped
 // by XCD (workgroup b runs on XCD b % 8): the workgroups of one XCD take
 // consecutive units -- the same block -- and sweep the parts in order, with
 // their rows' next values accumulated in LDS (ds_or) across the sweep.  While
@@ -532,13 +568,17 @@ struct PgArgs {
 // parts, a scan of the rows' run lengths per part, and the unit's pairs
 // grouped by part (in any order within a part) with the part offsets.
 // DMAX >= dpad + 1 bounds the ids a row holds in registers.
+// sel names the units to build (one workgroup each); with sel.cnt = 0
+// workgroup u builds unit u.
+struct PlUnits { int u[4]; int cnt; };
+
 template <int DMAX>
 __global__ __launch_bounds__(PG_UNIT) void k_plist(const int* __restrict__ nbr, int n, int dpad, int sp, int P,
-                                                   u32* __restrict__ list, int* __restrict__ poff)
+                                                   u32* __restrict__ list, int* __restrict__ poff, PlUnits sel)
 {
   __shared__ int scan[PG_MAXP][PG_UNIT];
   __shared__ int tot[PG_MAXP];
-  const int u = blockIdx.x, t = threadIdx.x, r = u * PG_UNIT + t;
+  const int u = sel.cnt ? sel.u[blockIdx.x] : blockIdx.x, t = threadIdx.x, r = u * PG_UNIT + t;
   int id[DMAX];
   int run[PG_MAXP];
 #pragma unroll

@@ -75,6 +75,9 @@ CASES = [
     (100, 3, 200, 100.0, 1.0, True, False, 5, None),
     (96, 5, 250, 20.0, 0.2, False, True, 7, None),
     (600, 5, 60, 80.0, 1.0, False, False, 11, dict(plan=odp.PLAN_LEVEL, kw=4, upw=4, seg_major=0, xcd=1, direct=-1)),
+    # row-part gathers: every move and undo patches the tables and the units' pair lists
+    (2100, 12, 40, 80.0, 1.0, False, False, 13,
+     dict(plan=odp.PLAN_LEVEL, kw=16, upw=8, seg_major=0, xcd=1, direct=-1, parts=4)),
 ]
 
 

@@ -304,6 +304,58 @@ def test_resident_table_follows_swaps_and_uploads():
         odp.finalize_aspl()
 
 
+@pytest.mark.parametrize("plan", [
+    dict(plan=LEV, kw=16, upw=8, seg_major=0, xcd=1, direct=-1, parts=4),
+    dict(plan=LEV, kw=16, upw=8, seg_major=0, xcd=1, direct=0, parts=2, group=2),
+    dict(plan=LEV, kw=4, upw=4, seg_major=0, xcd=1, direct=3),
+])
+def test_level_table_follows_swaps(plan):
+    """The level plan keeps its row-neighbour tables (32-bit, and the 16-bit
+    copy k_direct reads) and the row-part pair lists between evaluations: a
+    device swap patches the tables and rebuilds the pair lists of the units
+    (1024 rows) its four rows lie in; an upload, ODP_Set_aspl or a plan change
+    makes the next evaluation rebuild them.  Every evaluation must equal the
+    oracle, rejected moves included (they change nothing)."""
+    n, d = 3000, 12                                   # 3 units; 47 words: 2 full 16-word blocks + 15
+    g1 = odp.generate_regular(n, d, 91)
+    g2 = odp.generate_regular(n, d, 92)
+    rng = odp.Rng(17)
+    odp.init_aspl_general(n, d)
+    try:
+        odp.set_plan(**plan)
+        host = g1.copy()
+        odp.upload_adjacency(host)
+        for _ in range(3):                            # frontier first, then the plain levels
+            assert same(odp.set_aspl_resident(), oracle.evaluate(host, nodes=n))
+        for i in range(8):
+            a, b, c, dd = odp.two_opt(host, rng)
+            assert odp.swap_device(a, b, c, dd) == 0
+            if i % 3 == 2:                            # two moves between evaluations
+                a, b, c, dd = odp.two_opt(host, rng)
+                assert odp.swap_device(a, b, c, dd) == 0
+            assert same(odp.set_aspl_resident(), oracle.evaluate(host, nodes=n)), i
+        assert odp.swap_device(0, 0, 0, 0) == -1      # not a move: nothing changes
+        a, b = 5, int(host[5][0])                     # a-b an edge, c-d not: nothing changes
+        c = next(v for v in range(n) if v not in (a, b) and v not in host[a] and v not in host[b])
+        dd = next(v for v in range(n) if v not in (a, b, c) and v not in host[c])
+        assert odp.swap_device(a, b, c, dd) == -1
+        assert same(odp.set_aspl_resident(), oracle.evaluate(host, nodes=n))
+        odp.upload_adjacency(g2)
+        assert same(odp.set_aspl_resident(), oracle.evaluate(g2, nodes=n))
+        assert same(odp.set_aspl(host), oracle.evaluate(host, nodes=n))
+        for _ in range(3):
+            a, b, c, dd = odp.two_opt(host, rng)
+            assert odp.swap_device(a, b, c, dd) == 0
+            assert same(odp.set_aspl_resident(), oracle.evaluate(host, nodes=n))
+        odp.set_plan(**plan)                          # a new plan rebuilds everything
+        assert same(odp.set_aspl_resident(), oracle.evaluate(host, nodes=n))
+        a, b, c, dd = odp.two_opt(host, rng)
+        assert odp.swap_device(a, b, c, dd) == 0
+        assert same(odp.set_aspl_resident(), oracle.evaluate(host, nodes=n))
+    finally:
+        odp.finalize_aspl()
+
+
 def test_drop_in_small_changes_as_patches():
     """ODP_Set_aspl on an array that differs from the last upload in a few
     entries (the annealing pattern: a move, then Set_aspl on the same array)
