@@ -376,12 +376,15 @@ struct Evaluator {
     return nullptr;
   }
 
-  // A 2-opt move a-b, c-d -> a-c, b-d on the device copy (k_swap, all or
-  // nothing).  A current table is patched with it: the resident plan's 16-bit
-  // table, or the level plan's 32-bit table and 16-bit copy, whose pair lists
-  // are then rebuilt for the units of the four rows (k_plist, at most four
-  // workgroups).  Otherwise the next evaluation rebuilds the table.
-  void swap_launch(int a, int b, int c, int d)
+  // A 2-opt move a-b, c-d -> a-c, b-d on the device copy.  A current table is
+  // patched with it: the resident plan's 16-bit table, or the level plan's
+  // 32-bit table and 16-bit copy, whose pair lists are then rebuilt for the
+  // units of the four rows (at most four workgroups).  Otherwise the next
+  // evaluation rebuilds the table.  With the positions of the four entries
+  // known (pos: found on the host copy by k_swap's rule, the move checked
+  // there) one launch stores them and rebuilds the units (k_swap_at);
+  // otherwise k_swap finds them (all or nothing) and k_swap_at rebuilds.
+  void swap_launch(int a, int b, int c, int d, const int* pos = nullptr)
   {
     SwapTables tb{};
     const bool cur = table_fresh && table_src == d_adj && s == 1 && !grid_s;
@@ -392,23 +395,30 @@ struct Evaluator {
     } else {
       table_fresh = false;
     }
-    hipLaunchKernelGGL(k_swap, dim3(1), dim3(64), 0, C.stream, d_adj, degree, a, b, c, d, d_err, tb);
-    HIPC(hipGetLastError());
+    const int rows[4] = {a, b, c, d}, from[4] = {b, a, d, c}, to[4] = {c, d, a, b};
+    PlUnits units{};
+    PlArgs pl{};
     if (tb.t32 && pl_fresh) {
-      PlUnits sel{};
-      const int rows[4] = {a, b, c, d};
       for (int r : rows) {
         const int u = r / PG_UNIT;
         bool seen = false;
-        for (int i = 0; i < sel.cnt; i++) seen = seen || sel.u[i] == u;
-        if (!seen) sel.u[sel.cnt++] = u;
+        for (int i = 0; i < units.cnt; i++) seen = seen || units.u[i] == u;
+        if (!seen) units.u[units.cnt++] = u;
       }
-      LevelRun& q = lr;
-      auto kern = dpad + 1 <= 20 ? k_plist<20> : dpad + 1 <= 36 ? k_plist<36> : k_plist<64>;
-      hipLaunchKernelGGL(kern, dim3(sel.cnt), dim3(PG_UNIT), 0, C.stream, d_nbr, n, dpad, q.psp, q.parts, q.plist,
-                         q.ppoff, sel);
-      HIPC(hipGetLastError());
+      pl.nbr = d_nbr; pl.n = n; pl.dpad = dpad; pl.sp = lr.psp; pl.P = lr.parts; pl.list = lr.plist; pl.poff = lr.ppoff;
     }
+    PlMove mv{};
+    if (pos) {
+      for (int e = 0; e < 4; e++) { mv.row[e] = rows[e]; mv.pos[e] = pos[e]; mv.from[e] = from[e]; mv.to[e] = to[e]; }
+      mv.cnt = 4;
+    } else {
+      hipLaunchKernelGGL(k_swap, dim3(1), dim3(64), 0, C.stream, d_adj, degree, a, b, c, d, d_err, tb);
+      HIPC(hipGetLastError());
+      if (!units.cnt) return;
+    }
+    auto kern = dpad + 1 <= 20 ? k_swap_at<20> : dpad + 1 <= 36 ? k_swap_at<36> : k_swap_at<64>;
+    hipLaunchKernelGGL(kern, dim3(std::max(1, units.cnt)), dim3(PG_UNIT), 0, C.stream, d_adj, degree, tb, mv, units, pl);
+    HIPC(hipGetLastError());
   }
   ODPX_Plan plan{};
   LevelRun lr;
@@ -1224,8 +1234,7 @@ struct Evaluator {
     if (pg_on() && !lv.plfresh) {
       // the units' (row, neighbour) pairs grouped by part for k_pgather
       auto kern = dpad + 1 <= 20 ? k_plist<20> : dpad + 1 <= 36 ? k_plist<36> : k_plist<64>;
-      hipLaunchKernelGGL(kern, dim3(r.pupt), dim3(PG_UNIT), 0, st, d_nbr, n, dpad, r.psp, r.parts, r.plist, r.ppoff,
-                         PlUnits{});
+      hipLaunchKernelGGL(kern, dim3(r.pupt), dim3(PG_UNIT), 0, st, d_nbr, n, dpad, r.psp, r.parts, r.plist, r.ppoff);
       HIPC(hipGetLastError());
     }
   }
@@ -1847,7 +1856,7 @@ extern "C" int ODPX_Swap_device(int a, int b, int c, int d)
     }
     for (int e = 0; e < 4; e++) v.h_mirror[(size_t)rows[e] * v.degree + pos[e]] = to[e];
     v.stage_is_device = false;
-    v.swap_launch(a, b, c, d);
+    v.swap_launch(a, b, c, d, pos);
   } else {
     v.stage_is_device = false;         // the device copy moves on its own
     v.swap_launch(a, b, c, d);

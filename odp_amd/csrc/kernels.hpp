@@ -185,21 +185,26 @@ struct SwapTables {
 // `from` is the same edge.
 __global__ void k_swap(int* adj, int degree, int a, int b, int c, int d, int* err, SwapTables tb)
 {
-  if (threadIdx.x != 0) return;
-  const int rows[4] = {a, b, c, d}, from[4] = {b, a, d, c}, to[4] = {c, d, a, b};
-  int pos[4];
-  for (int e = 0; e < 4; e++) {
-    const int* r = adj + (int64_t)rows[e] * degree;
-    pos[e] = -1;
-    for (int j = 0; j < degree; j++)
-      if (r[j] == from[e]) { pos[e] = j; break; }
-    if (pos[e] < 0) { atomicOr(err, 4); return; }
+  __shared__ int spos[4];
+  auto row = [&](int e) { return e == 0 ? a : e == 1 ? b : e == 2 ? c : d; };
+  auto from = [&](int e) { return e == 0 ? b : e == 1 ? a : e == 2 ? d : c; };
+  auto to = [&](int e) { return e == 0 ? c : e == 1 ? d : e == 2 ? a : b; };
+  if (threadIdx.x < 4) spos[threadIdx.x] = INT_MAX;
+  __syncthreads();
+  // the four rows' entries are read at once; a row's first match wins
+  for (int i = threadIdx.x; i < 4 * degree; i += blockDim.x) {
+    const int e = i / degree, j = i - e * degree;
+    if (adj[(int64_t)row(e) * degree + j] == from(e)) atomicMin(&spos[e], j);
   }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  for (int e = 0; e < 4; e++)
+    if (spos[e] == INT_MAX) { atomicOr(err, 4); return; }
   for (int e = 0; e < 4; e++) {
-    adj[(int64_t)rows[e] * degree + pos[e]] = to[e];
-    const int64_t k = (int64_t)rows[e] * tb.tdp + pos[e];
-    if (tb.t16 && tb.t16[k] == (unsigned short)from[e]) tb.t16[k] = (unsigned short)to[e];
-    if (tb.t32 && tb.t32[k] == from[e]) tb.t32[k] = to[e];
+    adj[(int64_t)row(e) * degree + spos[e]] = to(e);
+    const int64_t k = (int64_t)row(e) * tb.tdp + spos[e];
+    if (tb.t16 && tb.t16[k] == (unsigned short)from(e)) tb.t16[k] = (unsigned short)to(e);
+    if (tb.t32 && tb.t32[k] == from(e)) tb.t32[k] = to(e);
   }
 }
 
@@ -568,26 +573,41 @@ struct PgArgs {
 // parts, a scan of the rows' run lengths per part, and the unit's pairs
 // grouped by part (in any order within a part) with the part offsets.
 // DMAX >= dpad + 1 bounds the ids a row holds in registers.
-// sel names the units to build (one workgroup each); with sel.cnt = 0
-// workgroup u builds unit u.
-struct PlUnits { int u[4]; int cnt; };
+//
+// A device swap hands the moved entries to the units it rebuilds (rows,
+// positions in the row, old and new ids), so that they see them whether or
+// not the table already holds them.
+struct PlMove { int row[4], pos[4], from[4], to[4]; int cnt; };
 
+// one unit's pair lists (a workgroup of PG_UNIT threads, unit u)
 template <int DMAX>
-__global__ __launch_bounds__(PG_UNIT) void k_plist(const int* __restrict__ nbr, int n, int dpad, int sp, int P,
-                                                   u32* __restrict__ list, int* __restrict__ poff, PlUnits sel)
+__device__ __forceinline__ void plist_unit(const int* __restrict__ nbr, int n, int dpad, int sp, int P,
+                                           u32* __restrict__ list, int* __restrict__ poff, int u, const PlMove& mv)
 {
-  __shared__ int scan[PG_MAXP][PG_UNIT];
-  __shared__ int tot[PG_MAXP];
-  const int u = sel.cnt ? sel.u[blockIdx.x] : blockIdx.x, t = threadIdx.x, r = u * PG_UNIT + t;
+  __shared__ u64 wtot[PG_UNIT / 64];
+  const int t = threadIdx.x, r = u * PG_UNIT + t, lane = t & 63, wave = t >> 6;
   int id[DMAX];
-  int run[PG_MAXP];
-#pragma unroll
-  for (int p = 0; p < PG_MAXP; p++) run[p] = 0;
 #pragma unroll
   for (int j = 0; j < DMAX; j++) {
     id[j] = -1;
     if (r < n && j <= dpad) id[j] = j < dpad ? nbr[(int64_t)r * dpad + j] : r;
   }
+  for (int e = 0; e < mv.cnt; e++)
+    if (r == mv.row[e]) {
+#pragma unroll
+      for (int j = 0; j < DMAX; j++)
+        if (j == mv.pos[e] && id[j] == mv.from[e]) id[j] = mv.to[e];
+    }
+  // part of an id: floor(id P / n), the number of cuts ceil(p n / P) (0 < p < P) at or below it
+  int cut[PG_MAXP - 1];
+#pragma unroll
+  for (int p = 1; p < PG_MAXP; p++) cut[p - 1] = p < P ? (int)(((int64_t)p * n + P - 1) / P) : INT_MAX;
+  auto part = [&](int v) {
+    int q = 0;
+#pragma unroll
+    for (int p = 0; p < PG_MAXP - 1; p++) q += v >= cut[p] ? 1 : 0;
+    return q;
+  };
   // drop repeats (an id equal to an earlier one), count the rest per part
 #pragma unroll
   for (int j = 1; j < DMAX; j++) {
@@ -596,47 +616,78 @@ __global__ __launch_bounds__(PG_UNIT) void k_plist(const int* __restrict__ nbr, 
     for (int k = 0; k < j; k++) dup = dup || id[k] == id[j];
     if (dup) id[j] = -1;
   }
+  u64 run = 0;                                      // four 16-bit counts (a unit holds < 65536 pairs of a part)
 #pragma unroll
   for (int j = 0; j < DMAX; j++)
-    if (id[j] >= 0) {
-      const int q = (int)(((int64_t)id[j] * P) / n);
+    if (id[j] >= 0) run += 1ULL << (16 * part(id[j]));
+  // inclusive scan over the unit's rows: within the wave, then the waves before
+  u64 inc = run;
 #pragma unroll
-      for (int p = 0; p < PG_MAXP; p++) run[p] += (p == q) ? 1 : 0;
-    }
-#pragma unroll
-  for (int p = 0; p < PG_MAXP; p++) scan[p][t] = run[p];
-  __syncthreads();
-  // inclusive scans (Hillis-Steele) over the unit's rows, one per part
-  for (int o = 1; o < PG_UNIT; o <<= 1) {
-    int add[PG_MAXP];
-#pragma unroll
-    for (int p = 0; p < PG_MAXP; p++) add[p] = t >= o ? scan[p][t - o] : 0;
-    __syncthreads();
-#pragma unroll
-    for (int p = 0; p < PG_MAXP; p++) scan[p][t] += add[p];
-    __syncthreads();
+  for (int o = 1; o < 64; o <<= 1) {
+    const u64 v = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += v;
   }
-  if (t < PG_MAXP) tot[t] = scan[t][PG_UNIT - 1];
+  if (lane == 63) wtot[wave] = inc;
   __syncthreads();
+  u64 tot = 0;
+  for (int i = 0; i < PG_UNIT / 64; i++) {
+    const u64 v = wtot[i];
+    if (i < wave) inc += v;
+    tot += v;
+  }
+  __syncthreads();                                  // wtot is free for the next unit
   u32* ul = list + (int64_t)u * PG_UNIT * sp;
   int pos[PG_MAXP], base = 0;
 #pragma unroll
   for (int p = 0; p < PG_MAXP; p++) {
-    pos[p] = base + scan[p][t] - run[p];
+    pos[p] = base + (int)((inc >> (16 * p)) & 0xffff) - (int)((run >> (16 * p)) & 0xffff);
     if (t == 0 && p < P) poff[u * (PG_MAXP + 1) + p] = base;
-    base += p < P ? tot[p] : 0;
+    base += p < P ? (int)((tot >> (16 * p)) & 0xffff) : 0;
   }
   if (t == 0) poff[u * (PG_MAXP + 1) + P] = base;
 #pragma unroll
   for (int j = 0; j < DMAX; j++)
     if (id[j] >= 0) {
-      const int q = (int)(((int64_t)id[j] * P) / n);
+      const int q = part(id[j]);
       int at = 0;
 #pragma unroll
       for (int p = 0; p < PG_MAXP; p++)
         if (p == q) at = pos[p]++;
       ul[at] = ((u32)t << 22) | (u32)id[j];
     }
+}
+
+// every unit (workgroup u builds unit u)
+template <int DMAX>
+__global__ __launch_bounds__(PG_UNIT) void k_plist(const int* __restrict__ nbr, int n, int dpad, int sp, int P,
+                                                   u32* __restrict__ list, int* __restrict__ poff)
+{
+  PlMove none;
+  none.cnt = 0;
+  plist_unit<DMAX>(nbr, n, dpad, sp, P, list, poff, blockIdx.x, none);
+}
+
+// A 2-opt move at positions the host found (with k_swap's rule, on its copy
+// of the adjacency, which it also checked the move on): workgroup 0 stores
+// the mv.cnt entries into the adjacency and the tables; workgroup i < units.cnt
+// rebuilds the pair lists of unit units.u[i], applying the moved entries to
+// its rows as it reads them.  With mv.cnt = 0 it only rebuilds the units.
+struct PlUnits { int u[4]; int cnt; };
+struct PlArgs { const int* nbr; int n, dpad, sp, P; u32* list; int* poff; };
+
+template <int DMAX>
+__global__ __launch_bounds__(PG_UNIT) void k_swap_at(int* adj, int degree, SwapTables tb, PlMove mv, PlUnits units,
+                                                     PlArgs pl)
+{
+  if (blockIdx.x == 0 && (int)threadIdx.x < mv.cnt) {
+    const int e = threadIdx.x;
+    adj[(int64_t)mv.row[e] * degree + mv.pos[e]] = mv.to[e];
+    const int64_t k = (int64_t)mv.row[e] * tb.tdp + mv.pos[e];
+    if (tb.t16 && tb.t16[k] == (unsigned short)mv.from[e]) tb.t16[k] = (unsigned short)mv.to[e];
+    if (tb.t32 && tb.t32[k] == mv.from[e]) tb.t32[k] = mv.to[e];
+  }
+  if ((int)blockIdx.x < units.cnt)
+    plist_unit<DMAX>(pl.nbr, pl.n, pl.dpad, pl.sp, pl.P, pl.list, pl.poff, units.u[blockIdx.x], mv);
 }
 
 typedef u32 u32x4v __attribute__((ext_vector_type(4)));
