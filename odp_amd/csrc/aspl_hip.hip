@@ -52,6 +52,8 @@ static const size_t LDS_LIMIT = 163840;         // LDS a gfx950 workgroup may al
 static const size_t LDS_STATIC = 512;           // k_resident's own __shared__ data, rounded up
 static const size_t LDS_DYN_MAX = LDS_LIMIT - LDS_STATIC;
 static const double RES_SPIN_S = 200e-6;         // a resident evaluation's wait watches its result this long
+static const double LV_SPIN_S = 20e-3;           // a level batch's wait watches its sequence number this long
+static const int LVFIN_MAX = 16384;              // counters k_lvfin hands over (more: copies)
 static const int RING = 260;                    // per-level counter slots kept on the device
 static const int MAX_BATCH_LEVELS = 256;        // levels launched between two host reads (< RING - 2)
 static const int PG_PARTS = 4;                  // row parts of the XCD-phased gather (2 MiB of a 65536-row slab)
@@ -281,7 +283,8 @@ struct LevelRun {
   u32* plist = nullptr; int* ppoff = nullptr;  // per unit of PG_UNIT rows: pairs grouped by part, offsets
   size_t plist_cap = 0, ppoff_cap = 0;
   LevelTask* tasks = nullptr; int* wg_start = nullptr;
-  u64* h_cnt = nullptr;
+  u64* h_cnt = nullptr;                 // coherent host memory (k_lvfin writes it)
+  u64* h_cnt_dev = nullptr;
   std::vector<int> l;                   // sources per task
 };
 
@@ -336,6 +339,12 @@ struct Evaluator {
   // the level plan's pair lists (k_plist) match its table (device swaps rebuild
   // the units of the rows they change); meaningful while table_fresh holds
   bool pl_fresh = false;
+  // a batch of levels ends with k_lvfin: counters and error flag in host memory,
+  // then its sequence number (lvseq: the last one enqueued; d_lvseq: the device's)
+  LvHost* lvh = nullptr;
+  LvHost* lvh_dev = nullptr;
+  u32* d_lvseq = nullptr;
+  u32 lvseq = 0;
   // table entries the next resident launch changes first (triples row, position, id)
   std::vector<int> pend_patch;
   // h_stage holds what the device copy holds (or will, once the pending entries are stored):
@@ -488,6 +497,8 @@ struct Evaluator {
     hfree(&h_stage); hfree(&h_err); h_stage_dev = nullptr; adj_on_host = false;
     std::vector<int>().swap(h_mirror);
     dfree(&lr.bufs); dfree(&lr.cnt); dfree(&lr.done); dfree(&lr.tasks); dfree(&lr.wg_start); hfree(&lr.h_cnt);
+    lr.h_cnt_dev = nullptr;
+    hfree(&lvh); lvh_dev = nullptr; dfree(&d_lvseq); lvseq = 0;
     dfree(&lr.flags); hfree(&lr.h_acnt); dfree(&lr.wgm); dfree(&lr.list); dfree(&lr.list_n); dfree(&lr.tq);
     dfree(&lr.plist); dfree(&lr.ppoff);
     dfree(&rr.tasks); dfree(&rr.out); dfree(&rr.qhead); dfree(&rr.done); hfree(&rr.host); rr.host_dev = nullptr;
@@ -824,7 +835,7 @@ struct Evaluator {
       dalloc(&r.done, (size_t)ntasks + 1);          // done flags, then the table's asymmetry flag
       hfree(&r.h_acnt); halloc(&r.h_acnt, (size_t)RING * ntasks);
       dalloc(&r.tasks, (size_t)ntasks); dalloc(&r.wg_start, (size_t)ntasks + 1);
-      halloc(&r.h_cnt, (size_t)RING * ntasks);
+      halloc_coherent(&r.h_cnt, &r.h_cnt_dev, (size_t)RING * ntasks);
       r.task_cap = ntasks;
     }
     // the previous contents are never read: level 0 rewrites every word it uses
@@ -1015,6 +1026,7 @@ struct Evaluator {
     int tside = TILE_MIN;               // k_tile's tile side (halo = (64 - side) / 2)
     int spec = -1;                      // level of the current group launched count-only (expected last)
     bool graphed = false;
+    bool pend_fin = false;              // the enqueued batch ends with k_lvfin (sequence number lvseq)
     bool fresh = false, plfresh = false; // the preparation keeps the table / the pair lists
     int first = 0, cnt_t = 0;
     int launched = 0, examined = 0, running = 0, batch_end = 0;
@@ -1069,7 +1081,7 @@ struct Evaluator {
     LevelRun& r = lr;
     if (lv.tk) {
       for (int L = from; L < to; L += lv.tk) lv_tile(L);
-      lv_copies();
+      lv_copies(to);
       return;
     }
     const bool listed = !lv.fr && lv.spec - 1 >= lv.L1 && r.ntasks < 65536 && n < (1 << 24) && r.kwp < (1 << 24);
@@ -1083,7 +1095,7 @@ struct Evaluator {
       }
       lv_launch(L, mode);
     }
-    lv_copies();
+    lv_copies(to);
   }
 
   // levels L .. L+tk-1 of the current group by temporal tiles (k_tile)
@@ -1153,9 +1165,29 @@ struct Evaluator {
     timed_end(tl);
   }
 
-  void lv_copies()
+  // the end of a batch that ends at level `to` (the counters of levels
+  // lv.examined .. to-1 are read): k_lvfin on the library's stream, copies and a
+  // stream synchronisation otherwise (timed passes, serving slots, long batches)
+  bool lv_fin_ok(int from, int to) const
+  {
+    return !timed() && st == C.stream && lvh && (int64_t)(to - from) * lr.ntasks <= LVFIN_MAX;
+  }
+  void note_fin(bool fin)
+  {
+    lv.pend_fin = fin;
+    if (fin) lvseq++;
+  }
+  void lv_copies(int to)
   {
     LevelRun& r = lr;
+    if (lv_fin_ok(lv.examined, to)) {
+      hipLaunchKernelGGL(k_lvfin, dim3(1), dim3(256), 0, st, r.cnt, r.ntasks, lv.examined, to, RING, r.h_cnt_dev,
+                         d_err, lvh_dev, d_lvseq);
+      HIPC(hipGetLastError());
+      note_fin(true);
+      return;
+    }
+    note_fin(false);
     HIPC(hipMemcpyAsync(r.h_cnt, r.cnt, sizeof(u64) * RING * r.ntasks, hipMemcpyDeviceToHost, st));
     HIPC(hipMemcpyAsync(h_err, d_err, sizeof(int), hipMemcpyDeviceToHost, st));
     if (timed())
@@ -1178,6 +1210,8 @@ struct Evaluator {
       const int be = lv.batch_end, L1 = lv.L1, key = lv_key(be);
       if (!graph_ready(key))
         capture([&]() { lv_prepare(); lv_level0(); lv_levels(L1, be); }, key);
+      else
+        note_fin(lv_fin_ok(0, be));       // the replay ends as the captured batch did
       HIPC(hipGraphLaunch(gexec, st));
     } else {
       lv_level0();
@@ -1252,8 +1286,16 @@ struct Evaluator {
     flush_adj();
     LevelRun& r = lr;
     trace("run_level");
-    // the table (and the pair lists) the last evaluation built still match the adjacency
-    lv.fresh = table_fresh && table_src == cur_adj();
+    if (!lvh && st == C.stream) {
+      halloc_coherent(&lvh, &lvh_dev, 1);
+      dalloc(&d_lvseq, 1);
+      HIPC(hipMemsetAsync(d_lvseq, 0, sizeof(u32), st));
+      lvh->seq = 0; lvh->err = 0;
+      lvseq = 0;
+    }
+    // the table (and the pair lists) the last evaluation built still match the
+    // adjacency (never assumed of a caller's device array: it may have changed)
+    lv.fresh = table_fresh && table_src == cur_adj() && !ext_adj;
     if (!lv.fresh) pl_fresh = false;
     lv.K = r.direct;                      // levels 0..K-1 come from k_direct (0: k_level0 does level 0)
     lv.L1 = lv.K ? lv.K : 1;              // first level the gather kernel runs
@@ -1330,11 +1372,29 @@ struct Evaluator {
     if (timed()) flush_level_stats(lv.sts.data(), 0, 0);
   }
 
+  // waits for the enqueued batch: its sequence number in host memory, watched
+  // for up to LV_SPIN_S, else (and without one) a stream synchronisation
+  void lv_wait()
+  {
+    bool seen = false;
+    if (lv.pend_fin) {
+      const volatile u32* sq = &lvh->seq;
+      const double t0 = wall_time();
+      for (int i = 0; !seen; i++) {
+        seen = *sq == lvseq;
+        if (!seen && (i & 255) == 255 && wall_time() - t0 > LV_SPIN_S) break;
+      }
+      if (seen) std::atomic_thread_fence(std::memory_order_acquire);
+    }
+    if (!seen) HIPC(hipStreamSynchronize(st));
+    if (lv.pend_fin) *h_err = lvh->err;
+  }
+
   void run_level(u64* excess, int* diam, bool* disc)
   {
     lv_begin();
     for (;;) {
-      HIPC(hipStreamSynchronize(st));
+      lv_wait();
       trace("level batch synchronized");
       check_err();
       if (lv_step()) break;

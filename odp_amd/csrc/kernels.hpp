@@ -1754,6 +1754,36 @@ __device__ __forceinline__ void res_commit(ResOut* out, int graph, const ResOut&
   }
 }
 
+// End of a batch of levels as the host reads it (pinned, coherent host memory).
+struct LvHost {
+  int err;          // the device error flag as it stood at the end of the batch
+  u32 seq;          // the batch's sequence number, written after err and the counters
+};
+
+// The counters of levels [from, to) (ring slots L % ring of every task) and
+// the error flag, stored into host memory; then the next sequence number
+// (kept on the device, so that a replayed graph advances it too), which the
+// host watches for instead of synchronising the stream.
+__global__ __launch_bounds__(256) void k_lvfin(const u64* __restrict__ cnt, int ntasks, int from, int to, int ring,
+                                              u64* hcnt, const int* err, LvHost* host, u32* dseq)
+{
+  const int m = (to - from) * ntasks;
+  for (int i = threadIdx.x; i < m; i += blockDim.x) {
+    const int L = from + i / ntasks, t = i - (i / ntasks) * ntasks;
+    const int64_t k = (int64_t)(L % ring) * ntasks + t;
+    hcnt[k] = cnt[k];
+  }
+  __threadfence_system();                          // each thread's stores reach host memory first
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    host->err = *err;
+    const u32 sq = *dseq + 1;
+    *dseq = sq;
+    __threadfence_system();
+    __hip_atomic_store(&host->seq, sq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
+}
+
 // Result of a single-graph launch as the host reads it (pinned, coherent host memory).
 struct ResHost {
   ResOut out;
