@@ -586,12 +586,21 @@ __device__ __forceinline__ void plist_unit(const int* __restrict__ nbr, int n, i
 {
   __shared__ u64 wtot[PG_UNIT / 64];
   const int t = threadIdx.x, r = u * PG_UNIT + t, lane = t & 63, wave = t >> 6;
+  // the row's table entries, 16 bytes a load (dpad is a multiple of 4), every
+  // thread loading (a row past n reads the last row's): a load under a lane
+  // mask would make the compiler wait for each one before the next
   int id[DMAX];
+  const int4* src = (const int4*)(nbr + (int64_t)(r < n ? r : n - 1) * dpad);
 #pragma unroll
-  for (int j = 0; j < DMAX; j++) {
-    id[j] = -1;
-    if (r < n && j <= dpad) id[j] = j < dpad ? nbr[(int64_t)r * dpad + j] : r;
+  for (int j4 = 0; j4 < DMAX / 4; j4++) {
+    int4 v = make_int4(-1, -1, -1, -1);
+    if (4 * j4 < dpad) v = src[j4];                 // uniform
+    id[4 * j4] = v.x; id[4 * j4 + 1] = v.y; id[4 * j4 + 2] = v.z; id[4 * j4 + 3] = v.w;
   }
+#pragma unroll
+  for (int j = 0; j < DMAX; j++)
+    if (r >= n || j > dpad) id[j] = -1;
+    else if (j == dpad) id[j] = r;
   for (int e = 0; e < mv.cnt; e++)
     if (r == mv.row[e]) {
 #pragma unroll
