@@ -1254,8 +1254,7 @@ struct Evaluator {
     LevelRun& r = lr;
     prepare(r.cnt, lv.nzero, r.done, r.ntasks + 1, !lv.fresh);
     if (lv.tk) {
-      const int64_t total = (int64_t)n * dpad;
-      const int grid = (int)std::min<int64_t>(std::max<int64_t>(1, (total + 255) / 256), 4096);
+      const int grid = std::min(std::max(1, (n + 255) / 256), 4096);     // a thread per cell
       hipLaunchKernelGGL(k_tgeom, dim3(grid), dim3(256), 0, st, d_nbr, n, dpad, d_row2xy, geo_lmax, d_toff, d_err);
       HIPC(hipGetLastError());
     }

@@ -1060,6 +1060,7 @@ __global__ __launch_bounds__(LEVEL_WG) void k_sparse(const LevelTask* __restrict
 #define TILE_THREADS 1024
 #define TILE_PER_THREAD (TILE_SLOTS / TILE_THREADS)
 #define TILE_KMAX 8
+#define TILE_DPMAX 16                             // widest table k_tile takes
 
 struct TileArgs {
   const int* xy2row;     // cell x * height + y -> row
@@ -1076,15 +1077,41 @@ struct TileArgs {
 
 // cell offsets of the row-neighbour table for k_tile; an entry longer than
 // lmax (which the host measured on the adjacency) is an internal error
+// One thread per cell.  A cell's offsets are stored in order of their y
+// offset: k_tile's lanes are consecutive y of one region column and read
+// neighbour j together, and a region slot's LDS banks follow its y (rows are
+// TILE_RW slots apart), so lanes whose j-th neighbours share a y offset read
+// distinct banks; random order put several lanes on one bank.
 __global__ void k_tgeom(const int* __restrict__ nbr, int n, int dpad, const int* __restrict__ row2xy, int lmax,
                         short* __restrict__ toff, int* err)
 {
-  const int64_t total = (int64_t)n * dpad;
-  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
-    const int a = row2xy[i / dpad], b = row2xy[nbr[i]];
-    int dx = (b >> 16) - (a >> 16), dy = (b & 0xffff) - (a & 0xffff);
-    if (dx > lmax || dx < -lmax || dy > lmax || dy < -lmax) { atomicOr(err, 8); dx = 0; dy = 0; }
-    toff[i] = (short)(dx * TILE_RW + dy);
+  for (int r = blockIdx.x * blockDim.x + threadIdx.x; r < n; r += gridDim.x * blockDim.x) {
+    const int a = row2xy[r];
+    int o[TILE_DPMAX], key[TILE_DPMAX];
+#pragma unroll
+    for (int j = 0; j < TILE_DPMAX; j++) {
+      o[j] = 0;
+      key[j] = INT_MAX;                                      // past the width: sorts last
+      if (j < dpad) {
+        const int b = row2xy[nbr[(int64_t)r * dpad + j]];
+        int dx = (b >> 16) - (a >> 16), dy = (b & 0xffff) - (a & 0xffff);
+        if (dx > lmax || dx < -lmax || dy > lmax || dy < -lmax) { atomicOr(err, 8); dx = 0; dy = 0; }
+        o[j] = dx * TILE_RW + dy;
+        key[j] = dy;
+      }
+    }
+    // odd-even transposition sort by y offset (every index fixed: registers only)
+#pragma unroll
+    for (int pass = 0; pass < TILE_DPMAX; pass++)
+#pragma unroll
+      for (int j = pass & 1; j + 1 < TILE_DPMAX; j += 2)
+        if (key[j] > key[j + 1]) {
+          const int tk = key[j]; key[j] = key[j + 1]; key[j + 1] = tk;
+          const int to = o[j]; o[j] = o[j + 1]; o[j + 1] = to;
+        }
+#pragma unroll
+    for (int j = 0; j < TILE_DPMAX; j++)
+      if (j < dpad) toff[(int64_t)r * dpad + j] = (short)o[j];
   }
 }
 
