@@ -356,6 +356,38 @@ def test_level_table_follows_swaps(plan):
         odp.finalize_aspl()
 
 
+def test_long_level_sequences_end_their_batches():
+    """A batch of levels ends with the counters handed to host memory by a
+    kernel that then writes a sequence number (small batches), or with copies
+    and a stream synchronisation (batches of more than 16384 counters, and
+    timed passes).  A 20000-cycle (diameter 10000) under a 1-word plan (313
+    blocks, batches growing to 256 levels) takes both; a path and an
+    unchanged-graph repeat check the sequence numbers stay in step."""
+    n = 20000
+    adj = cycle_graph(n)
+    want = (n // 2, cycle_sum(n))
+    for plan in (dict(plan=LEV, kw=1, upw=16, seg_major=0, xcd=1), dict(plan=LEV, kw=16, upw=8, seg_major=0, xcd=1)):
+        odp.init_aspl_general(n, 2)
+        try:
+            odp.set_plan(**plan)
+            for _ in range(3):
+                got = odp.set_aspl(adj)
+                assert got[:2] == want, (plan, got)
+            odp.stats_enable(True)                    # timed: the copies and a synchronisation
+            assert odp.set_aspl(adj)[:2] == want
+            odp.stats_enable(False)
+            assert odp.set_aspl(adj)[:2] == want
+        finally:
+            odp.finalize_aspl()
+    path, nd = path_graph(3000)
+    odp.init_aspl_general(3000, path.shape[1], nd)
+    try:
+        for _ in range(2):
+            assert odp.set_aspl(path)[:2] == (2999, 2999 * 3000 * 3001 // 6)
+    finally:
+        odp.finalize_aspl()
+
+
 def test_drop_in_small_changes_as_patches():
     """ODP_Set_aspl on an array that differs from the last upload in a few
     entries (the annealing pattern: a move, then Set_aspl on the same array)
