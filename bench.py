@@ -571,6 +571,20 @@ def main():
     pcie_rate = 10 / (time.perf_counter() - t)
     if (r[0], r[1]) != (first[0], first[1]):
         raise SystemExit(f"PCIe-inclusive calls gave {r}, not {first}")
+    # The reference's annealing pattern through the drop-in call: a 2-opt move on
+    # the caller's array, then ODP_Set_aspl on it (the library sends the changed
+    # entries, not the table); the last result joins the reference checks.
+    dropin_rate, dropin_check = None, None
+    if chain:
+        arr = adj.copy()
+        rng_d = odp.Rng(args.seed + 777)
+        odp.set_aspl(arr)
+        t = time.perf_counter()
+        for i in range(20):
+            odp.two_opt(arr, rng_d)
+            r_d = odp.set_aspl(arr)
+        dropin_rate = 20 / (time.perf_counter() - t)
+        dropin_check = (arr.copy(), r_d)
     odp.finalize_aspl()
 
     # The dominant kernel: the gather level (k_tile's launches of several levels
@@ -630,12 +644,13 @@ def main():
                       "diameter": first[0], "sum": first[1], "aspl": first[2],
                       "last_step_result": list(results[-1]),
                       "same_graph_evals_per_s": round(args.steps / el_same, 3),
-                      "pcie_inclusive_evals_per_s": round(pcie_rate, 3)},
+                      "pcie_inclusive_evals_per_s": round(pcie_rate, 3),
+                      "dropin_move_evals_per_s": round(dropin_rate, 3) if dropin_rate else None},
            "roofline": roofline}
     checks = []
     if chain:
-        # every timed step against the reference (bit for bit)
-        checks = [(g, r) for g, r in zip(states, results)]
+        # every timed step against the reference (bit for bit), and the drop-in pattern's last call
+        checks = [(g, r) for g, r in zip(states, results)] + [dropin_check]
     if not args.no_extras and args.workload == "random65536":
         bchecks, out["batched"] = batched_extra(1024, 3)
         checks += bchecks

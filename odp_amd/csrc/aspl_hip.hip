@@ -385,15 +385,11 @@ struct Evaluator {
     return nullptr;
   }
 
-  // A 2-opt move a-b, c-d -> a-c, b-d on the device copy.  A current table is
-  // patched with it: the resident plan's 16-bit table, or the level plan's
-  // 32-bit table and 16-bit copy, whose pair lists are then rebuilt for the
-  // units of the four rows (at most four workgroups).  Otherwise the next
-  // evaluation rebuilds the table.  With the positions of the four entries
-  // known (pos: found on the host copy by k_swap's rule, the move checked
-  // there) one launch stores them and rebuilds the units (k_swap_at);
-  // otherwise k_swap finds them (all or nothing) and k_swap_at rebuilds.
-  void swap_launch(int a, int b, int c, int d, const int* pos = nullptr)
+  // The tables a change of the device adjacency patches along with it: the
+  // resident plan's 16-bit table, or the level plan's 32-bit table and 16-bit
+  // copy (whose pair lists the change's units then rebuild), when current;
+  // otherwise none, and the next evaluation rebuilds the table.
+  SwapTables patch_tables()
   {
     SwapTables tb{};
     const bool cur = table_fresh && table_src == d_adj && s == 1 && !grid_s;
@@ -404,18 +400,40 @@ struct Evaluator {
     } else {
       table_fresh = false;
     }
-    const int rows[4] = {a, b, c, d}, from[4] = {b, a, d, c}, to[4] = {c, d, a, b};
+    return tb;
+  }
+
+  // k_swap_at: the entries of mv (none: the units alone) into the adjacency and
+  // the tables, and the pair lists of the units of rows[0..nrows) rebuilt when
+  // the level plan's lists are current
+  void entries_launch(const SwapTables& tb, const PlMove& mv, const int* rows, int nrows)
+  {
     PlUnits units{};
     PlArgs pl{};
     if (tb.t32 && pl_fresh) {
-      for (int r : rows) {
-        const int u = r / PG_UNIT;
+      for (int q = 0; q < nrows; q++) {
+        const int u = rows[q] / PG_UNIT;
         bool seen = false;
         for (int i = 0; i < units.cnt; i++) seen = seen || units.u[i] == u;
         if (!seen) units.u[units.cnt++] = u;
       }
       pl.nbr = d_nbr; pl.n = n; pl.dpad = dpad; pl.sp = lr.psp; pl.P = lr.parts; pl.list = lr.plist; pl.poff = lr.ppoff;
     }
+    if (!mv.cnt && !units.cnt) return;
+    auto kern = dpad + 1 <= 20 ? k_swap_at<20> : dpad + 1 <= 36 ? k_swap_at<36> : k_swap_at<64>;
+    hipLaunchKernelGGL(kern, dim3(std::max(1, units.cnt)), dim3(PG_UNIT), 0, C.stream, d_adj, degree, tb, mv, units, pl);
+    HIPC(hipGetLastError());
+  }
+
+  // A 2-opt move a-b, c-d -> a-c, b-d on the device copy, with the current
+  // tables patched (patch_tables).  With the positions of the four entries
+  // known (pos: found on the host copy by k_swap's rule, the move checked
+  // there) one launch stores them and rebuilds the units (k_swap_at);
+  // otherwise k_swap finds them (all or nothing) and k_swap_at rebuilds.
+  void swap_launch(int a, int b, int c, int d, const int* pos = nullptr)
+  {
+    const SwapTables tb = patch_tables();
+    const int rows[4] = {a, b, c, d}, from[4] = {b, a, d, c}, to[4] = {c, d, a, b};
     PlMove mv{};
     if (pos) {
       for (int e = 0; e < 4; e++) { mv.row[e] = rows[e]; mv.pos[e] = pos[e]; mv.from[e] = from[e]; mv.to[e] = to[e]; }
@@ -423,11 +441,8 @@ struct Evaluator {
     } else {
       hipLaunchKernelGGL(k_swap, dim3(1), dim3(64), 0, C.stream, d_adj, degree, a, b, c, d, d_err, tb);
       HIPC(hipGetLastError());
-      if (!units.cnt) return;
     }
-    auto kern = dpad + 1 <= 20 ? k_swap_at<20> : dpad + 1 <= 36 ? k_swap_at<36> : k_swap_at<64>;
-    hipLaunchKernelGGL(kern, dim3(std::max(1, units.cnt)), dim3(PG_UNIT), 0, C.stream, d_adj, degree, tb, mv, units, pl);
-    HIPC(hipGetLastError());
+    entries_launch(tb, mv, rows, 4);
   }
   ODPX_Plan plan{};
   LevelRun lr;
@@ -607,7 +622,9 @@ struct Evaluator {
   // -- changes four entries.
   bool upload_as_patch(const int* a)
   {
-    if (plan.plan != ODPX_PLAN_RESIDENT || s != 1 || grid_s || has_numdeg || ext_adj || adj_on_host) return false;
+    const bool lev = plan.plan == ODPX_PLAN_LEVEL && !bfs;   // the BFS kind rebuilds its table every run
+    if ((plan.plan != ODPX_PLAN_RESIDENT && !lev) || s != 1 || grid_s || has_numdeg || ext_adj || adj_on_host)
+      return false;
     if (!stage_is_device || !table_fresh || table_src != d_adj || !pend_patch.empty()) return false;
     const size_t m = adj_elems(), chunk = 64;
     int idx[RES_PATCH_MAX];
@@ -622,11 +639,25 @@ struct Evaluator {
         }
     }
     mirror_ok = false;
-    for (int q = 0; q < k; q++) {
-      const int i = idx[q];
-      h_stage[i] = a[i];
-      const int e[3] = {i / degree, i % degree, a[i]};
-      pend_patch.insert(pend_patch.end(), e, e + 3);
+    if (lev) {
+      // the level plan stores them now (k_swap_at: adjacency, tables, units)
+      static_assert(RES_PATCH_MAX <= PL_MOVE_MAX, "a small change fits one launch");
+      PlMove mv{};
+      int rows[RES_PATCH_MAX];
+      for (int q = 0; q < k; q++) {
+        const int i = idx[q];
+        mv.row[q] = rows[q] = i / degree; mv.pos[q] = i % degree; mv.from[q] = h_stage[i]; mv.to[q] = a[i];
+        h_stage[i] = a[i];
+      }
+      mv.cnt = k;
+      entries_launch(patch_tables(), mv, rows, k);
+    } else {
+      for (int q = 0; q < k; q++) {
+        const int i = idx[q];
+        h_stage[i] = a[i];
+        const int e[3] = {i / degree, i % degree, a[i]};
+        pend_patch.insert(pend_patch.end(), e, e + 3);
+      }
     }
     if (d_xy2row && k) geo_lmax = grid_lmax(h_stage);
     return true;

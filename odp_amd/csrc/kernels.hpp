@@ -574,10 +574,11 @@ struct PgArgs {
 // grouped by part (in any order within a part) with the part offsets.
 // DMAX >= dpad + 1 bounds the ids a row holds in registers.
 //
-// A device swap hands the moved entries to the units it rebuilds (rows,
-// positions in the row, old and new ids), so that they see them whether or
-// not the table already holds them.
-struct PlMove { int row[4], pos[4], from[4], to[4]; int cnt; };
+// A device swap (or a small change of an uploaded table) hands the changed
+// entries to the units it rebuilds (rows, positions in the row, old and new
+// ids), so that they see them whether or not the table already holds them.
+#define PL_MOVE_MAX 8
+struct PlMove { int row[PL_MOVE_MAX], pos[PL_MOVE_MAX], from[PL_MOVE_MAX], to[PL_MOVE_MAX]; int cnt; };
 
 // one unit's pair lists (a workgroup of PG_UNIT threads, unit u)
 template <int DMAX>
@@ -676,12 +677,13 @@ __global__ __launch_bounds__(PG_UNIT) void k_plist(const int* __restrict__ nbr, 
   plist_unit<DMAX>(nbr, n, dpad, sp, P, list, poff, blockIdx.x, none);
 }
 
-// A 2-opt move at positions the host found (with k_swap's rule, on its copy
-// of the adjacency, which it also checked the move on): workgroup 0 stores
-// the mv.cnt entries into the adjacency and the tables; workgroup i < units.cnt
-// rebuilds the pair lists of unit units.u[i], applying the moved entries to
-// its rows as it reads them.  With mv.cnt = 0 it only rebuilds the units.
-struct PlUnits { int u[4]; int cnt; };
+// Entries at positions the host found (a 2-opt move, checked with k_swap's
+// rule on the host's copy of the adjacency, or the entries an upload changed):
+// workgroup 0 stores the mv.cnt entries into the adjacency and the tables;
+// workgroup i < units.cnt rebuilds the pair lists of unit units.u[i],
+// applying the entries to its rows as it reads them.  With mv.cnt = 0 it only
+// rebuilds the units.
+struct PlUnits { int u[PL_MOVE_MAX]; int cnt; };
 struct PlArgs { const int* nbr; int n, dpad, sp, P; u32* list; int* poff; };
 
 template <int DMAX>

@@ -431,6 +431,52 @@ def test_drop_in_small_changes_as_patches():
         odp.finalize_aspl()
 
 
+@pytest.mark.parametrize("plan", [
+    dict(plan=LEV, kw=16, upw=8, seg_major=0, xcd=1, direct=-1, parts=4),
+    dict(plan=LEV, kw=4, upw=4, seg_major=0, xcd=1, direct=3),
+])
+def test_level_plan_small_changes_as_patches(plan):
+    """The level plan takes ODP_Set_aspl on an array that differs from the last
+    upload in at most 8 entries as a patch: the entries, the row-neighbour
+    tables and the row-part lists of their units are updated in one launch, no
+    table is sent or rebuilt.  Moves, restores, an unchanged array, two moves
+    at once, three (a full upload), a device swap in between (the staging copy
+    no longer mirrors the device) and moves after it must all give the
+    oracle's result."""
+    n, d = 3000, 6
+    rng = odp.Rng(8)
+    host = odp.generate_regular(n, d, 63)
+    odp.init_aspl_general(n, d)
+    try:
+        odp.set_plan(**plan)
+        for _ in range(2):                               # frontier first, then the plain levels
+            assert same(odp.set_aspl(host), oracle.evaluate(host, nodes=n))
+        for step in range(10):
+            saved = host.copy()
+            odp.two_opt(host, rng)                       # four entries change
+            assert same(odp.set_aspl(host), oracle.evaluate(host, nodes=n)), step
+            if step % 3 == 2:
+                host = saved                             # and back
+                assert same(odp.set_aspl(host), oracle.evaluate(host, nodes=n)), step
+        assert same(odp.set_aspl(host), oracle.evaluate(host, nodes=n))      # unchanged
+        odp.two_opt(host, rng)
+        odp.two_opt(host, rng)
+        assert same(odp.set_aspl(host), oracle.evaluate(host, nodes=n))      # up to eight entries
+        for _ in range(3):
+            odp.two_opt(host, rng)
+        assert same(odp.set_aspl(host), oracle.evaluate(host, nodes=n))      # more: full upload
+        dev = host.copy()
+        a, b, c, dd = odp.two_opt(dev, rng)
+        assert odp.swap_device(a, b, c, dd) == 0
+        assert same(odp.set_aspl_resident(), oracle.evaluate(dev, nodes=n))
+        assert same(odp.set_aspl(host), oracle.evaluate(host, nodes=n))      # device had moved on
+        for step in range(3):
+            odp.two_opt(host, rng)
+            assert same(odp.set_aspl(host), oracle.evaluate(host, nodes=n)), step
+    finally:
+        odp.finalize_aspl()
+
+
 def test_small_upload_read_by_table_build():
     """Under the resident plan a small table reaches the device through the
     table build reading pinned host memory (no DMA); two uploads in a row, a
