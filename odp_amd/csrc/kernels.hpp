@@ -702,6 +702,7 @@ __global__ __launch_bounds__(PG_UNIT) void k_swap_at(int* adj, int degree, SwapT
 }
 
 typedef u32 u32x4v __attribute__((ext_vector_type(4)));
+typedef u32 u32x2v __attribute__((ext_vector_type(2)));
 
 // Persistent: workgroup b runs on XCD b % 8 and takes every (gridDim.x / 8)-th
 // unit of its XCD's contiguous range of units (a unit: PG_UNIT rows of one
@@ -1596,6 +1597,43 @@ __device__ __forceinline__ void row_or(RowT<WB>& r, const u32* cur, int v)
   for (int k = 0; k < WB; k++) r.w[k] |= a.w[k];
 }
 
+// The two rows a 32-bit word of 16-bit ids names, ORed into r: their LDS byte
+// addresses id * 4 WB + base each take one v_mad_u32_u16 (op_sel picks the
+// high half), where the compiler's own extract, shift and add took two or
+// three instructions -- the pulled levels are bound by VALU and LDS issue.
+typedef __attribute__((address_space(3))) const u32 lds_u32_t;
+template <int WB>
+__device__ __forceinline__ RowT<WB> lds_row_at(u32 addr)
+{
+  typedef __attribute__((address_space(3))) const u32x2v lds_u32x2_t;
+  typedef __attribute__((address_space(3))) const u32x4v lds_u32x4_t;
+  RowT<WB> r;
+  if constexpr (WB == 1) {
+    r.w[0] = *(lds_u32_t*)(uintptr_t)addr;
+  } else if constexpr (WB == 2) {
+    const u32x2v v = *(lds_u32x2_t*)(uintptr_t)addr;
+    r.w[0] = v.x; r.w[1] = v.y;
+  } else {
+#pragma unroll
+    for (int k = 0; k < WB; k += 4) {
+      const u32x4v v = *(lds_u32x4_t*)(uintptr_t)(addr + 4u * k);
+      r.w[k] = v.x; r.w[k + 1] = v.y; r.w[k + 2] = v.z; r.w[k + 3] = v.w;
+    }
+  }
+  return r;
+}
+
+template <int WB>
+__device__ __forceinline__ void row_or2(RowT<WB>& r, u32 base, u32 word)
+{
+  u32 lo, hi;
+  asm("v_mad_u32_u16 %0, %1, %2, %3" : "=v"(lo) : "v"(word), "v"((u32)(4 * WB)), "v"(base));
+  asm("v_mad_u32_u16 %0, %1, %2, %3 op_sel:[1,0,0,0]" : "=v"(hi) : "v"(word), "v"((u32)(4 * WB)), "v"(base));
+  const RowT<WB> a = lds_row_at<WB>(lo), b = lds_row_at<WB>(hi);
+#pragma unroll
+  for (int k = 0; k < WB; k++) r.w[k] |= a.w[k] | b.w[k];
+}
+
 // No divergent branch may sit in a loop that contains a barrier.  With an
 // `if (lane == 0)` around the queue pop, the compiler structurized the block
 // loop per lane: lane 0 left the loop to pop the next block while the other
@@ -1622,12 +1660,15 @@ __device__ __forceinline__ u64 block_total(u64 v, u64* part, int wave, int nw)
 
 // All levels of one block of T.l <= 32*WB sources; returns the block's totals
 // (the same value in every thread).
-template <int WB, bool STOP = false, bool WALK = true>
+// MAD: pulled rows address their neighbours with row_or2 (k_anneal keeps
+// row_or: its larger register set spills more with it)
+template <int WB, bool STOP = false, bool WALK = true, bool MAD = true>
 __device__ __forceinline__ ResOut res_run(const ResTask& T, u32* cur, u64 (*s_part)[16],
                                           int tid, int nth, int lane, int wave, int nw)
 {
   constexpr int RPT = res_rpt(WB);
   const u64 target = (u64)T.n * (u64)T.l;
+  const u32 cbase = (u32)(uintptr_t)(__attribute__((address_space(3))) u32*)cur;   // cur's LDS byte address
   RowT<WB> full;
 #pragma unroll
   for (int k = 0; k < WB; k++) {
@@ -1744,10 +1785,15 @@ __device__ __forceinline__ ResOut res_run(const ResTask& T, u32* cur, u64 (*s_pa
           bool done = false;
           for (int j = 0; j < (T.dq >> 1); j++) {
             const uint4 v = ids[off + j];
-            row_or<WB>(row, cur, v.x & 0xffff); row_or<WB>(row, cur, v.x >> 16);
-            row_or<WB>(row, cur, v.y & 0xffff); row_or<WB>(row, cur, v.y >> 16);
-            row_or<WB>(row, cur, v.z & 0xffff); row_or<WB>(row, cur, v.z >> 16);
-            row_or<WB>(row, cur, v.w & 0xffff); row_or<WB>(row, cur, v.w >> 16);
+            if (MAD) {
+              row_or2<WB>(row, cbase, v.x); row_or2<WB>(row, cbase, v.y);
+              row_or2<WB>(row, cbase, v.z); row_or2<WB>(row, cbase, v.w);
+            } else {
+              row_or<WB>(row, cur, v.x & 0xffff); row_or<WB>(row, cur, v.x >> 16);
+              row_or<WB>(row, cur, v.y & 0xffff); row_or<WB>(row, cur, v.y >> 16);
+              row_or<WB>(row, cur, v.z & 0xffff); row_or<WB>(row, cur, v.z >> 16);
+              row_or<WB>(row, cur, v.w & 0xffff); row_or<WB>(row, cur, v.w >> 16);
+            }
             if (STOP) {                            // a row filled by 8 neighbours reads no more
               done = true;
 #pragma unroll
@@ -1757,8 +1803,12 @@ __device__ __forceinline__ ResOut res_run(const ResTask& T, u32* cur, u64 (*s_pa
           }
           if ((T.dq & 1) && !done) {
             const uint2 v = ((const uint2*)T.nbr)[2 * off + (T.dq - 1)];
-            row_or<WB>(row, cur, v.x & 0xffff); row_or<WB>(row, cur, v.x >> 16);
-            row_or<WB>(row, cur, v.y & 0xffff); row_or<WB>(row, cur, v.y >> 16);
+            if (MAD) {
+              row_or2<WB>(row, cbase, v.x); row_or2<WB>(row, cbase, v.y);
+            } else {
+              row_or<WB>(row, cur, v.x & 0xffff); row_or<WB>(row, cur, v.x >> 16);
+              row_or<WB>(row, cur, v.y & 0xffff); row_or<WB>(row, cur, v.y >> 16);
+            }
           }
         }
         nx[i] = row;
@@ -2116,7 +2166,7 @@ __global__ __launch_bounds__(1024) void k_anneal(AnnealArgs a)
       if (tid == 0) S.stop = 1;
       break;
     }
-    const ResOut b = res_run<WB, false, false>(T, bits, s_part, tid, nth, lane, wave, nw);   // no walks (measured faster)
+    const ResOut b = res_run<WB, false, false, false>(T, bits, s_part, tid, nth, lane, wave, nw);   // no walks (measured faster)
     const int par = k & 1;                             // slots of this step (k is uniform; S.steps is thread 0's)
     if (tid == 0) {
       a.slots[par * G + g] = b;
