@@ -1304,11 +1304,13 @@ struct Evaluator {
   }
 
   // graph key of a first batch ending at be: its length, the kernel variant,
-  // whether its last level is count-only and which tables the preparation builds
+  // whether its last level is count-only, which tables the preparation builds
+  // and, in tile mode, the longest edge k_tgeom and the tiles' window take
   int lv_key(int be) const
   {
-    return 4 * (64 * (16 * (4 * be + (lv.fr ? 1 : 0) + (spec_at(be) ? 2 : 0)) + lv.tk) + lv.tside) +
-           (lv.fresh ? 1 : 0) + (lv.plfresh ? 2 : 0);
+    const int key = 4 * (64 * (16 * (4 * be + (lv.fr ? 1 : 0) + (spec_at(be) ? 2 : 0)) + lv.tk) + lv.tside) +
+                    (lv.fresh ? 1 : 0) + (lv.plfresh ? 2 : 0);
+    return 64 * key + (lv.tk ? std::min(std::max(geo_lmax, 0), 63) : 0);
   }
 
   void lv_begin()

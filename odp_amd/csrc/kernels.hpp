@@ -1274,14 +1274,14 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(const LevelTask* __restri
           if (!(o.x == f0 && o.y == f1))
 #pragma unroll
           for (int j = 0; j < DP / 2; j++) {
-            // the window keeps every neighbour inside the region; the clamp only
-            // keeps the reads in LDS (branch-free, so the reads issue together)
-            const unsigned s0 = (unsigned)(sl + (int)(short)(off[m][j] & 0xffff));
-            const unsigned s1 = (unsigned)(sl + (int)(short)(off[m][j] >> 16));
-            const u64x2 a = src[min(s0, (unsigned)(TILE_SLOTS - 1))], c = src[min(s1, (unsigned)(TILE_SLOTS - 1))];
-            const u64 ma = s0 < TILE_SLOTS ? ~0ULL : 0ULL, mc = s1 < TILE_SLOTS ? ~0ULL : 0ULL;
-            x.x |= (a.x & ma) | (c.x & mc);
-            x.y |= (a.y & ma) | (c.y & mc);
+            // the window keeps every neighbour inside the region: a cell at least
+            // (i + 1) lmax from its edges, and no offset longer than lmax cells
+            // (k_tgeom flags longer ones, which end the run)
+            const int s0 = sl + (int)(short)(off[m][j] & 0xffff);
+            const int s1 = sl + (int)(short)(off[m][j] >> 16);
+            const u64x2 a = src[s0], c = src[s1];
+            x.x |= a.x | c.x;
+            x.y |= a.y | c.y;
           }
           dst[sl] = x;
           if (lx >= g.halo && lx < g.halo + g.tile && lane >= g.halo && lane < g.halo + g.tile)

@@ -161,6 +161,47 @@ def test_tile_swaps_lengthen_halo():
         odp.finalize_aspl()
 
 
+def test_tile_graph_follows_a_longer_edge():
+    """A device swap that makes the longest edge longer without changing the
+    tiles' levels per launch (lmax 9 -> 10..16: one level per launch either
+    way) must not replay the launch sequence captured for the shorter edges:
+    its offsets and window were built for them.  Three evaluations capture and
+    replay the sequence; then swaps add longer edges, each result against the
+    oracle."""
+    w, h, L = 64, 64, 9
+    adj = odp.generate_grid(w, h, 4, L, 1, 21)
+    cheb = lambda p, q: max(abs(p // h - q // h), abs(p % h - q % h))
+    lmax = max(cheb(v, int(u)) for v in range(w * h) for u in adj[v])
+    assert lmax <= L
+    init(w, h, 4, 1)
+    try:
+        odp.set_plan(**TILE_PLAN)
+        for _ in range(3):                              # eager, captured, replayed
+            assert same(odp.set_aspl(adj), oracle.evaluate(adj, width=w, height=h))
+        rng = np.random.default_rng(5)
+        cur = adj.copy()
+        done = 0
+        while done < 3:
+            a = int(rng.integers(w * h))
+            b = int(cur[a][int(rng.integers(4))])
+            c = int(rng.integers(w * h))
+            dd = int(cur[c][int(rng.integers(4))])
+            if len({a, b, c, dd}) < 4 or c in cur[a] or dd in cur[b]:
+                continue
+            longest = max(cheb(a, c), cheb(b, dd))
+            if not (lmax < longest <= 16):
+                continue
+            assert odp.swap_device(a, b, c, dd) == 0
+            for row, old, new in ((a, b, c), (b, a, dd), (c, dd, a), (dd, c, b)):
+                cur[row][list(cur[row]).index(old)] = new
+            lmax = longest
+            done += 1
+            for _ in range(2):
+                assert same(odp.set_aspl_resident(), oracle.evaluate(cur, width=w, height=h)), (done, lmax)
+    finally:
+        odp.finalize_aspl()
+
+
 def test_tile_autotune_includes_tiles():
     """The autotuner times the tile plan beside the others and checks that all
     agree (it exits otherwise)."""
