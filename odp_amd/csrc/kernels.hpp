@@ -1739,19 +1739,36 @@ __device__ __forceinline__ ResOut res_run(const ResTask& T, u32* cur, u64 (*s_pa
     prev = c;
     L++;
     const u32 deg = (u32)T.deg, steps = (u32)L + 1;
+    // a thread takes a walk's first steps - 1 steps (its prefix, decoded with
+    // divisions) and then every last step at once: the whole table row of the
+    // vertex reached, 8 ids a load -- deg walks per decoded prefix
     u32 per = 1;
-    for (u32 i = 0; i < steps; i++) per *= deg;
-    const u32 items = (u32)T.l * per;
-    for (u32 it = (u32)tid; it < items; it += (u32)nth) {
+    for (u32 i = 0; i + 1 < steps; i++) per *= deg;
+    const u32 prefixes = (u32)T.l * per;
+    for (u32 it = (u32)tid; it < prefixes; it += (u32)nth) {
       const u32 c0 = it / per;
       u32 rem = it - c0 * per;
       int v = T.src0 + (int)c0;
-      for (u32 i = 0; i < steps; i++) {
+      for (u32 i = 0; i + 1 < steps; i++) {
         const u32 j = rem % deg;
         rem /= deg;
         v = T.nbr[v * T.dp + (int)j];
       }
-      atomicOr(&cur[v * WB + (int)(c0 >> 5)], 1u << (c0 & 31));
+      u32* col = cur + (int)(c0 >> 5);
+      const u32 b = 1u << (c0 & 31);
+      const uint4* ids = (const uint4*)(T.nbr + v * T.dp);
+      for (int q = 0; q < (T.dq >> 1); q++) {
+        const uint4 g = ids[q];
+        atomicOr(col + (g.x & 0xffff) * WB, b); atomicOr(col + (g.x >> 16) * WB, b);
+        atomicOr(col + (g.y & 0xffff) * WB, b); atomicOr(col + (g.y >> 16) * WB, b);
+        atomicOr(col + (g.z & 0xffff) * WB, b); atomicOr(col + (g.z >> 16) * WB, b);
+        atomicOr(col + (g.w & 0xffff) * WB, b); atomicOr(col + (g.w >> 16) * WB, b);
+      }
+      if (T.dq & 1) {
+        const uint2 g = ((const uint2*)ids)[T.dq - 1];
+        atomicOr(col + (g.x & 0xffff) * WB, b); atomicOr(col + (g.x >> 16) * WB, b);
+        atomicOr(col + (g.y & 0xffff) * WB, b); atomicOr(col + (g.y >> 16) * WB, b);
+      }
     }
     __syncthreads();
     pop = 0;
