@@ -190,7 +190,7 @@ static int res_walk_cap()
 static int res_walk_env(int n, int deg, int l)
 {
   static double cost = -1;
-  if (cost < 0) { const char* v = getenv("ODP_HIP_RES_WALK_COST"); cost = v ? atof(v) : 4.0; }
+  if (cost < 0) { const char* v = getenv("ODP_HIP_RES_WALK_COST"); cost = v ? atof(v) : 1.0; }
   return std::min(res_walk_levels(n, deg, l, cost), res_walk_cap());
 }
 

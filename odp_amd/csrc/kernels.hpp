@@ -1513,12 +1513,14 @@ struct ResTask {
 
 // Levels a block builds from walks out of its l sources instead of pulling
 // every row's neighbours: level L (sources within L + 1 hops) adds the walks
-// of exactly L + 1 steps, l * deg^(L+1) LDS bit-sets with L + 1 dependent table
-// reads each, against n * (deg + 1) row reads for a pulled level; a level is
-// walked while its walks cost less than half a pulled level (counting a table
-// read as two row reads).  Returns the first pulled level (1: level 0 alone,
-// whose l * (deg + 1) walks of at most one step always beat reading the table).
-__host__ __device__ inline int res_walk_levels(int n, int deg, int l, double cost = 4.0)
+// of exactly L + 1 steps, l * deg^(L+1) LDS bit-sets (each decoded prefix of L
+// steps reads one table row for its deg last steps), against n * (deg + 1) row
+// reads for a pulled level; a level is walked while cost * (L + 1) times its
+// walks stay below a pulled level's row reads (cost 1: measured best for the
+// 1024-graph batch, 20.2 ms against 20.3 at 2, 20.55 at 4, 20.5 at 0.5).
+// Returns the first pulled level (1: level 0 alone, whose l * (deg + 1) walks
+// of at most one step always beat reading the table).
+__host__ __device__ inline int res_walk_levels(int n, int deg, int l, double cost = 1.0)
 {
   int K = 1;
   double w = (double)l * deg;                        // walks of one step
