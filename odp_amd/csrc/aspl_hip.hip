@@ -53,7 +53,7 @@ static const size_t LDS_STATIC = 512;           // k_resident's own __shared__ d
 static const size_t LDS_DYN_MAX = LDS_LIMIT - LDS_STATIC;
 static const double RES_SPIN_S = 200e-6;         // a resident evaluation's wait watches its result this long
 static const double LV_SPIN_S = 20e-3;           // a level batch's wait watches its sequence number this long
-static const int LVFIN_MAX = 16384;              // counters k_lvfin hands over (more: copies)
+static const int LVFIN_MAX = 4096;               // counters k_lvfin hands over (more: copies, faster than one workgroup's writes to host memory)
 static const int RING = 260;                    // per-level counter slots kept on the device
 static const int MAX_BATCH_LEVELS = 256;        // levels launched between two host reads (< RING - 2)
 static const int PG_PARTS = 4;                  // row parts of the XCD-phased gather (2 MiB of a 65536-row slab)

@@ -359,7 +359,7 @@ def test_level_table_follows_swaps(plan):
 def test_long_level_sequences_end_their_batches():
     """A batch of levels ends with the counters handed to host memory by a
     kernel that then writes a sequence number (small batches), or with copies
-    and a stream synchronisation (batches of more than 16384 counters, and
+    and a stream synchronisation (batches of more than 4096 counters, and
     timed passes).  A 20000-cycle (diameter 10000) under a 1-word plan (313
     blocks, batches growing to 256 levels) takes both; a path and an
     unchanged-graph repeat check the sequence numbers stay in step."""
