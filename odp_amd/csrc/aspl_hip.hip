@@ -572,6 +572,8 @@ struct Evaluator {
 
   void set_itable(const int* itable)
   {
+    drop_graph();                      // captured launches hold the old table's address
+    table_fresh = false;               // the row-neighbour table follows the row order
     dfree(&d_itable);
     dalloc(&d_itable, (size_t)n);
     HIPC(hipMemcpy(d_itable, itable, sizeof(int) * n, hipMemcpyHostToDevice));
