@@ -1263,7 +1263,7 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(const LevelTask* __restri
       const u64x2* __restrict__ src = rows[i & 1];
       u64x2* __restrict__ dst = rows[(i + 1) & 1];
       const int lo = (i + 1) * g.lmax, hi = TILE_RW - lo;
-      u64 gain = 0;
+      u32 gain = 0;                                  // at most 4 * 128 bits a thread
 #pragma unroll
       for (int m = 0; m < TILE_PER_THREAD; m++) {
         const int lx = wave + m * (TILE_THREADS / 64), sl = lx * TILE_RW + lane;
@@ -1285,11 +1285,11 @@ __global__ __launch_bounds__(TILE_THREADS) void k_tile(const LevelTask* __restri
           }
           dst[sl] = x;
           if (lx >= g.halo && lx < g.halo + g.tile && lane >= g.halo && lane < g.halo + g.tile)
-            gain += (u64)(__popcll(x.x) + __popcll(x.y) - __popcll(o.x) - __popcll(o.y));
+            gain += (u32)(__popcll(x.x ^ o.x) + __popcll(x.y ^ o.y));   // x holds o: its new bits
         }
       }
-      gain = wave_sum(gain);
-      if (lane == 0) red[wave][i] = gain;
+      const u64 wgain = wave_sum((u64)gain);
+      if (lane == 0) red[wave][i] = wgain;
       if (i == 0 && tid == 0) claim_s = claim;
       __syncthreads();                               // level i is in LDS
       // the next unit's loads, one dependent step per level
