@@ -348,6 +348,52 @@ def batched_extra(count, loops):
                         "d in {4,8,12,16,24}, one 2-opt swap each"}
 
 
+BATCH_CLASSES = (("n<=4096", 0, 4096), ("4096<n<=10208", 4096, 10208), ("n>10208", 10208, 1 << 30))
+
+
+def batch_classes():
+    """The 1024-graph batch split by graph size (the 10208 cut is where blocks
+    narrow from 4 to 2 words): per class the batched run against looping the
+    single-graph auto plan, best of 3 (batch) and 2 (loop) runs after warm-up."""
+    params, graphs = c4_params(1024), c4_graphs(1024)
+
+    def best(f, k):
+        out = []
+        for _ in range(k):
+            t = time.perf_counter()
+            f()
+            out.append(time.perf_counter() - t)
+        return min(out)
+    rows, tb_sum, ta_sum = [], 0.0, 0.0
+    for name, lo, hi in BATCH_CLASSES:
+        gs = [g for g, (n, _) in zip(graphs, params) if lo < n <= hi]
+        ng = odp.batch_load(odp.pack(gs))
+        ref = odp.batch_run(ng)
+        odp.batch_autotune(2)
+        odp.batch_run(ng)
+        odp.batch_run(ng)
+        tb = best(lambda: odp.batch_run(ng), 3)
+        la = odp.loop_run(ng, odp.PLAN_AUTO)
+        ta = best(lambda: odp.loop_run(ng, odp.PLAN_AUTO), 2)
+        assert all((x == y).all() for x, y in zip(la, ref))
+        odp.stats_enable(True, live=True)        # the loop's kernel time alone, from HIP events
+        odp.stats_reset()
+        odp.loop_run(ng, odp.PLAN_AUTO)
+        st = odp.stats()
+        odp.stats_enable(False)
+        odp.batch_release()
+        tb_sum += tb
+        ta_sum += ta
+        rows.append({"class": name, "graphs": len(gs), "batch_ms": round(1e3 * tb, 3),
+                     "loop_auto_ms": round(1e3 * ta, 3), "loop_auto_us_per_graph": round(1e6 * ta / len(gs), 1),
+                     "loop_kernel_us_per_graph": round(1e3 * st["resident_ms"] / len(gs), 1),
+                     "speedup_vs_loop_auto": round(ta / tb, 3)})
+        log(f"batch class {name}: {rows[-1]}")
+    print(json.dumps({"batch_classes": rows, "batch_ms": round(1e3 * tb_sum, 3),
+                      "loop_auto_ms": round(1e3 * ta_sum, 3), "speedup_vs_loop_auto": round(ta_sum / tb_sum, 3)}),
+          flush=True)
+
+
 def batch_workload(args, desc):
     """The batched path as the measured step (for profiling k_resident_w): one batch
     run of the 1024 graphs per step, with the tuned launch policy."""
@@ -448,6 +494,8 @@ def main():
                          "548/s is 60%% of the capacity measured with 2 small + 4 large slots, kept so that "
                          "latencies compare at one load")
     ap.add_argument("--served-only", action="store_true", help="run the served workload alone (no timed evaluation)")
+    ap.add_argument("--batch-classes", action="store_true",
+                    help="the batched workload split by graph size, each class against looping the auto plan")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic.json"))
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="first run one device-to-device copy of MIB MiB (a plain copy kernel of known "
@@ -465,6 +513,8 @@ def main():
         res, _ = served_extra(small_slots=sm, large_slots=lg, rate=args.served_rate or None)
         print(json.dumps({"served": res, "slots": [sm, lg]}), flush=True)
         return
+    if args.batch_classes:
+        return batch_classes()
     gkind, n, d, desc = WORKLOADS[args.workload]
     if gkind == "batch":
         return batch_workload(args, desc)

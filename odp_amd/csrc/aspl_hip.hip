@@ -194,6 +194,27 @@ static int res_walk_env(int n, int deg, int l)
   return std::min(res_walk_levels(n, deg, l, cost), res_walk_cap());
 }
 
+// Block width (32-bit words) of one graph's resident evaluation, up to wbmax:
+// the width with the least work on the busiest CU -- blocks per CU
+// (ceil(blocks / ncu): blocks that share a CU share its LDS and VALU issue)
+// times a block's cost at that width relative to one word (measured on single
+// graphs of n = 1024..16384, d = 6..24: two words 1.2, four 1.45, eight 2).
+// Ties go to the narrower block (a lone block is latency-bound).  Giving every
+// CU a block first had picked one-word blocks up to n = 16384: 1.6-1.8 x slower
+// than two-word blocks from n = 8192 on (two blocks per CU against one).
+static int res_width(int nsrc, int wbmax, int ncu)
+{
+  static const double rel[4] = {1.0, 1.2, 1.45, 2.0};
+  int wb = 1;
+  double best = 1e300;
+  for (int c = 1, i = 0; c <= wbmax && i < 4; c <<= 1, i++) {
+    const long blocks = (nsrc + 32L * c - 1) / (32L * c);
+    const double t = (double)((blocks + ncu - 1) / ncu) * rel[i];
+    if (t < best) { best = t; wb = c; }
+  }
+  return wb;
+}
+
 // cache policy of k_pgather's gathers (env ODP_HIP_PG_AUX: 0 default, 1 sc0, 16 sc1, 2 nt)
 static int pg_aux_env()
 {
@@ -728,10 +749,7 @@ struct Evaluator {
     int wbmax = resident_wb_max();
     if ((wbmax > 0 && !force_level) || (force_res && wbmax > 0)) {
       p.plan = ODPX_PLAN_RESIDENT;
-      // widest block that still gives every CU a block, else the narrowest
-      int wb = 1;
-      for (int c = wbmax; c >= 1; c >>= 1)
-        if ((nsrc + 32 * c - 1) / (32 * c) >= C.ncu) { wb = c; break; }
+      const int wb = res_width(nsrc, wbmax, C.ncu);
       p.wb = wb;
       // workgroup size: at most 2 rows per thread up to n = 1024 (measured at
       // n=1024, d=6: 25.3 us with 512 threads, 26.8 with 256, 28.1 with 1024;
@@ -2938,9 +2956,7 @@ extern "C" int ODPX_Set_aspl_noalloc(int nodes, int degree, const int* adjacency
   int wbmax = 0;
   for (int wb = RES_WB_MAX; wb >= 1 && !wbmax; wb >>= 1)
     if ((size_t)nodes * wb * 4 <= LDS_DYN_MAX && nodes <= res_rpt(wb) * 1024) wbmax = wb;
-  int wb = 1;
-  for (int c = wbmax; c >= 1; c >>= 1)
-    if ((nodes + 32 * c - 1) / (32 * c) >= C.ncu) { wb = c; break; }
+  const int wb = res_width(nodes, std::max(wbmax, 1), C.ncu);
   int threads = 1024;
   for (int th : {256, 512, 1024})
     if (nodes <= res_rpt(wb) * th && (nodes >= 4096 ? th == 1024 : (nodes > 512 ? th >= 512 : true))) {
