@@ -39,6 +39,8 @@ static const int RES_WB_MAX = 8;                // widest resident block: 256 so
 static const int BATCH_CLASSES = 5;             // size classes of the batch launch policy
 static const int FRONTIER_MIN_LEVELS = 16;      // frontier-mode levels when the last evaluation needed more
 static const u32 LIST_CAP = 1u << 22;           // unsaturated chunks a level may list (32 MiB)
+static const double LIST_MAX_FRAC = 1.0 / 8;     // ... listed only when at most this share was left last time
+static const int LV_MISS_LEVELS = 64;            // levels whose missing pairs an evaluation records
 // adjacency tables up to this size reach the device through the resident
 // plan's table build reading pinned host memory (one kernel instead of a DMA
 // command plus the kernel); larger ones are copied by DMA
@@ -221,6 +223,15 @@ static int pg_aux_env()
   const char* v = getenv("ODP_HIP_PG_AUX");
   const int a = v ? atoi(v) : 0;
   return (a == 1 || a == 16 || a == 2) ? a : 0;
+}
+
+// the share of unsaturated chunks up to which a count-only last level lists
+// them (env ODP_HIP_LIST_FRAC; a large value always lists)
+static double list_frac_env()
+{
+  const char* v = getenv("ODP_HIP_LIST_FRAC");
+  const double f = v ? atof(v) : LIST_MAX_FRAC;
+  return f >= 0 ? f : LIST_MAX_FRAC;
 }
 
 // capacity of the unsaturated-chunk list (env ODP_HIP_LIST_CAP lowers it; a
@@ -469,6 +480,12 @@ struct Evaluator {
   LevelRun lr;
   ResRun rr;
   int guess_levels = 0;                 // levels the last level-plan evaluation needed
+  // the count-only last level goes over a list of the chunks its level before
+  // left unsaturated (k_sparse) while the last evaluations left few: their share
+  // of the 16-byte chunks (bounded by the missing pairs over the chunks) at most
+  // list_frac_env(), or after a dense one at most half that (no flip-flop of the
+  // captured launch sequence between evaluations of similar graphs)
+  bool list_on = true;
   int batch_cap = MAX_BATCH_LEVELS;     // gather levels launched between two host reads (serving: few)
   hipStream_t st = nullptr;             // launch stream (the library's main stream unless set)
   bool timed() const { return g_stats_on && st == C.stream; }
@@ -543,7 +560,7 @@ struct Evaluator {
     bfs_q_cap = bfs_vis_cap = 0; bfs = false;
     lr = LevelRun(); rr = ResRun();
     adj_cap = nbr_cap = stage_cap = 0;
-    guess_levels = 0;
+    guess_levels = 0; list_on = true;
   }
 
   // shape of the graph; (re)allocates the graph buffers when they grow
@@ -1017,7 +1034,7 @@ struct Evaluator {
     plan = p;
     if (plan.plan == ODPX_PLAN_LEVEL) setup_level();
     else setup_resident();
-    guess_levels = 0;
+    guess_levels = 0; list_on = true;
   }
 
   // ---- evaluation of the adjacency on the device ----
@@ -1084,6 +1101,7 @@ struct Evaluator {
     std::vector<LvTask> sts;
     bool any_disc = false;
     int max_d = 1, lmax = 0;
+    double miss[LV_MISS_LEVELS] = {};   // pairs not reached after level L, over the tasks
   } lv;
 
   void lv_level0()
@@ -1135,7 +1153,8 @@ struct Evaluator {
       lv_copies(to);
       return;
     }
-    const bool listed = !lv.fr && lv.spec - 1 >= lv.L1 && r.ntasks < 65536 && n < (1 << 24) && r.kwp < (1 << 24);
+    const bool listed = !lv.fr && list_ok() && lv.spec - 1 >= lv.L1 && r.ntasks < 65536 && n < (1 << 24) &&
+                        r.kwp < (1 << 24);
     for (int L = from; L < to; L++) {
       int mode = LV_STORE | LV_COUNT;
       if (L == lv.spec) mode = listed ? LV_SPARSE : LV_COUNT;
@@ -1326,9 +1345,14 @@ struct Evaluator {
   // graph key of a first batch ending at be: its length, the kernel variant,
   // whether its last level is count-only, which tables the preparation builds
   // and, in tile mode, the longest edge k_tgeom and the tiles' window take
+  // listing many chunks serialises on the list's counter, and k_sparse's
+  // scattered chunks then cost more than a plain count-only level
+  bool list_ok() const { return list_on; }
+
   int lv_key(int be) const
   {
-    const int key = 4 * (64 * (16 * (4 * be + (lv.fr ? 1 : 0) + (spec_at(be) ? 2 : 0)) + lv.tk) + lv.tside) +
+    const int key = 4 * (64 * (16 * (8 * be + (lv.fr ? 1 : 0) + (spec_at(be) ? 2 : 0) + (list_ok() ? 4 : 0)) +
+                               lv.tk) + lv.tside) +
                     (lv.fresh ? 1 : 0) + (lv.plfresh ? 2 : 0);
     return 64 * key + (lv.tk ? std::min(std::max(geo_lmax, 0), 63) : 0);
   }
@@ -1368,6 +1392,7 @@ struct Evaluator {
     lv.sts.assign(r.ntasks, LvTask{1, 0, 0, 0});
     lv.any_disc = false;
     lv.max_d = 1; lv.lmax = 0;
+    for (double& m : lv.miss) m = 0;
     lv.first = 0;
     lv_group_start();
     // the preparation is enqueued: the table matches the adjacency, and the
@@ -1390,6 +1415,7 @@ struct Evaluator {
         if ((L >= 1 && c == q.prev) || L + 1 >= n) { q.running = 0; q.lwork = L; lv.any_disc = true; break; }
         q.sum += (target - c) * (u64)s;
         q.prev = c;
+        if (L < LV_MISS_LEVELS) lv.miss[L] += (double)(target - c);
       }
       if (!q.running) lv.running--;
     }
@@ -1410,6 +1436,11 @@ struct Evaluator {
     if (lv.any_disc) return true;          // the result is the sentinel whatever the other blocks give
     for (int t = lv.first; t < lv.first + lv.cnt_t; t++) lv.lmax = std::max(lv.lmax, lv.sts[t].lwork);
     guess_levels = lv.lmax + 1;
+    if (lv.first + r.group >= r.ntasks) {         // the evaluation's last group
+      const double frac = lv.lmax >= 1 && lv.lmax - 1 < LV_MISS_LEVELS
+                              ? lv.miss[lv.lmax - 1] / ((double)n * (r.kwp / 2) * r.ntasks) : 0;
+      list_on = frac <= (list_on ? 1.0 : 0.5) * list_frac_env();
+    }
     if (timed()) flush_level_stats(lv.sts.data(), lv.first, lv.cnt_t);
     lv.first += r.group;
     if (lv.first < r.ntasks) { lv_group_start(); return false; }

@@ -641,17 +641,21 @@ SPARSE_PLANS = [
 ]
 
 
+@pytest.mark.parametrize("frac", [None, "1000"])
 @pytest.mark.parametrize("cap", [None, "0", "5"])
 @pytest.mark.parametrize("plan", range(len(SPARSE_PLANS)))
-def test_sparse_last_level(plan, cap, monkeypatch):
+def test_sparse_last_level(plan, cap, frac, monkeypatch):
     """An evaluation that expects to end at the level the previous one ended on
     lists, at the level before, the 16-byte chunks left unsaturated, and counts
     the last level from those alone (k_sparse); when the list is too small
-    (ODP_HIP_LIST_CAP) it visits every chunk instead.  Repeated evaluations
-    (eager, captured, replayed) of a connected graph, a disconnected one and one
-    with absent edges all give the oracle's results."""
+    (ODP_HIP_LIST_CAP) it visits every chunk instead.  It lists only while the
+    last evaluation left few chunks unsaturated (ODP_HIP_LIST_FRAC=1000: always).
+    Repeated evaluations (eager, captured, replayed) of a connected graph, a
+    disconnected one and one with absent edges all give the oracle's results."""
     if cap is not None:
         monkeypatch.setenv("ODP_HIP_LIST_CAP", cap)
+    if frac is not None:
+        monkeypatch.setenv("ODP_HIP_LIST_FRAC", frac)
     g = odp.generate_regular(1500, 5, 17)
     half = odp.generate_regular(750, 5, 18)
     split = np.concatenate([half, half + 750]).astype(np.int32)
@@ -665,6 +669,28 @@ def test_sparse_last_level(plan, cap, monkeypatch):
                 assert same(odp.set_aspl(adj), want)
         finally:
             odp.finalize_aspl()
+
+
+@pytest.mark.parametrize("n,d", [(20000, 16), (24576, 8)])
+def test_dense_next_to_last_level(n, d):
+    """Graphs whose next-to-last level leaves most chunks unsaturated (n = 20000,
+    d = 16: diameter 5; n = 24576, d = 8: diameter 7): the first evaluation
+    lists them for the sparse last level, later ones count that level with a
+    plain count-only launch.  Every evaluation, and a 2-opt neighbour after it,
+    gives the oracle's results."""
+    adj = odp.generate_regular(n, d, 11 + n + d)
+    want = oracle.evaluate(adj, nodes=n)
+    moved = adj.copy()
+    odp.two_opt(moved, odp.Rng(5))
+    want_moved = oracle.evaluate(moved, nodes=n)
+    odp.init_aspl_general(n, d)
+    try:
+        for _ in range(4):
+            assert same(odp.set_aspl(adj), want)
+        for _ in range(2):
+            assert same(odp.set_aspl(moved), want_moved)
+    finally:
+        odp.finalize_aspl()
 
 
 def test_batch_policy_autotune_and_graph_replay():
