@@ -234,8 +234,9 @@ static double list_frac_env()
   return f >= 0 ? f : LIST_MAX_FRAC;
 }
 
-// capacity of the unsaturated-chunk list (env ODP_HIP_LIST_CAP lowers it; a
-// level that lists more chunks than that is counted by visiting every chunk)
+// capacity of the unsaturated-chunk list (env ODP_HIP_LIST_CAP lowers it; the
+// sub-lists share it evenly, and a level that lists more chunks than a
+// sub-list holds is counted by visiting every chunk)
 static u32 list_cap_env()
 {
   const char* v = getenv("ODP_HIP_LIST_CAP");
@@ -911,8 +912,8 @@ struct Evaluator {
     r.upw = plan.upw; r.seg_major = plan.seg_major; r.xcd = plan.xcd; r.urow = (lpr_log2 == 6);
     r.batch = plan.batch == 8 || plan.batch == 16 ? plan.batch : 4;
     r.per_task_wg = (units + 4 * r.upw - 1) / (4 * r.upw);
-    if (!r.list) { dalloc(&r.list, (size_t)LIST_CAP); dalloc(&r.list_n, 1); }
-    r.list_cap = list_cap_env();
+    if (!r.list) { dalloc(&r.list, (size_t)LIST_CAP); dalloc(&r.list_n, (size_t)LIST_SUBS * LIST_PAD); }
+    r.list_cap = (list_cap_env() + LIST_SUBS - 1) / LIST_SUBS;        // per sub-list
     if (plan.tile && tile_capable()) {
       const size_t cap = 3 + (size_t)group * ((width + TILE_MIN - 1) / TILE_MIN) * ((height + TILE_MIN - 1) / TILE_MIN);
       if (cap > r.tq_cap) { dfree(&r.tq); dalloc(&r.tq, cap); r.tq_cap = cap; }
@@ -1160,7 +1161,8 @@ struct Evaluator {
       if (L == lv.spec) mode = listed ? LV_SPARSE : LV_COUNT;
       else if (listed && L == lv.spec - 1) {
         mode |= LV_LIST;
-        hipLaunchKernelGGL(k_clear, dim3(1), dim3(64), 0, st, (u64*)nullptr, 0, (int*)r.list_n, 1);
+        hipLaunchKernelGGL(k_clear, dim3(LIST_SUBS * LIST_PAD / 256), dim3(256), 0, st, (u64*)nullptr, 0,
+                           (int*)r.list_n, LIST_SUBS * LIST_PAD);
         HIPC(hipGetLastError());
       }
       lv_launch(L, mode);
