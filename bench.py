@@ -394,6 +394,39 @@ def batch_classes():
           flush=True)
 
 
+SWEEP_NODES = (1024, 4096, 8192, 12000, 16384, 20000, 24576, 32768, 40000, 49152, 65536)
+SWEEP_DEGREES = (4, 8, 16, 32)
+
+
+def size_sweep(steps):
+    """The library's default plan on seeded random d-regular graphs over a grid
+    of sizes: the median time of `steps` evaluations of the adjacency resident
+    in HBM (ODPX_Set_aspl_resident), every result equal to the first."""
+    rows = []
+    for n in SWEEP_NODES:
+        for d in SWEEP_DEGREES:
+            adj = odp.generate_regular(n, d, 11 + n + d)
+            odp.init_aspl_general(n, d)
+            try:
+                ref = odp.set_aspl(adj)
+                odp.upload_adjacency(adj)
+                odp.set_aspl_resident()
+                odp.set_aspl_resident()
+                ts = []
+                for _ in range(steps):
+                    t = time.perf_counter()
+                    r = odp.set_aspl_resident()
+                    ts.append(time.perf_counter() - t)
+                    assert r == ref, (n, d, r, ref)
+                plan = odp.get_plan()
+            finally:
+                odp.finalize_aspl()
+            rows.append({"nodes": n, "degree": d, "diameter": ref[0], "us": round(1e6 * statistics.median(ts), 1),
+                         "plan": {k: v for k, v in plan.items() if v}})
+            log(f"sweep n={n} d={d}: {rows[-1]['us']} us")
+    print(json.dumps({"size_sweep": rows, "steps": steps}), flush=True)
+
+
 def batch_workload(args, desc):
     """The batched path as the measured step (for profiling k_resident_w): one batch
     run of the 1024 graphs per step, with the tuned launch policy."""
@@ -494,6 +527,8 @@ def main():
                          "548/s is 60%% of the capacity measured with 2 small + 4 large slots, kept so that "
                          "latencies compare at one load")
     ap.add_argument("--served-only", action="store_true", help="run the served workload alone (no timed evaluation)")
+    ap.add_argument("--size-sweep", action="store_true",
+                    help="the default plan's time per evaluation over a grid of graph sizes and degrees")
     ap.add_argument("--batch-classes", action="store_true",
                     help="the batched workload split by graph size, each class against looping the auto plan")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "traffic.json"))
@@ -515,6 +550,8 @@ def main():
         return
     if args.batch_classes:
         return batch_classes()
+    if args.size_sweep:
+        return size_sweep(args.steps)
     gkind, n, d, desc = WORKLOADS[args.workload]
     if gkind == "batch":
         return batch_workload(args, desc)
