@@ -59,7 +59,8 @@ static const int LVFIN_MAX = 4096;               // counters k_lvfin hands over 
 static const int RING = 260;                    // per-level counter slots kept on the device
 static const int MAX_BATCH_LEVELS = 256;        // levels launched between two host reads (< RING - 2)
 static const int PG_PARTS = 4;                  // row parts of the XCD-phased gather (2 MiB of a 65536-row slab)
-static const int PG_MIN_ROWS = 32768;           // ... used by default from this many rows (a 4 MiB slab)
+static const int PG_MIN_ROWS = 24576;           // ... used by default from this many rows (a 3 MiB slab),
+static const int PG_MIN_ROWS_NARROW = 20000;    // ... and from this many with tables of at most 16 ids
 
 // ODP_HIP_TRACE=1 prints each host step to stderr (diagnostics)
 static int g_trace = -1;
@@ -792,7 +793,11 @@ struct Evaluator {
     p.group = (kind == KIND_MATRIX) ? 0 : SAVING_GROUP;
     p.direct = -1;
     // slabs larger than an XCD's L2 (n x 128 B > 4 MB): the XCD-phased gather
-    p.parts = (p.kw == LEVEL_BLOCK_WORDS && n >= PG_MIN_ROWS) ? PG_PARTS : 0;
+    // (measured against k_level at n = 16386-28000, d = 8, 16, 32: from n = 24576 the
+    // part gather is 4-30 % faster at every degree but one (28000, d = 32: 2 % slower);
+    // at n = 20000 it is 11-22 % faster for d <= 16 and 5 % slower for d = 32)
+    p.parts = (p.kw == LEVEL_BLOCK_WORDS && (n >= PG_MIN_ROWS || (n >= PG_MIN_ROWS_NARROW && dpad <= 16)))
+                  ? PG_PARTS : 0;
     if (tile_capable()) {
       // grid graphs: 2-word blocks and temporal tiles (k_tile); MATRIX_SAVING
       // keeps the same number of source words resident as with 16-word blocks
