@@ -139,6 +139,8 @@ static void ctx_init()
   HIPC(hipFuncSetAttribute((const void*)k_resident_w<2, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_DYN_MAX));
   HIPC(hipFuncSetAttribute((const void*)k_resident_w<4, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_DYN_MAX));
   HIPC(hipFuncSetAttribute((const void*)k_resident_w<8, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_DYN_MAX));
+  HIPC(hipFuncSetAttribute((const void*)k_resident_w<2, false, RES_RPT_WIDE>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                           (int)LDS_DYN_MAX));
   HIPC(hipFuncSetAttribute((const void*)k_anneal<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_DYN_MAX));
   HIPC(hipFuncSetAttribute((const void*)k_anneal<2>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_DYN_MAX));
   HIPC(hipFuncSetAttribute((const void*)k_anneal<4>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_DYN_MAX));
@@ -197,6 +199,10 @@ static int res_walk_env(int n, int deg, int l)
   return std::min(res_walk_levels(n, deg, l, cost), res_walk_cap());
 }
 
+// rows per thread a single-graph resident launch may take at width wb (the
+// two-word instance with RES_RPT_WIDE rows runs past res_rpt's)
+static int res_rows(int wb) { return wb == 2 ? RES_RPT_WIDE : res_rpt(wb); }
+
 // Block width (32-bit words) of one graph's resident evaluation, up to wbmax:
 // the width with the least work on the busiest CU -- blocks per CU
 // (ceil(blocks / ncu): blocks that share a CU share its LDS and VALU issue)
@@ -205,12 +211,13 @@ static int res_walk_env(int n, int deg, int l)
 // Ties go to the narrower block (a lone block is latency-bound).  Giving every
 // CU a block first had picked one-word blocks up to n = 16384: 1.6-1.8 x slower
 // than two-word blocks from n = 8192 on (two blocks per CU against one).
-static int res_width(int nsrc, int wbmax, int ncu)
+static int res_width(int n, int nsrc, int wbmax, int ncu)
 {
   static const double rel[4] = {1.0, 1.2, 1.45, 2.0};
-  int wb = 1;
+  int wb = wbmax;
   double best = 1e300;
   for (int c = 1, i = 0; c <= wbmax && i < 4; c <<= 1, i++) {
+    if (n > res_rows(c) * 1024) continue;          // too many rows for this width's instance
     const long blocks = (nsrc + 32L * c - 1) / (32L * c);
     const double t = (double)((blocks + ncu - 1) / ncu) * rel[i];
     if (t < best) { best = t; wb = c; }
@@ -734,12 +741,12 @@ struct Evaluator {
 
   // ---- plans ----
   // widest block the resident kernel can run for this graph with `threads`
-  // threads: one LDS buffer of n*wb*4 bytes and n <= res_rpt(wb)*threads rows
+  // threads: one LDS buffer of n*wb*4 bytes and n <= res_rows(wb)*threads rows
   int resident_wb_max(int threads = 1024) const
   {
     if (n > 65535 || !d_nbr16) return 0;
     for (int wb = RES_WB_MAX; wb >= 1; wb >>= 1)
-      if ((size_t)n * wb * 4 <= LDS_DYN_MAX && n <= res_rpt(wb) * threads) return wb;
+      if ((size_t)n * wb * 4 <= LDS_DYN_MAX && n <= res_rows(wb) * threads) return wb;
     return 0;
   }
 
@@ -768,14 +775,14 @@ struct Evaluator {
     int wbmax = resident_wb_max();
     if ((wbmax > 0 && !force_level) || (force_res && wbmax > 0)) {
       p.plan = ODPX_PLAN_RESIDENT;
-      const int wb = res_width(nsrc, wbmax, C.ncu);
+      const int wb = res_width(n, nsrc, wbmax, C.ncu);
       p.wb = wb;
       // workgroup size: at most 2 rows per thread up to n = 1024 (measured at
       // n=1024, d=6: 25.3 us with 512 threads, 26.8 with 256, 28.1 with 1024;
       // n=2000, d=12: 29.1 / 33.9 / 30.0), all 1024 threads from n = 4096
       p.threads = 1024;
       for (int th : {256, 512, 1024})
-        if (n <= res_rpt(wb) * th && (n >= 4096 ? th == 1024 : (n > 512 ? th >= 512 : true))) { p.threads = th; break; }
+        if (n <= res_rows(wb) * th && (n >= 4096 ? th == 1024 : (n > 512 ? th >= 512 : true))) { p.threads = th; break; }
       return p;
     }
     // Column blocks of 16 words (1024 sources), each stored as its own
@@ -814,7 +821,7 @@ struct Evaluator {
     if (p.plan == ODPX_PLAN_RESIDENT) {
       if (p.wb != 1 && p.wb != 2 && p.wb != 4 && p.wb != 8) return false;
       if (p.threads != 256 && p.threads != 512 && p.threads != 1024) return false;
-      return n <= 65535 && d_nbr16 && (size_t)n * p.wb * 4 <= LDS_DYN_MAX && n <= res_rpt(p.wb) * p.threads;
+      return n <= 65535 && d_nbr16 && (size_t)n * p.wb * 4 <= LDS_DYN_MAX && n <= res_rows(p.wb) * p.threads;
     }
     if (p.plan == ODPX_PLAN_LEVEL) {
       if (!(p.kw >= 0 && p.upw >= 1 && p.upw <= 64 && p.group >= 0)) return false;
@@ -1546,7 +1553,9 @@ struct Evaluator {
       }
       TimedLaunch tl;
       timed_begin(tl, T_RESIDENT, 0, st);
-      auto kern = r.wb == 1 ? k_resident_w<1> : r.wb == 2 ? k_resident_w<2> : r.wb == 4 ? k_resident_w<4> : k_resident_w<8>;
+      auto kern = r.wb == 1 ? k_resident_w<1>
+                : r.wb == 2 ? (n > res_rpt(2) * r.threads ? k_resident_w<2, false, RES_RPT_WIDE> : k_resident_w<2>)
+                : r.wb == 4 ? k_resident_w<4> : k_resident_w<8>;
       ResFin fin;
       fin.done = r.done; fin.part = r.part; fin.host = r.host_dev; fin.err = d_err;
       fin.patch = make_patch();
@@ -2994,7 +3003,7 @@ extern "C" int ODPX_Set_aspl_noalloc(int nodes, int degree, const int* adjacency
   int wbmax = 0;
   for (int wb = RES_WB_MAX; wb >= 1 && !wbmax; wb >>= 1)
     if ((size_t)nodes * wb * 4 <= LDS_DYN_MAX && nodes <= res_rpt(wb) * 1024) wbmax = wb;
-  const int wb = res_width(nodes, std::max(wbmax, 1), C.ncu);
+  const int wb = res_width(nodes, nodes, std::max(wbmax, 1), C.ncu);
   int threads = 1024;
   for (int th : {256, 512, 1024})
     if (nodes <= res_rpt(wb) * th && (nodes >= 4096 ? th == 1024 : (nodes > 512 ? th >= 512 : true))) {

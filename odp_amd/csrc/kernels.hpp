@@ -1562,6 +1562,10 @@ struct ResOut {     // per-graph accumulators
 // many as LDS holds: graphs of 8192 < n <= 10208 take 4-word blocks, 23.5 ms
 // against 24.6 for the 1024-graph batch)
 __host__ __device__ constexpr int res_rpt(int wb) { return wb == 1 ? 16 : wb == 2 ? 16 : wb == 4 ? 10 : 2; }
+// rows per thread of the single-graph instance for two-word blocks past 16384
+// rows (their LDS buffer holds up to 20416): its own instantiation, so that the
+// other launches keep their register use
+#define RES_RPT_WIDE 20
 
 template <int WB> struct RowT { u32 w[WB]; };
 
@@ -1681,11 +1685,10 @@ __device__ __forceinline__ u64 block_total(u64 v, u64* part, int wave, int nw)
 // (the same value in every thread).
 // MAD: pulled rows address their neighbours with row_or2 (k_anneal keeps
 // row_or: its larger register set spills more with it)
-template <int WB, bool STOP = false, bool WALK = true, bool MAD = true>
+template <int WB, bool STOP = false, bool WALK = true, bool MAD = true, int RPT = res_rpt(WB)>
 __device__ __forceinline__ ResOut res_run(const ResTask& T, u32* cur, u64 (*s_part)[16],
                                           int tid, int nth, int lane, int wave, int nw)
 {
-  constexpr int RPT = res_rpt(WB);
   const u64 target = (u64)T.n * (u64)T.l;
   const u32 cbase = (u32)(uintptr_t)(__attribute__((address_space(3))) u32*)cur;   // cur's LDS byte address
   RowT<WB> full;
@@ -2007,7 +2010,7 @@ __device__ __forceinline__ void res_finish(const ResFin& fin, int* qhead, int la
 // width spill), and a launch can size its LDS to its own blocks.  STOP: a pulled
 // row stops reading neighbours once full (measured: the 1024-graph batch 25.05
 // -> 24.6 ms; one n=1024 graph 39.6k -> 36.0k evaluations/s, so only batches use it).
-template <int WB, bool STOP = false>
+template <int WB, bool STOP = false, int RPT = res_rpt(WB)>
 __global__ __launch_bounds__(1024) void k_resident_w(const ResTask* __restrict__ tasks, int ntasks,
                                                      int* __restrict__ qhead, ResOut* __restrict__ out, ResFin fin)
 {
@@ -2029,7 +2032,7 @@ __global__ __launch_bounds__(1024) void k_resident_w(const ResTask* __restrict__
   // from *qhead (which then counts past gridDim.x)
   int t = blockIdx.x;
   while (t < ntasks) {
-    const ResOut b = res_run<WB, STOP>(tasks[t], lds, s_part, tid, nth, lane, wave, nw);
+    const ResOut b = res_run<WB, STOP, true, true, RPT>(tasks[t], lds, s_part, tid, nth, lane, wave, nw);
     if (fin.done) {
       mine.sum += b.sum; mine.diam = max(mine.diam, b.diam); mine.flags |= b.flags; mine.levels += b.levels;
     } else {
