@@ -779,10 +779,12 @@ struct Evaluator {
       p.wb = wb;
       // workgroup size: at most 2 rows per thread up to n = 1024 (measured at
       // n=1024, d=6: 25.3 us with 512 threads, 26.8 with 256, 28.1 with 1024;
-      // n=2000, d=12: 29.1 / 33.9 / 30.0), all 1024 threads from n = 4096
+      // n=2000, d=12: 29.1 / 33.9 / 30.0), 512 threads up to n = 6144 (n = 4096,
+      // d = 6, 12, 24: 31.9 / 29.6 / 31.2 us against 33.5 / 30.6 / 32.3 with
+      // 1024), all 1024 threads above (n = 8192: 1-3 % faster than 512)
       p.threads = 1024;
       for (int th : {256, 512, 1024})
-        if (n <= res_rows(wb) * th && (n >= 4096 ? th == 1024 : (n > 512 ? th >= 512 : true))) { p.threads = th; break; }
+        if (n <= res_rows(wb) * th && (n > 6144 ? th == 1024 : (n > 512 ? th >= 512 : true))) { p.threads = th; break; }
       return p;
     }
     // Column blocks of 16 words (1024 sources), each stored as its own
