@@ -139,6 +139,8 @@ static void ctx_init()
   HIPC(hipFuncSetAttribute((const void*)k_resident_w<2, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_DYN_MAX));
   HIPC(hipFuncSetAttribute((const void*)k_resident_w<4, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_DYN_MAX));
   HIPC(hipFuncSetAttribute((const void*)k_resident_w<8, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_DYN_MAX));
+  HIPC(hipFuncSetAttribute((const void*)k_resident_w<1, false, RES_RPT_WIDE1>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                           (int)LDS_DYN_MAX));
   HIPC(hipFuncSetAttribute((const void*)k_resident_w<2, false, RES_RPT_WIDE>, hipFuncAttributeMaxDynamicSharedMemorySize,
                            (int)LDS_DYN_MAX));
   HIPC(hipFuncSetAttribute((const void*)k_anneal<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_DYN_MAX));
@@ -200,8 +202,8 @@ static int res_walk_env(int n, int deg, int l)
 }
 
 // rows per thread a single-graph resident launch may take at width wb (the
-// two-word instance with RES_RPT_WIDE rows runs past res_rpt's)
-static int res_rows(int wb) { return wb == 2 ? RES_RPT_WIDE : res_rpt(wb); }
+// one- and two-word instances with RES_RPT_WIDE1 / RES_RPT_WIDE rows run past res_rpt's)
+static int res_rows(int wb) { return wb == 1 ? RES_RPT_WIDE1 : wb == 2 ? RES_RPT_WIDE : res_rpt(wb); }
 
 // Block width (32-bit words) of one graph's resident evaluation, up to wbmax:
 // the width with the least work on the busiest CU -- blocks per CU
@@ -1555,7 +1557,7 @@ struct Evaluator {
       }
       TimedLaunch tl;
       timed_begin(tl, T_RESIDENT, 0, st);
-      auto kern = r.wb == 1 ? k_resident_w<1>
+      auto kern = r.wb == 1 ? (n > res_rpt(1) * r.threads ? k_resident_w<1, false, RES_RPT_WIDE1> : k_resident_w<1>)
                 : r.wb == 2 ? (n > res_rpt(2) * r.threads ? k_resident_w<2, false, RES_RPT_WIDE> : k_resident_w<2>)
                 : r.wb == 4 ? k_resident_w<4> : k_resident_w<8>;
       ResFin fin;

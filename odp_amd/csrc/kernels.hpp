@@ -1562,10 +1562,12 @@ struct ResOut {     // per-graph accumulators
 // many as LDS holds: graphs of 8192 < n <= 10208 take 4-word blocks, 23.5 ms
 // against 24.6 for the 1024-graph batch)
 __host__ __device__ constexpr int res_rpt(int wb) { return wb == 1 ? 16 : wb == 2 ? 16 : wb == 4 ? 10 : 2; }
-// rows per thread of the single-graph instance for two-word blocks past 16384
-// rows (their LDS buffer holds up to 20416): its own instantiation, so that the
+// rows per thread of the single-graph instances past 16384 rows, for two-word
+// blocks (their LDS buffer holds up to 20416 rows) and one-word blocks (28 rows
+// a thread: 28672 rows; 32 spilled): instantiations of their own, so that the
 // other launches keep their register use
 #define RES_RPT_WIDE 20
+#define RES_RPT_WIDE1 28
 
 template <int WB> struct RowT { u32 w[WB]; };
 

@@ -53,7 +53,7 @@ def plan_fits(plan, n):
     if plan is None or plan["plan"] != RES:
         return True
     wb, th = plan["wb"], plan["threads"]
-    rpt = {1: 16, 2: 20, 4: 10, 8: 2}[wb]
+    rpt = {1: 28, 2: 20, 4: 10, 8: 2}[wb]
     return n <= 65535 and n * wb * 4 <= LDS_DYN_MAX and n <= rpt * th
 
 
@@ -671,22 +671,24 @@ def test_sparse_last_level(plan, cap, frac, monkeypatch):
             odp.finalize_aspl()
 
 
-@pytest.mark.parametrize("n,d", [(16386, 6), (18000, 12), (20416, 4)])
+@pytest.mark.parametrize("n,d", [(16386, 6), (18000, 12), (20416, 4), (20418, 8), (24576, 16), (28672, 4)])
 def test_resident_past_16384_rows(n, d):
-    """Single graphs of 16384 < n <= 20416 rows keep the resident plan (two-word
-    blocks, up to 20 rows a thread in their own kernel instance): the default
+    """Single graphs of 16384 < n <= 28672 rows keep the resident plan, in kernel
+    instances of their own: two-word blocks with up to 20 rows a thread up to
+    n = 20416 (what LDS holds), one-word blocks with up to 28 above.  The default
     plan and an explicit one match the oracle, also after a device swap."""
     adj = odp.generate_regular(n, d, 5 + n + d)
     want = oracle.evaluate(adj, nodes=n)
     moved = adj.copy()
     odp.two_opt(moved, odp.Rng(9))
     want_moved = oracle.evaluate(moved, nodes=n)
-    for plan in (None, dict(plan=RES, wb=2, threads=1024)):
+    wb = 2 if n <= 20416 else 1
+    for plan in (None, dict(plan=RES, wb=wb, threads=1024)):
         odp.init_aspl_general(n, d)
         try:
             if plan is not None:
                 odp.set_plan(**plan)
-            assert odp.get_plan()["plan"] == RES and odp.get_plan()["wb"] == 2
+            assert odp.get_plan()["plan"] == RES and odp.get_plan()["wb"] == wb
             for _ in range(3):
                 assert same(odp.set_aspl(adj), want)
             assert same(odp.set_aspl(moved), want_moved)
