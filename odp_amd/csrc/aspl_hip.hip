@@ -141,6 +141,10 @@ static void ctx_init()
   HIPC(hipFuncSetAttribute((const void*)k_resident_w<8, true>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_DYN_MAX));
   HIPC(hipFuncSetAttribute((const void*)k_resident_w<1, false, RES_RPT_WIDE1>, hipFuncAttributeMaxDynamicSharedMemorySize,
                            (int)LDS_DYN_MAX));
+  HIPC(hipFuncSetAttribute((const void*)k_resident_w<1, true, RES_RPT_WIDE1>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                           (int)LDS_DYN_MAX));
+  HIPC(hipFuncSetAttribute((const void*)k_resident_w<2, true, RES_RPT_WIDE>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                           (int)LDS_DYN_MAX));
   HIPC(hipFuncSetAttribute((const void*)k_resident_w<2, false, RES_RPT_WIDE>, hipFuncAttributeMaxDynamicSharedMemorySize,
                            (int)LDS_DYN_MAX));
   HIPC(hipFuncSetAttribute((const void*)k_anneal<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)LDS_DYN_MAX));
@@ -1557,9 +1561,15 @@ struct Evaluator {
       }
       TimedLaunch tl;
       timed_begin(tl, T_RESIDENT, 0, st);
-      auto kern = r.wb == 1 ? (n > res_rpt(1) * r.threads ? k_resident_w<1, false, RES_RPT_WIDE1> : k_resident_w<1>)
-                : r.wb == 2 ? (n > res_rpt(2) * r.threads ? k_resident_w<2, false, RES_RPT_WIDE> : k_resident_w<2>)
-                : r.wb == 4 ? k_resident_w<4> : k_resident_w<8>;
+      const bool stop = degree > 8;
+      auto kern = r.wb == 1 ? (n > res_rpt(1) * r.threads
+                                   ? (stop ? k_resident_w<1, true, RES_RPT_WIDE1> : k_resident_w<1, false, RES_RPT_WIDE1>)
+                                   : stop ? k_resident_w<1, true> : k_resident_w<1>)
+                : r.wb == 2 ? (n > res_rpt(2) * r.threads
+                                   ? (stop ? k_resident_w<2, true, RES_RPT_WIDE> : k_resident_w<2, false, RES_RPT_WIDE>)
+                                   : stop ? k_resident_w<2, true> : k_resident_w<2>)
+                : r.wb == 4 ? (stop ? k_resident_w<4, true> : k_resident_w<4>)
+                : (stop ? k_resident_w<8, true> : k_resident_w<8>);
       ResFin fin;
       fin.done = r.done; fin.part = r.part; fin.host = r.host_dev; fin.err = d_err;
       fin.patch = make_patch();
