@@ -46,7 +46,7 @@ typedef struct {
   int plan;       /* ODPX_PLAN_RESIDENT or ODPX_PLAN_LEVEL                          */
   int wb;         /* resident: 32-bit words per row of a block (1, 2, 4 or 8); the
                      block's n x wb words must fit LDS                                 */
-  int threads;    /* resident: threads per workgroup (256, 512 or 1024); at most 32
+  int threads;    /* resident: threads per workgroup (256, 512 or 1024); at most 40
                      (wb 1), 20 (wb 2), 10 (wb 4) or 2 (wb 8) rows a thread          */
   int kw;         /* level: 64-bit words per row of a column block; 0 = whole row    */
   int upw;        /* level: row segments per wave per launch                         */

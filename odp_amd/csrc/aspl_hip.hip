@@ -147,6 +147,10 @@ static void ctx_init()
                            (int)LDS_DYN_MAX));
   HIPC(hipFuncSetAttribute((const void*)k_resident_w<1, true, RES_RPT_WIDE1B>, hipFuncAttributeMaxDynamicSharedMemorySize,
                            (int)LDS_DYN_MAX));
+  HIPC(hipFuncSetAttribute((const void*)k_resident_w<1, false, RES_RPT_WIDE1C>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                           (int)LDS_DYN_MAX));
+  HIPC(hipFuncSetAttribute((const void*)k_resident_w<1, true, RES_RPT_WIDE1C>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                           (int)LDS_DYN_MAX));
   HIPC(hipFuncSetAttribute((const void*)k_resident_w<2, true, RES_RPT_WIDE>, hipFuncAttributeMaxDynamicSharedMemorySize,
                            (int)LDS_DYN_MAX));
   HIPC(hipFuncSetAttribute((const void*)k_resident_w<2, false, RES_RPT_WIDE>, hipFuncAttributeMaxDynamicSharedMemorySize,
@@ -210,8 +214,8 @@ static int res_walk_env(int n, int deg, int l)
 }
 
 // rows per thread a single-graph resident launch may take at width wb (the
-// one- and two-word instances with RES_RPT_WIDE1(B) / RES_RPT_WIDE rows run past res_rpt's)
-static int res_rows(int wb) { return wb == 1 ? RES_RPT_WIDE1B : wb == 2 ? RES_RPT_WIDE : res_rpt(wb); }
+// one- and two-word instances with RES_RPT_WIDE1(B, C) / RES_RPT_WIDE rows run past res_rpt's)
+static int res_rows(int wb) { return wb == 1 ? RES_RPT_WIDE1C : wb == 2 ? RES_RPT_WIDE : res_rpt(wb); }
 
 // Block width (32-bit words) of one graph's resident evaluation, up to wbmax:
 // the width with the least work on the busiest CU -- blocks per CU
@@ -1566,7 +1570,9 @@ struct Evaluator {
       TimedLaunch tl;
       timed_begin(tl, T_RESIDENT, 0, st);
       const bool stop = degree > 8;
-      auto kern = r.wb == 1 ? (n > RES_RPT_WIDE1 * r.threads
+      auto kern = r.wb == 1 ? (n > RES_RPT_WIDE1B * r.threads
+                                   ? (stop ? k_resident_w<1, true, RES_RPT_WIDE1C> : k_resident_w<1, false, RES_RPT_WIDE1C>)
+                               : n > RES_RPT_WIDE1 * r.threads
                                    ? (stop ? k_resident_w<1, true, RES_RPT_WIDE1B> : k_resident_w<1, false, RES_RPT_WIDE1B>)
                                : n > res_rpt(1) * r.threads
                                    ? (stop ? k_resident_w<1, true, RES_RPT_WIDE1> : k_resident_w<1, false, RES_RPT_WIDE1>)

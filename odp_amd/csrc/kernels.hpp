@@ -1563,13 +1563,14 @@ struct ResOut {     // per-graph accumulators
 // against 24.6 for the 1024-graph batch)
 __host__ __device__ constexpr int res_rpt(int wb) { return wb == 1 ? 16 : wb == 2 ? 16 : wb == 4 ? 10 : 2; }
 // rows per thread of the single-graph instances past 16384 rows, for two-word
-// blocks (their LDS buffer holds up to 20416 rows) and one-word blocks (28 rows
-// a thread without spills, 32 with 52 bytes a lane spilled, still faster than
-// the level plan up to 32768 rows): instantiations of their own, so that the
-// other launches keep their register use
+// blocks (their LDS buffer holds up to 20416 rows) and one-word blocks (up to
+// 40832 rows: 28 rows a thread without spills, 32 and 40 with 52 and 196 bytes
+// a lane spilled, still as fast as the level plan or faster at every degree):
+// instantiations of their own, so that the other launches keep their registers
 #define RES_RPT_WIDE 20
 #define RES_RPT_WIDE1 28
 #define RES_RPT_WIDE1B 32
+#define RES_RPT_WIDE1C 40
 
 template <int WB> struct RowT { u32 w[WB]; };
 

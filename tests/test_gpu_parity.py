@@ -53,7 +53,7 @@ def plan_fits(plan, n):
     if plan is None or plan["plan"] != RES:
         return True
     wb, th = plan["wb"], plan["threads"]
-    rpt = {1: 32, 2: 20, 4: 10, 8: 2}[wb]
+    rpt = {1: 40, 2: 20, 4: 10, 8: 2}[wb]
     return n <= 65535 and n * wb * 4 <= LDS_DYN_MAX and n <= rpt * th
 
 
@@ -672,11 +672,11 @@ def test_sparse_last_level(plan, cap, frac, monkeypatch):
 
 
 @pytest.mark.parametrize("n,d", [(16386, 6), (18000, 12), (20416, 4), (20418, 8), (24576, 16), (28672, 4),
-                                 (30000, 12), (32768, 6)])
+                                 (30000, 12), (32768, 6), (40832, 8)])
 def test_resident_past_16384_rows(n, d):
-    """Single graphs of 16384 < n <= 32768 rows keep the resident plan, in kernel
+    """Single graphs of 16384 < n <= 40832 rows keep the resident plan, in kernel
     instances of their own: two-word blocks with up to 20 rows a thread up to
-    n = 20416 (what LDS holds), one-word blocks with up to 28 or 32 above.  The default
+    n = 20416 (what LDS holds), one-word blocks with up to 28, 32 or 40 above.  The default
     plan and an explicit one match the oracle, also after a device swap."""
     adj = odp.generate_regular(n, d, 5 + n + d)
     want = oracle.evaluate(adj, nodes=n)
