@@ -204,6 +204,22 @@ def test_symmetric_generated_vs_expanded():
                 assert same(got, want), (w, s, plan)
 
 
+def test_symmetric_resident_past_16384_rows():
+    """Symmetric graphs past 16384 rows under the default plan, now resident in
+    the wide kernel instances: a 4-fold 144x144 grid (20736 cells) and a 4-fold
+    general graph of 36000 vertices, against the oracle on the expanded graphs."""
+    w = 144
+    full = odp.generate_grid(w, w, 8, 4, 4, 11)
+    base = odp.grid_base_table(w, w, 8, 4, full)
+    got = hip_eval(base, width=w, height=w, symmetries=4)
+    assert same(got, oracle.evaluate(full, width=w, height=w))
+    n = 36000
+    base = odp.generate_general_s(n, 8, 4, 12)
+    full = oracle.expand(base, nodes=n, symmetries=4)
+    got = hip_eval(base, nodes=n, symmetries=4)
+    assert same(got, oracle.evaluate(full, nodes=n))
+
+
 def test_nonregular_symmetric_num_degrees():
     # sample/graph/grid/w10h10d4r4g4 with a vertex-orbit trimmed through num_degrees
     w, h, d, _, base = grid_fixture("w10h10d4r4g4", 4)
